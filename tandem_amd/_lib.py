@@ -1,0 +1,92 @@
+"""Loader of the native library libtandem_dg.so (the C ABI declared in include/tandem_dg.h).
+
+The library is built in-tree by tandem_amd/build.py; there is no fallback implementation: if the
+library is missing this module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libtandem_dg.so")
+
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_ip32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class TdgError(RuntimeError):
+    pass
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("degree", C.c_int),
+        ("num_vertices", C.c_int64),
+        ("vertices", C.c_void_p),
+        ("num_elements", C.c_int64),
+        ("elements", C.c_void_p),
+        ("num_boundary_facets", C.c_int64),
+        ("boundary_facets", C.c_void_p),
+        ("boundary_bc", C.c_void_p),
+        ("geometry_nodes", C.c_void_p),
+    ]
+
+
+EXPORTS = [
+    "tdg_create", "tdg_destroy", "tdg_last_error", "tdg_sizes", "tdg_volume_points", "tdg_facet_points",
+    "tdg_facets", "tdg_set_material", "tdg_upload", "tdg_mult", "tdg_mult_f32", "tdg_mult_batched", "tdg_mult_batched_f32",
+    "tdg_set_stream", "tdg_rhs", "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
+]
+
+
+def lib():
+    """The loaded library (raises TdgError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TdgError("native library %s is missing: run `python -m tandem_amd.build` first" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.tdg_last_error.restype = C.c_char_p
+    L.tdg_create.argtypes = [C.POINTER(MeshDesc), C.c_int, C.POINTER(vp)]
+    L.tdg_destroy.argtypes = [vp]
+    L.tdg_sizes.argtypes = [vp, _ip64]
+    L.tdg_volume_points.argtypes = [vp, _dp]
+    L.tdg_facet_points.argtypes = [vp, _dp]
+    L.tdg_facets.argtypes = [vp, _ip64]
+    L.tdg_set_material.argtypes = [vp, vp, vp]
+    L.tdg_upload.argtypes = [vp]
+    L.tdg_mult.argtypes = [vp, vp, vp]
+    L.tdg_mult_f32.argtypes = [vp, vp, vp]
+    L.tdg_mult_batched.argtypes = [vp, vp, vp, C.c_int, C.c_int64]
+    L.tdg_mult_batched_f32.argtypes = [vp, vp, vp, C.c_int, C.c_int64]
+    L.tdg_set_stream.argtypes = [vp, vp]
+    L.tdg_rhs.argtypes = [vp, vp, vp, _dp]
+    L.tdg_apply_counts.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.tdg_host_array.argtypes = [vp, C.c_char_p, vp, C.POINTER(C.c_int64)]
+    L.tdg_box_mesh.argtypes = [C.c_int, _ip64, C.c_int, _ip32, _ip64, vp, vp, vp, vp]
+    L.tdg_reference_nodes.argtypes = [C.c_int, C.c_int, _dp]
+    for name in EXPORTS:
+        if getattr(L, name, None) is None:
+            raise TdgError("symbol %s missing from %s" % (name, LIB_PATH))
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise TdgError("tandem_dg error %d: %s" % (rc, lib().tdg_last_error().decode()))
+    return rc
+
+
+def ptr(a):
+    """ctypes pointer of a numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    return C.c_void_p(a.ctypes.data)

@@ -1,0 +1,247 @@
+// extern "C" boundary of the MI355X DG operator (declared in include/tandem_dg.h).
+#include "../../include/tandem_dg.h"
+
+#include "host/mesh.h"
+#include "host/operator.h"
+#include "host/refelem.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+struct tdg_op {
+    std::unique_ptr<tdg::Operator> impl;
+};
+
+namespace {
+thread_local std::string g_error;
+
+int fail(int code, std::string const& msg) {
+    g_error = msg;
+    return code;
+}
+
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return TDG_OK;
+    } catch (std::invalid_argument const& e) {
+        return fail(TDG_ERR_ARG, e.what());
+    } catch (std::logic_error const& e) {
+        return fail(TDG_ERR_STATE, e.what());
+    } catch (std::exception const& e) {
+        std::string m = e.what();
+        return fail(m.rfind("HIP error", 0) == 0 ? TDG_ERR_HIP : TDG_ERR_INTERNAL, m);
+    } catch (...) {
+        return fail(TDG_ERR_INTERNAL, "unknown exception");
+    }
+}
+
+void require(bool cond, char const* msg) {
+    if (!cond) throw std::invalid_argument(msg);
+}
+tdg::Operator& ready(tdg_op* op) {
+    require(op && op->impl, "null operator");
+    if (!op->impl->ready()) throw std::logic_error("tdg_set_material() has not been called");
+    return *op->impl;
+}
+} // namespace
+
+extern "C" {
+
+char const* tdg_last_error(void) { return g_error.c_str(); }
+
+int tdg_create(tdg_mesh_desc const* m, int kind, tdg_op** op) {
+    return guarded([&] {
+        require(m && op, "null argument");
+        require(m->dim == 2 || m->dim == 3, "dim must be 2 or 3");
+        require(m->num_vertices > 0 && m->vertices, "no vertices");
+        require(m->num_elements > 0 && m->elements, "no elements");
+        require(m->num_boundary_facets == 0 || (m->boundary_facets && m->boundary_bc), "boundary arrays missing");
+        tdg::SimplexMesh mesh;
+        mesh.dim = m->dim;
+        mesh.vertices.assign(m->vertices, m->vertices + m->num_vertices * m->dim);
+        mesh.elements.assign(m->elements, m->elements + m->num_elements * (m->dim + 1));
+        for (int64_t e = 0; e < m->num_elements; ++e)
+            std::sort(mesh.elements.begin() + e * (m->dim + 1), mesh.elements.begin() + (e + 1) * (m->dim + 1));
+        mesh.bnd_facets.assign(m->boundary_facets, m->boundary_facets + m->num_boundary_facets * m->dim);
+        mesh.bnd_bc.assign(m->boundary_bc, m->boundary_bc + m->num_boundary_facets);
+        for (int b : mesh.bnd_bc)
+            require(b == TDG_BC_NONE || b == TDG_BC_NATURAL || b == TDG_BC_FAULT || b == TDG_BC_DIRICHLET,
+                    "unknown boundary condition");
+        std::vector<double> nodes;
+        if (m->geometry_nodes) {
+            std::size_t n = m->num_elements * tdg::num_basis(m->dim, m->degree) * m->dim;
+            nodes.assign(m->geometry_nodes, m->geometry_nodes + n);
+        }
+        auto h = std::make_unique<tdg_op>();
+        h->impl = std::make_unique<tdg::Operator>(std::move(mesh), kind, m->degree,
+                                                  m->geometry_nodes ? &nodes : nullptr);
+        *op = h.release();
+    });
+}
+
+int tdg_destroy(tdg_op* op) {
+    delete op;
+    return TDG_OK;
+}
+
+int tdg_sizes(tdg_op const* op, int64_t* out) {
+    return guarded([&] {
+        require(op && op->impl && out, "null argument");
+        auto const& o = *op->impl;
+        out[0] = int64_t(o.block_size());
+        out[1] = int64_t(o.num_elements());
+        out[2] = int64_t(o.local_size());
+        out[3] = int64_t(o.num_facets());
+        out[4] = o.nq_vol();
+        out[5] = o.nq_fct();
+        out[6] = o.dim();
+    });
+}
+
+int tdg_volume_points(tdg_op const* op, double* coords) {
+    return guarded([&] {
+        require(op && op->impl && coords, "null argument");
+        auto const& v = op->impl->volume_points();
+        std::memcpy(coords, v.data(), v.size() * sizeof(double));
+    });
+}
+
+int tdg_facet_points(tdg_op const* op, double* coords) {
+    return guarded([&] {
+        require(op && op->impl && coords, "null argument");
+        auto const& v = op->impl->facet_points();
+        std::memcpy(coords, v.data(), v.size() * sizeof(double));
+    });
+}
+
+int tdg_facets(tdg_op const* op, int64_t* out) {
+    return guarded([&] {
+        require(op && op->impl && out, "null argument");
+        auto const& t = op->impl->topology();
+        for (std::size_t i = 0; i < t.facets.size(); ++i) {
+            auto const& f = t.facets[i];
+            out[5 * i + 0] = f.elem[0];
+            out[5 * i + 1] = f.elem[1];
+            out[5 * i + 2] = f.local[0];
+            out[5 * i + 3] = f.local[1];
+            out[5 * i + 4] = f.bc;
+        }
+    });
+}
+
+int tdg_set_material(tdg_op* op, double const* c0_Q, double const* c1_Q) {
+    return guarded([&] {
+        require(op && op->impl, "null operator");
+        op->impl->set_material(c0_Q, c1_Q);
+    });
+}
+
+int tdg_upload(tdg_op* op) {
+    return guarded([&] { ready(op).upload_device(); });
+}
+
+int tdg_mult(tdg_op* op, double const* x, double* y) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(x && y && x != y, "x and y must be distinct device buffers");
+        o.apply(x, y);
+    });
+}
+
+int tdg_mult_f32(tdg_op* op, float const* x, float* y) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(x && y && x != y, "x and y must be distinct device buffers");
+        o.apply(x, y);
+    });
+}
+
+int tdg_mult_batched(tdg_op* op, double const* X, double* Y, int ncols, int64_t ld) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(X && Y && X != Y && ncols > 0 && ncols <= 65535 && ld >= int64_t(o.local_size()),
+                "bad batched arguments");
+        o.apply_batched(X, Y, ncols, ld);
+    });
+}
+
+int tdg_mult_batched_f32(tdg_op* op, float const* X, float* Y, int ncols, int64_t ld) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(X && Y && X != Y && ncols > 0 && ncols <= 65535 && ld >= int64_t(o.local_size()),
+                "bad batched arguments");
+        o.apply_batched(X, Y, ncols, ld);
+    });
+}
+
+int tdg_set_stream(tdg_op* op, void* stream) {
+    return guarded([&] {
+        require(op && op->impl, "null operator");
+        op->impl->set_stream(static_cast<hipStream_t>(stream));
+    });
+}
+
+int tdg_rhs(tdg_op const* op, double const* force_Q, double const* dirichlet_q, double* b) {
+    return guarded([&] {
+        require(op && op->impl && b, "null argument");
+        op->impl->rhs(force_Q, dirichlet_q, b);
+    });
+}
+
+int tdg_apply_counts(tdg_op const* op, int pb, double* bytes, double* flops) {
+    return guarded([&] {
+        require(op && op->impl && bytes && flops, "null argument");
+        *bytes = op->impl->apply_bytes(pb);
+        *flops = op->impl->apply_flops();
+    });
+}
+
+int tdg_host_array(tdg_op const* op, char const* name, double* out, int64_t* n) {
+    return guarded([&] {
+        require(op && op->impl && name && n, "null argument");
+        auto v = op->impl->host_array(name);
+        require(v != nullptr, "unknown array name");
+        *n = int64_t(v->size());
+        if (out) std::memcpy(out, v->data(), v->size() * sizeof(double));
+    });
+}
+
+int tdg_box_mesh(int dim, int64_t const* n, int scheme, int const* bc, int64_t* sizes, double* vertices,
+                 int64_t* elements, int64_t* boundary_facets, int* boundary_bc) {
+    return guarded([&] {
+        require(n && bc && sizes, "null argument");
+        require(dim == 2 || dim == 3, "dim must be 2 or 3");
+        std::array<uint64_t, 3> nn{1, 1, 1};
+        for (int d = 0; d < dim; ++d) {
+            require(n[d] > 0, "cell counts must be positive");
+            nn[d] = uint64_t(n[d]);
+        }
+        std::array<int, 6> b{};
+        for (int i = 0; i < 2 * dim; ++i) b[i] = bc[i];
+        auto m = tdg::box_mesh(dim, nn, scheme, b);
+        sizes[0] = int64_t(m.vertices.size() / dim);
+        sizes[1] = int64_t(m.num_elements());
+        sizes[2] = int64_t(m.bnd_bc.size());
+        if (vertices) std::memcpy(vertices, m.vertices.data(), m.vertices.size() * sizeof(double));
+        if (elements) std::memcpy(elements, m.elements.data(), m.elements.size() * sizeof(int64_t));
+        if (boundary_facets) std::memcpy(boundary_facets, m.bnd_facets.data(), m.bnd_facets.size() * sizeof(int64_t));
+        if (boundary_bc) std::memcpy(boundary_bc, m.bnd_bc.data(), m.bnd_bc.size() * sizeof(int));
+    });
+}
+
+int tdg_reference_nodes(int dim, int degree, double* out) {
+    return guarded([&] {
+        require(out && (dim == 2 || dim == 3) && degree >= 1, "bad arguments");
+        auto v = tdg::warp_blend_nodes(dim, degree);
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    });
+}
+
+} // extern "C"

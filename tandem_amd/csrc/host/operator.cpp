@@ -1,0 +1,591 @@
+// Host side of the MI355X DG operator: FP64 precompute and device residency.
+//
+// The precompute restates tandem's DGOperator constructor (src/form/DGOperator.h:54-105):
+// geometry per element and facet (DGCurvilinearCommon.cpp:41-101, Curvilinear.cpp:195-275),
+// L2 projection of the coefficients onto the nodal P_N space (Elasticity.cpp:104-164,
+// Poisson.cpp:133-160), facet traces of the coefficients (Elasticity.cpp:189-225), the
+// weighted volume coefficients (Elasticity.cpp:227-253) and the SIPG penalty
+// (Elasticity.cpp:255-292, Poisson.cpp:209-223).  Everything here stays FP64; FP32 device copies
+// are rounded from it.
+#include "operator.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace tdg {
+
+#define TDG_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> void DeviceArray<T>::reset() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+}
+template <class T> void DeviceArray<T>::alloc(std::size_t count) {
+    reset();
+    if (count == 0) return;
+    TDG_HIP(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)));
+    n = count;
+}
+template <class T> void DeviceArray<T>::upload(std::vector<T> const& h) {
+    alloc(h.size());
+    if (!h.empty()) TDG_HIP(hipMemcpy(ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+template struct DeviceArray<double>;
+template struct DeviceArray<float>;
+template struct DeviceArray<int32_t>;
+
+static double trace_constant(unsigned D, unsigned N) { return (N + 1) * (N + D) / static_cast<double>(D); }
+
+static double det(unsigned D, double const* J) {
+    if (D == 2) return J[0] * J[3] - J[2] * J[1];
+    return J[0] * (J[4] * J[8] - J[7] * J[5]) - J[3] * (J[1] * J[8] - J[7] * J[2]) + J[6] * (J[1] * J[5] - J[4] * J[2]);
+}
+static void inverse(unsigned D, double const* J, double* R) {
+    std::vector<double> A(J, J + D * D), I(D * D, 0.0);
+    for (unsigned i = 0; i < D; ++i) I[i + i * D] = 1.0;
+    dense_solve(A, D, I.data(), D);
+    std::memcpy(R, I.data(), D * D * sizeof(double));
+}
+
+Operator::Operator(SimplexMesh mesh, int kind, unsigned degree, std::vector<double> const* geometry_nodes)
+    : kind_(kind), D_(mesh.dim), N_(degree), mesh_(std::move(mesh)), ref_(mesh_.dim) {
+    if (D_ != 2 && D_ != 3) throw std::runtime_error("dimension must be 2 or 3");
+    if (N_ < 1 || N_ > 6) throw std::runtime_error("polynomial degree must be in [1, 6]");
+    if (kind_ != Kind_Elasticity && kind_ != Kind_Poisson) throw std::runtime_error("unknown operator kind");
+    E_ = mesh_.num_elements();
+    if (E_ == 0) throw std::runtime_error("empty mesh");
+    int64_t nv = static_cast<int64_t>(mesh_.vertices.size() / D_);
+    for (auto v : mesh_.elements)
+        if (v < 0 || v >= nv) throw std::runtime_error("element references a vertex out of range");
+    Q_ = kind_ == Kind_Elasticity ? D_ : 1;
+    topo_ = build_topology(mesh_);
+    build_tables();
+    build_geometry(geometry_nodes);
+}
+
+Operator::~Operator() = default;
+
+void Operator::build_tables() {
+    unsigned order = 2 * N_ + 1; // MinQuadOrder() (app/config.h.in:8-11)
+    vol_ = simplex_rule(D_, order);
+    fct_ = simplex_rule(D_ - 1, order);
+    Nq_ = vol_.size();
+    nq_ = fct_.size();
+    Basis sol(D_, N_, kind_ == Kind_Elasticity); // Poisson: modal solution space (Poisson.h:130)
+    Basis mat(D_, N_, true);
+    Nbf_ = sol.size();
+    E_Q_ = sol.values(vol_.points);
+    Dxi_Q_ = sol.grads(vol_.points);
+    matE_Q_ = mat.values(vol_.points);
+    E_q_.clear();
+    Dxi_q_.clear();
+    matE_q_.clear();
+    faceJinv_.clear();
+    for (unsigned f = 0; f <= D_; ++f) {
+        auto pts = ref_.facet_points(f, fct_.points);
+        E_q_.push_back(sol.values(pts));
+        Dxi_q_.push_back(sol.grads(pts));
+        matE_q_.push_back(mat.values(pts));
+        faceJinv_.push_back(ref_.face_map_jacobian_inverse(f));
+    }
+    perm_.assign((D_ + 1) * Nbf_, 0);
+    iperm_.assign((D_ + 1) * Nbf_, 0);
+    if (sol.nodal()) {
+        // The warp-and-blend node set is invariant under the vertex permutations of the simplex, so
+        // the basis on face f is the face-0 basis with nodes relabelled: E_q[f](pi_f(j)) = E_q[0](j).
+        auto const& nodes = sol.nodes();
+        const double rv[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        for (unsigned f = 0; f <= D_; ++f) {
+            auto const& fv = ref_.faces[f];
+            for (unsigned j = 0; j < Nbf_; ++j) {
+                double y[3] = {0, 0, 0};
+                for (unsigned i = 0; i < D_; ++i) {
+                    y[i] = rv[fv[0]][i];
+                    for (unsigned d = 0; d < D_; ++d) y[i] += (rv[fv[d + 1]][i] - rv[fv[0]][i]) * nodes[j * D_ + d];
+                }
+                int hit = -1;
+                for (unsigned k = 0; k < Nbf_; ++k) {
+                    double dist = 0.0;
+                    for (unsigned i = 0; i < D_; ++i) dist = std::max(dist, std::fabs(nodes[k * D_ + i] - y[i]));
+                    if (dist < 1e-10) hit = static_cast<int>(k);
+                }
+                if (hit < 0) throw std::runtime_error("node set is not symmetric under facet maps");
+                perm_[f * Nbf_ + j] = hit;
+                iperm_[f * Nbf_ + hit] = static_cast<int>(j);
+            }
+            // verify the relabelling reproduces the face tables
+            for (unsigned q = 0; q < nq_; ++q)
+                for (unsigned j = 0; j < Nbf_; ++j) {
+                    double a = E_q_[f][perm_[f * Nbf_ + j] + Nbf_ * q], b = E_q_[0][j + Nbf_ * q];
+                    if (std::fabs(a - b) > 1e-10) throw std::runtime_error("facet basis relabelling check failed");
+                }
+        }
+    }
+}
+
+void Operator::build_geometry(std::vector<double> const* geometry_nodes) {
+    unsigned D = D_, DD = D * D;
+    Basis geo(D, N_, true); // geometry degree = solution degree (static.cpp:81)
+    unsigned ng = geo.size();
+    std::vector<double> X;
+    if (geometry_nodes) {
+        if (geometry_nodes->size() != E_ * ng * D) throw std::runtime_error("geometry_nodes has the wrong size");
+        X = *geometry_nodes;
+    } else {
+        // affine map of the reference nodes with vertices in ascending order (Curvilinear.cpp:101-106)
+        X.resize(E_ * ng * D);
+        auto const& rn = geo.nodes();
+#pragma omp parallel for schedule(static)
+        for (std::size_t e = 0; e < E_; ++e) {
+            int64_t const* el = &mesh_.elements[e * (D + 1)];
+            for (unsigned p = 0; p < ng; ++p) {
+                double s = 0.0;
+                for (unsigned d = 0; d < D; ++d) s += rn[p * D + d];
+                for (unsigned i = 0; i < D; ++i) {
+                    double v = (1.0 - s) * mesh_.vertices[el[0] * D + i];
+                    for (unsigned d = 0; d < D; ++d) v = v + rn[p * D + d] * mesh_.vertices[el[d + 1] * D + i];
+                    X[(e * ng + p) * D + i] = v;
+                }
+            }
+        }
+    }
+    auto gE_Q = geo.values(vol_.points);
+    auto gD_Q = geo.grads(vol_.points);
+    std::vector<std::vector<double>> gE_q, gD_q;
+    for (unsigned f = 0; f <= D; ++f) {
+        auto pts = ref_.facet_points(f, fct_.points);
+        gE_q.push_back(geo.values(pts));
+        gD_q.push_back(geo.grads(pts));
+    }
+    // J(d, j, p) = sum_k X_k[d] dphi_k/dxi_j(p)
+    auto jac = [&](std::size_t e, std::vector<double> const& gD, unsigned np, double* J) {
+        double const* Xe = &X[e * ng * D];
+        for (unsigned p = 0; p < np; ++p)
+            for (unsigned j = 0; j < D; ++j)
+                for (unsigned d = 0; d < D; ++d) {
+                    double s = 0.0;
+                    for (unsigned k = 0; k < ng; ++k) s += Xe[k * D + d] * gD[k + ng * (j + D * p)];
+                    J[d + D * (j + D * p)] = s;
+                }
+    };
+    auto map = [&](std::size_t e, std::vector<double> const& gE, unsigned np, double* x) {
+        double const* Xe = &X[e * ng * D];
+        for (unsigned p = 0; p < np; ++p)
+            for (unsigned d = 0; d < D; ++d) {
+                double s = 0.0;
+                for (unsigned k = 0; k < ng; ++k) s += Xe[k * D + d] * gE[k + ng * p];
+                x[d + D * p] = s;
+            }
+    };
+    absDetJ_.assign(E_ * Nq_, 0.0);
+    JInv_.assign(E_ * Nq_ * DD, 0.0);
+    vcoords_.assign(E_ * Nq_ * D, 0.0);
+    volume_.assign(E_, 0.0);
+    bool bad = false;
+#pragma omp parallel for schedule(static)
+    for (std::size_t e = 0; e < E_; ++e) {
+        std::vector<double> J(Nq_ * DD);
+        jac(e, gD_Q, Nq_, J.data());
+        double vol = 0.0;
+        for (unsigned q = 0; q < Nq_; ++q) {
+            double dj = det(D, &J[q * DD]);
+            if (!(std::fabs(dj) > 0.0)) bad = true;
+            absDetJ_[e * Nq_ + q] = std::fabs(dj);
+            inverse(D, &J[q * DD], &JInv_[(e * Nq_ + q) * DD]);
+            vol += vol_.weights[q] * std::fabs(dj);
+        }
+        volume_[e] = vol;
+        map(e, gE_Q, Nq_, &vcoords_[e * Nq_ * D]);
+    }
+    if (bad) throw std::runtime_error("degenerate element (zero Jacobian determinant)");
+    std::size_t F = topo_.facets.size();
+    for (int s = 0; s < 2; ++s) fJInv_[s].assign(F * nq_ * DD, 0.0);
+    normal_.assign(F * nq_ * D, 0.0);
+    nlen_.assign(F * nq_, 0.0);
+    fcoords_.assign(F * nq_ * D, 0.0);
+    area_.assign(F, 0.0);
+#pragma omp parallel for schedule(static)
+    for (std::size_t i = 0; i < F; ++i) {
+        Facet const& fc = topo_.facets[i];
+        std::vector<double> J(nq_ * DD);
+        jac(fc.elem[0], gD_q[fc.local[0]], nq_, J.data());
+        auto const& rn = ref_.normals[fc.local[0]];
+        double a = 0.0;
+        for (unsigned q = 0; q < nq_; ++q) {
+            double dj = det(D, &J[q * DD]);
+            double* Ji = &fJInv_[0][(i * nq_ + q) * DD];
+            inverse(D, &J[q * DD], Ji);
+            // area-scaled normal n = |det J| J^{-T} N_ref (Curvilinear.cpp:256-267)
+            double ss = 0.0;
+            for (unsigned r = 0; r < D; ++r) {
+                double v = 0.0;
+                for (unsigned j = 0; j < D; ++j) v += Ji[j + r * D] * rn[j];
+                v *= std::fabs(dj);
+                normal_[(i * nq_ + q) * D + r] = v;
+                ss += v * v;
+            }
+            nlen_[i * nq_ + q] = std::sqrt(ss);
+            a += fct_.weights[q] * nlen_[i * nq_ + q];
+        }
+        area_[i] = a;
+        map(fc.elem[0], gE_q[fc.local[0]], nq_, &fcoords_[i * nq_ * D]);
+        jac(fc.elem[1], gD_q[fc.local[1]], nq_, J.data());
+        for (unsigned q = 0; q < nq_; ++q) inverse(D, &J[q * DD], &fJInv_[1][(i * nq_ + q) * DD]);
+    }
+}
+
+void Operator::set_material(double const* c0_Q, double const* c1_Q) {
+    unsigned D = D_, DD = D * D, Nm = static_cast<unsigned>(num_basis(D, N_));
+    bool el = kind_ == Kind_Elasticity;
+    if (!c0_Q || (el && !c1_Q)) throw std::runtime_error("missing coefficient array");
+    std::size_t F = topo_.facets.size();
+    // L2 projection onto the nodal material space: M c = b, M = matE^T diag(W|J|) matE
+    mat0_.assign(E_ * Nm, 0.0);
+    mat1_.assign(el ? E_ * Nm : 0, 0.0);
+    bool bad = false;
+#pragma omp parallel for schedule(static)
+    for (std::size_t e = 0; e < E_; ++e) {
+        std::vector<double> M(Nm * Nm, 0.0), b(Nm * 2, 0.0);
+        for (unsigned q = 0; q < Nq_; ++q) {
+            double wj = vol_.weights[q] * absDetJ_[e * Nq_ + q];
+            double const* phi = &matE_Q_[Nm * q];
+            for (unsigned l = 0; l < Nm; ++l) {
+                double pl = phi[l] * wj;
+                for (unsigned k = 0; k < Nm; ++k) M[k + l * Nm] += phi[k] * pl;
+                b[l] += c0_Q[e * Nq_ + q] * pl;
+                if (el) b[l + Nm] += c1_Q[e * Nq_ + q] * pl;
+            }
+        }
+        try {
+            dense_solve(M, Nm, b.data(), el ? 2 : 1);
+        } catch (...) {
+            bad = true;
+        }
+        std::memcpy(&mat0_[e * Nm], b.data(), Nm * sizeof(double));
+        if (el) std::memcpy(&mat1_[e * Nm], b.data() + Nm, Nm * sizeof(double));
+    }
+    if (bad) throw std::runtime_error("material projection failed (singular mass matrix)");
+    // volume coefficients: lamWJ, muWJ (precomputeVolume) or JWK (J_W_K_Q)
+    c0Q_.assign(E_ * Nq_, 0.0);
+    c1Q_.assign(el ? E_ * Nq_ : 0, 0.0);
+#pragma omp parallel for schedule(static)
+    for (std::size_t e = 0; e < E_; ++e)
+        for (unsigned q = 0; q < Nq_; ++q) {
+            double a = 0.0, b = 0.0;
+            for (unsigned t = 0; t < Nm; ++t) {
+                a += matE_Q_[t + Nm * q] * mat0_[e * Nm + t];
+                if (el) b += matE_Q_[t + Nm * q] * mat1_[e * Nm + t];
+            }
+            double wj = vol_.weights[q] * absDetJ_[e * Nq_ + q];
+            c0Q_[e * Nq_ + q] = el ? a * wj : wj * a;
+            if (el) c1Q_[e * Nq_ + q] = b * wj;
+        }
+    // facet coefficient traces (both sides; a boundary facet mirrors side 0) and kernel geometry
+    fc0_.assign(F * 2 * nq_, 0.0);
+    fc1_.assign(F * 2 * nq_, 0.0);
+    fGp_.assign(F * 2 * nq_ * DD, 0.0);
+#pragma omp parallel for schedule(static)
+    for (std::size_t i = 0; i < F; ++i) {
+        Facet const& fc = topo_.facets[i];
+        bool skel = fc.elem[0] != fc.elem[1];
+        for (int side = 0; side < 2; ++side) {
+            int s = skel ? side : 0;
+            int64_t e = fc.elem[s];
+            int lf = fc.local[s];
+            for (unsigned q = 0; q < nq_; ++q) {
+                double a = 0.0, b = 0.0;
+                for (unsigned t = 0; t < Nm; ++t) {
+                    a += matE_q_[lf][t + Nm * q] * mat0_[e * Nm + t];
+                    if (el) b += matE_q_[lf][t + Nm * q] * mat1_[e * Nm + t];
+                }
+                fc0_[(i * 2 + side) * nq_ + q] = a;
+                fc1_[(i * 2 + side) * nq_ + q] = b;
+                double const* Ji = &fJInv_[s][(i * nq_ + q) * DD];
+                double* out = &fGp_[((i * 2 + side) * nq_ + q) * DD];
+                if (el) {
+                    // G' = Jf^{-1} JInv: gradients of the face-0-relabelled basis (see build_tables)
+                    auto const& Jf = faceJinv_[lf];
+                    for (unsigned ep = 0; ep < D; ++ep)
+                        for (unsigned j = 0; j < D; ++j) {
+                            double v = 0.0;
+                            for (unsigned e2 = 0; e2 < D; ++e2) v += Jf[ep + e2 * D] * Ji[e2 + j * D];
+                            out[ep + j * D] = v;
+                        }
+                } else {
+                    // K_G_q = K_q JInv (poisson.py:121)
+                    for (unsigned k = 0; k < DD; ++k) out[k] = a * Ji[k];
+                }
+            }
+        }
+    }
+    // SIPG penalty (Elasticity.cpp:279-292 / Poisson.cpp:209-223)
+    penalty_.assign(F, 0.0);
+    double cN1 = trace_constant(D, N_ - 1);
+    auto pside = [&](std::size_t i, int s) {
+        int64_t e = topo_.facets[i].elem[s];
+        double lo = std::numeric_limits<double>::max(), hi = std::numeric_limits<double>::lowest();
+        for (unsigned t = 0; t < Nm; ++t) {
+            if (el) {
+                lo = std::min(lo, 2.0 * mat1_[e * Nm + t]);
+                hi = std::max(hi, D * mat0_[e * Nm + t] + 2.0 * mat1_[e * Nm + t]);
+            } else {
+                lo = std::min(lo, mat0_[e * Nm + t]);
+                hi = std::max(hi, mat0_[e * Nm + t]);
+            }
+        }
+        return (D + 1) * cN1 * (area_[i] / volume_[e]) * (hi * hi / lo);
+    };
+    for (std::size_t i = 0; i < F; ++i) {
+        Facet const& fc = topo_.facets[i];
+        penalty_[i] = fc.elem[0] != fc.elem[1] ? (pside(i, 0) + pside(i, 1)) / 4.0 : pside(i, 0);
+    }
+    // per element face: facet, neighbour, flags = kind | side << 2 | neighbour local face << 4
+    unsigned nf = D + 1;
+    ef_.assign(E_ * nf * 4, 0);
+    for (std::size_t e = 0; e < E_; ++e)
+        for (unsigned f = 0; f < nf; ++f) {
+            Side const& sd = topo_.sides[e * nf + f];
+            bool skel = sd.nb != static_cast<int64_t>(e);
+            bool fd = sd.bc == BC_Fault || sd.bc == BC_Dirichlet;
+            int kind = (sd.bc == BC_None || (skel && fd)) ? Face_Skeleton : (fd ? Face_Boundary : Face_Skip);
+            int32_t* o = &ef_[(e * nf + f) * 4];
+            o[0] = static_cast<int32_t>(sd.facet);
+            o[1] = static_cast<int32_t>(sd.nb);
+            o[2] = kind | (sd.side << 2) | (sd.nb_local << 4);
+            o[3] = 0;
+        }
+    if (E_ >= (std::size_t(1) << 31) || F >= (std::size_t(1) << 31)) throw std::runtime_error("mesh too large for 32-bit indices");
+    d64_.reset();
+    d32_.reset();
+    ready_ = true;
+}
+
+// Device residency of both precisions (first apply after set_material, or an explicit call).
+void Operator::upload_device() {
+    if (!ready_) throw std::logic_error("set_material() must be called before the device upload");
+    if (d64_ && d32_) return;
+    auto a = std::make_unique<DeviceOperator<double>>();
+    auto b = std::make_unique<DeviceOperator<float>>();
+    upload(*a);
+    upload(*b);
+    d64_ = std::move(a);
+    d32_ = std::move(b);
+}
+
+template <class T> static std::vector<T> cast(std::vector<double> const& v) {
+    return std::vector<T>(v.begin(), v.end());
+}
+
+template <class T> void Operator::upload(DeviceOperator<T>& d) {
+    unsigned D = D_, Nf = D + 1;
+    // volume tables: tabV[l][e][q] = Dxi_Q(l,e,q); tabVt[e][q][k] = Dxi_Q(k,e,q)
+    std::vector<double> tV(Nbf_ * D * Nq_), tVt(Nbf_ * D * Nq_);
+    for (unsigned l = 0; l < Nbf_; ++l)
+        for (unsigned e = 0; e < D; ++e)
+            for (unsigned q = 0; q < Nq_; ++q) {
+                double v = Dxi_Q_[l + Nbf_ * (e + D * q)];
+                tV[(l * D + e) * Nq_ + q] = v;
+                tVt[(e * Nq_ + q) * Nbf_ + l] = v;
+            }
+    // facet tables, channel c = 0: value, c = 1 + e: d/dxi_e.  Elasticity keeps face 0 only.
+    unsigned nfaces = kind_ == Kind_Elasticity ? 1 : Nf;
+    std::vector<double> tF(nfaces * Nbf_ * (D + 1) * nq_), tFt(tF.size());
+    for (unsigned f = 0; f < nfaces; ++f)
+        for (unsigned l = 0; l < Nbf_; ++l)
+            for (unsigned c = 0; c <= D; ++c)
+                for (unsigned q = 0; q < nq_; ++q) {
+                    double v = c == 0 ? E_q_[f][l + Nbf_ * q] : Dxi_q_[f][l + Nbf_ * ((c - 1) + D * q)];
+                    tF[((f * Nbf_ + l) * (D + 1) + c) * nq_ + q] = v;
+                    tFt[((f * (D + 1) + c) * nq_ + q) * Nbf_ + l] = v;
+                }
+    d.tabV.upload(cast<T>(tV));
+    d.tabVt.upload(cast<T>(tVt));
+    d.tabF.upload(cast<T>(tF));
+    d.tabFt.upload(cast<T>(tFt));
+    d.W.upload(cast<T>(vol_.weights));
+    d.w.upload(cast<T>(fct_.weights));
+    d.perm.upload(perm_);
+    d.iperm.upload(iperm_);
+    d.G.upload(cast<T>(JInv_));
+    d.c0.upload(cast<T>(c0Q_));
+    d.c1.upload(cast<T>(c1Q_));
+    d.fG.upload(cast<T>(fGp_));
+    d.fc0.upload(cast<T>(fc0_));
+    d.fc1.upload(cast<T>(fc1_));
+    d.fn.upload(cast<T>(normal_));
+    d.fpen.upload(cast<T>(penalty_));
+    d.ef.upload(ef_);
+}
+
+// rhs_volume + rhs_boundary with the interior-penalty lift (Elasticity.cpp:521-696 /
+// Poisson.cpp:383-540; rhsVolume, rhs_lift_ip, rhsFacet).  Dirichlet data on boundary facets only.
+void Operator::rhs(double const* force_Q, double const* dirichlet_q, double* b) const {
+    if (!ready_) throw std::runtime_error("set_material() must be called first");
+    unsigned D = D_, DD = D * D, Nm = static_cast<unsigned>(num_basis(D, N_));
+    std::size_t bs = block_size();
+    bool el = kind_ == Kind_Elasticity;
+    std::fill(b, b + bs * E_, 0.0);
+    if (force_Q) {
+#pragma omp parallel for schedule(static)
+        for (std::size_t e = 0; e < E_; ++e)
+            for (unsigned p = 0; p < Q_; ++p)
+                for (unsigned k = 0; k < Nbf_; ++k) {
+                    double s = 0.0;
+                    for (unsigned q = 0; q < Nq_; ++q)
+                        s += absDetJ_[e * Nq_ + q] * vol_.weights[q] * E_Q_[k + Nbf_ * q] * force_Q[(e * Q_ + p) * Nq_ + q];
+                    b[e * bs + k + p * Nbf_] += s;
+                }
+    }
+    if (!dirichlet_q) return;
+    std::size_t F = topo_.facets.size();
+    for (std::size_t i = 0; i < F; ++i) {
+        Facet const& fc = topo_.facets[i];
+        if (fc.elem[0] != fc.elem[1] || fc.bc != BC_Dirichlet) continue;
+        double const* fq = &dirichlet_q[i * Q_ * nq_];
+        double const* Ji = &fJInv_[0][i * nq_ * DD];
+        auto const& Eq = E_q_[fc.local[0]];
+        auto const& Dq = Dxi_q_[fc.local[0]];
+        double* B = b + fc.elem[0] * bs;
+        double pen = penalty_[i];
+        for (unsigned q = 0; q < nq_; ++q) {
+            double w = fct_.weights[q];
+            double const* n = &normal_[(i * nq_ + q) * D];
+            double nl = nlen_[i * nq_ + q];
+            for (unsigned k = 0; k < Nbf_; ++k) {
+                double dx[3];
+                for (unsigned j = 0; j < D; ++j) {
+                    double v = 0.0;
+                    for (unsigned e = 0; e < D; ++e) v += Ji[q * DD + e + j * D] * Dq[k + Nbf_ * (e + D * q)];
+                    dx[j] = v;
+                }
+                if (el) {
+                    double lq = fc0_[(i * 2) * nq_ + q], mq = fc1_[(i * 2) * nq_ + q];
+                    double fn = 0.0, nd = 0.0, dfd = 0.0;
+                    for (unsigned j = 0; j < D; ++j) {
+                        fn += fq[j * nq_ + q] * n[j];
+                        nd += dx[j] * n[j];
+                        dfd += dx[j] * fq[j * nq_ + q];
+                    }
+                    for (unsigned p = 0; p < D; ++p) {
+                        double tt = lq * dx[p] * fn + mq * (nd * fq[p * nq_ + q] + dfd * n[p]);
+                        B[k + p * Nbf_] += -tt * w + pen * w * Eq[k + Nbf_ * q] * fq[p * nq_ + q] * nl;
+                    }
+                } else {
+                    double kq = fc0_[(i * 2) * nq_ + q];
+                    double kn = 0.0;
+                    for (unsigned j = 0; j < D; ++j) kn += kq * dx[j] * n[j];
+                    B[k] += -w * kn * fq[q] + pen * w * Eq[k + Nbf_ * q] * nl * fq[q];
+                }
+            }
+        }
+    }
+    (void)Nm;
+}
+
+std::vector<double> const* Operator::host_array(std::string const& n) const {
+    if (n == "absDetJ") return &absDetJ_;
+    if (n == "JInv") return &JInv_;
+    if (n == "vcoords") return &vcoords_;
+    if (n == "volume") return &volume_;
+    if (n == "JInv0") return &fJInv_[0];
+    if (n == "JInv1") return &fJInv_[1];
+    if (n == "normal") return &normal_;
+    if (n == "normalLength") return &nlen_;
+    if (n == "fcoords") return &fcoords_;
+    if (n == "area") return &area_;
+    if (n == "penalty") return &penalty_;
+    if (n == "mat0") return &mat0_;
+    if (n == "mat1") return &mat1_;
+    if (n == "c0Q") return &c0Q_;
+    if (n == "c1Q") return &c1Q_;
+    if (n == "fc0") return &fc0_;
+    if (n == "fc1") return &fc1_;
+    return nullptr;
+}
+
+// Algorithmic bytes of one apply: x and y once, per volume point JInv + 2 coefficients, per facet
+// both sides' JInv, normal and unit normal, both sides' 2 coefficients and the penalty.
+double Operator::apply_bytes(int pb) const {
+    double D = D_, E = double(E_), F = double(topo_.facets.size());
+    double nqv = Nq_, nqf = nq_, bs = double(block_size());
+    double c = kind_ == Kind_Elasticity ? 2.0 : 1.0;
+    return pb * (E * (2.0 * bs + nqv * (D * D + c)) + F * (2.0 * D * D * nqf + 2.0 * D * nqf + 2.0 * c * nqf + 1.0));
+}
+
+// Algorithmic flops: volume gradient/divergence GEMMs + per-point transforms, per element face
+// own value/gradient traces and the lift, and one flux combination per facet.
+double Operator::apply_flops() const {
+    double D = D_, E = double(E_), F = double(topo_.facets.size());
+    double Nq = Nq_, nq = nq_, Nbf = Nbf_, Q = Q_, nf = D + 1;
+    double vol = 4 * D * Nq * Nbf * Q + 4 * D * D * Nq * Q + (3 * D * D + 2 * D) * Nq;
+    double face = 2 * nq * Nbf * Q + (2 * D * nq * Nbf * Q + 2 * D * D * nq * Q) +
+                  (2 * Nbf * D * nq * Q + 2 * D * D * nq * Q + 2 * Nbf * nq * Q);
+    return E * vol + nf * E * face + F * (6 * D * D + 6 * D) * nq;
+}
+
+} // namespace tdg
+
+#include "../kernels/kernels.h"
+
+namespace tdg {
+
+template <class T>
+void Operator::launch(DeviceOperator<T> const& d, T const* x, T* y, int ncols, int64_t ld) {
+    if (!ready_) throw std::logic_error("set_material() must be called before apply");
+    ApplyArgs<T> a;
+    a.tabV = d.tabV.ptr;
+    a.tabVt = d.tabVt.ptr;
+    a.tabF = d.tabF.ptr;
+    a.tabFt = d.tabFt.ptr;
+    a.w = d.w.ptr;
+    a.perm = d.perm.ptr;
+    a.iperm = d.iperm.ptr;
+    a.G = d.G.ptr;
+    a.c0 = d.c0.ptr;
+    a.c1 = d.c1.ptr;
+    a.fG = d.fG.ptr;
+    a.fc0 = d.fc0.ptr;
+    a.fc1 = d.fc1.ptr;
+    a.fn = d.fn.ptr;
+    a.fpen = d.fpen.ptr;
+    a.ef = reinterpret_cast<int4 const*>(d.ef.ptr);
+    a.num_elements = static_cast<int64_t>(E_);
+    int kq = 0, kf = 0;
+    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
+        throw std::runtime_error("no kernel compiled for this dimension/degree");
+    hipError_t err = kind_ == Kind_Elasticity ? launch_elasticity<T>(D_, N_, a, x, y, ncols, ld, stream_)
+                                              : launch_poisson<T>(D_, N_, a, x, y, ncols, ld, stream_);
+    TDG_HIP(err);
+}
+
+template <> DeviceOperator<double> const& Operator::device<double>() {
+    if (!d64_ || !d32_) upload_device();
+    return *d64_;
+}
+template <> DeviceOperator<float> const& Operator::device<float>() {
+    if (!d64_ || !d32_) upload_device();
+    return *d32_;
+}
+
+void Operator::apply(double const* x, double* y) { launch(device<double>(), x, y, 1, 0); }
+void Operator::apply(float const* x, float* y) { launch(device<float>(), x, y, 1, 0); }
+void Operator::apply_batched(double const* X, double* Y, int ncols, int64_t ld) {
+    launch(device<double>(), X, Y, ncols, ld);
+}
+void Operator::apply_batched(float const* X, float* Y, int ncols, int64_t ld) {
+    launch(device<float>(), X, Y, ncols, ld);
+}
+
+} // namespace tdg

@@ -1,0 +1,134 @@
+// Host side of the MI355X DG operator: FP64 precompute (geometry, material projection, penalty) in
+// the layouts the HIP kernels stream, device residency and kernel dispatch.
+#pragma once
+
+#include "mesh.h"
+#include "refelem.h"
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace tdg {
+
+enum OpKind : int { Kind_Elasticity = 0, Kind_Poisson = 1 };
+
+// Face kinds of the element-centric apply (Elasticity.cpp:721-748, Poisson.cpp:573-636).
+enum FaceKind : int { Face_Skip = 0, Face_Skeleton = 1, Face_Boundary = 2 };
+
+template <class T> struct DeviceArray {
+    T* ptr = nullptr;
+    std::size_t n = 0;
+    DeviceArray() = default;
+    DeviceArray(DeviceArray const&) = delete;
+    DeviceArray& operator=(DeviceArray const&) = delete;
+    ~DeviceArray() { reset(); }
+    void reset();
+    void upload(std::vector<T> const& h);
+    void alloc(std::size_t count);
+};
+
+// Everything one apply reads, in one precision.
+template <class T> struct DeviceOperator {
+    DeviceArray<T> tabV, tabVt, tabF, tabFt; // reference tables
+    DeviceArray<T> W, w;                       // volume / facet weights
+    DeviceArray<int32_t> perm, iperm;          // nodal facet permutations (elasticity)
+    DeviceArray<T> G, c0, c1;                  // per element & volume point: JInv, lamWJ/muWJ (or JWK)
+    DeviceArray<T> fG, fc0, fc1, fn, fpen;     // per facet
+    DeviceArray<int32_t> ef;                   // per element face: facet, neighbour, flags
+};
+
+class Operator {
+public:
+    Operator(SimplexMesh mesh, int kind, unsigned degree, std::vector<double> const* geometry_nodes);
+    ~Operator();
+
+    int kind() const { return kind_; }
+    unsigned dim() const { return D_; }
+    unsigned degree() const { return N_; }
+    std::size_t num_elements() const { return E_; }
+    std::size_t num_facets() const { return topo_.facets.size(); }
+    unsigned nbf() const { return Nbf_; }
+    unsigned nq_vol() const { return Nq_; }
+    unsigned nq_fct() const { return nq_; }
+    std::size_t block_size() const { return Nbf_ * Q_; }
+    std::size_t local_size() const { return block_size() * E_; }
+    bool ready() const { return ready_; }
+
+    std::vector<double> const& volume_points() const { return vcoords_; }
+    std::vector<double> const& facet_points() const { return fcoords_; }
+    Topology const& topology() const { return topo_; }
+
+    // Coefficients at the volume quadrature points: (lambda, mu) or (K, unused).  Projects them
+    // onto the nodal P_N material space, builds all per-point data and uploads it.
+    void set_material(double const* c0_Q, double const* c1_Q);
+
+    // Uploads both precisions to the device (done implicitly by the first apply).
+    void upload_device();
+    template <class T> DeviceOperator<T> const& device();
+
+    void set_stream(hipStream_t s) { stream_ = s; }
+    hipStream_t stream() const { return stream_; }
+
+    // y = A x on device buffers (x, y of local_size() entries).
+    void apply(double const* x, double* y);
+    void apply(float const* x, float* y);
+    // multi-vector apply: columns of length local_size() with leading dimension ld
+    void apply_batched(double const* X, double* Y, int ncols, int64_t ld);
+    void apply_batched(float const* X, float* Y, int ncols, int64_t ld);
+
+    // right-hand side (force at volume points, Dirichlet data at facet points), FP64, host.
+    void rhs(double const* force_Q, double const* dirichlet_q, double* b_host) const;
+
+    // named host arrays for inspection
+    std::vector<double> const* host_array(std::string const& name) const;
+
+    // algorithmic counts of one apply (bytes streamed, flops)
+    double apply_bytes(int precision_bytes) const;
+    double apply_flops() const;
+
+private:
+    void build_tables();
+    void build_geometry(std::vector<double> const* geometry_nodes);
+    template <class T> void upload(DeviceOperator<T>& d);
+    template <class T> void launch(DeviceOperator<T> const& d, T const* x, T* y, int ncols, int64_t ld);
+
+    int kind_;
+    unsigned D_, N_, Q_, Nbf_, Nq_, nq_;
+    std::size_t E_;
+    SimplexMesh mesh_;
+    Topology topo_;
+    RefSimplex ref_;
+    QuadRule vol_, fct_;
+    bool ready_ = false;
+    hipStream_t stream_ = nullptr;
+
+    // reference tables (FP64)
+    std::vector<double> E_Q_, Dxi_Q_;                       // solution basis at volume points
+    std::vector<std::vector<double>> E_q_, Dxi_q_;          // solution basis at facet points, per face
+    std::vector<double> matE_Q_;                            // material basis at volume points
+    std::vector<std::vector<double>> matE_q_;               // material basis at facet points, per face
+    std::vector<int32_t> perm_, iperm_;                     // facet node permutations (nodal bases)
+    std::vector<std::array<double, 9>> faceJinv_;           // Jacobian inverse of facet maps
+
+    // geometry (FP64)
+    std::vector<double> absDetJ_, JInv_, vcoords_, volume_;
+    std::vector<double> fJInv_[2], normal_, nlen_, fcoords_, area_;
+
+    // material-dependent (FP64)
+    std::vector<double> mat0_, mat1_;                       // projected nodal coefficients
+    std::vector<double> c0Q_, c1Q_;                         // lamWJ, muWJ (or JWK)
+    std::vector<double> fc0_, fc1_;                         // facet coefficient traces (2 sides)
+    std::vector<double> fGp_;                               // facet per-side geometry for kernels
+    std::vector<double> penalty_;
+    std::vector<int32_t> ef_;
+
+    std::unique_ptr<DeviceOperator<double>> d64_;
+    std::unique_ptr<DeviceOperator<float>> d32_;
+};
+
+} // namespace tdg

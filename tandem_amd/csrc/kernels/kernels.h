@@ -1,0 +1,43 @@
+// Launch interface of the HIP kernels of the DG operator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tdg {
+
+// Device views of one precision of the operator data (see host/operator.h for the layouts).
+template <class T> struct ApplyArgs {
+    T const* tabV;   // [l][e][q]   volume gradient table
+    T const* tabVt;  // [e][q][k]   volume lift table
+    T const* tabF;   // [f][l][c][q] facet trace table (c = 0 value, 1 + e gradient)
+    T const* tabFt;  // [f][c][q][k] facet lift table
+    T const* w;      // facet weights
+    int32_t const* perm;  // [f][j]  facet node relabelling (elasticity)
+    int32_t const* iperm; // [f][k]
+    T const* G;      // [E][Nq][D*D] JInv per volume point
+    T const* c0;     // [E][Nq] lamWJ (elasticity) / JWK (poisson)
+    T const* c1;     // [E][Nq] muWJ
+    T const* fG;     // [F][2][nq][D*D] per side facet geometry
+    T const* fc0;    // [F][2][nq] lambda traces / unused
+    T const* fc1;    // [F][2][nq] mu traces / unused
+    T const* fn;     // [F][nq][D] area-scaled normal of side 0
+    T const* fpen;   // [F] penalty
+    int4 const* ef;  // [E][D+1] facet, neighbour, flags
+    int64_t num_elements;
+};
+
+// y_c = A x_c for ncols columns (stride ld).  Returns hipErrorInvalidValue for an unsupported
+// (dim, degree) pair.
+template <class T>
+hipError_t launch_elasticity(int dim, int degree, ApplyArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld,
+                             hipStream_t stream);
+template <class T>
+hipError_t launch_poisson(int dim, int degree, ApplyArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld,
+                          hipStream_t stream);
+
+// Volume / facet rule sizes the kernels are compiled for (2N+1 rules of tandem's autorule).
+int kernel_rule_sizes(int dim, int degree, int* Nq, int* nq);
+
+} // namespace tdg

@@ -1,0 +1,206 @@
+"""Host-side mirror of tandem's operator interface over the C ABI.
+
+`DGOperator` mirrors tandem's AbstractDGOperator / DGOperator<LocalOperator>
+(src/form/AbstractDGOperator.h:18-50, src/form/DGOperator.h:26-400): block_size(),
+num_local_elements(), number_of_local_dofs(), apply(x, y), rhs(...).  `Elasticity` and `Poisson`
+mirror the local operators' constructors (app/localoperator/Elasticity.h:42-44, Poisson.h), with
+the coefficient functionals replaced by arrays evaluated at the operator's volume points.
+
+Device vectors are torch tensors on the GPU (torch is used for device memory and streams only; all
+arithmetic runs in the library's HIP kernels).
+"""
+import ctypes as C
+from math import comb
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr
+
+BC_NONE, BC_NATURAL, BC_FAULT, BC_DIRICHLET = 0, 1, 3, 5
+ELASTICITY, POISSON = 0, 1
+
+
+class Mesh:
+    """Simplex mesh input (vertices, elements, boundary facets with BCs)."""
+
+    def __init__(self, vertices, elements, boundary_facets=None, boundary_bc=None):
+        self.vertices = np.ascontiguousarray(vertices, dtype=np.float64)
+        self.dim = self.vertices.shape[1]
+        self.elements = np.sort(np.ascontiguousarray(elements, dtype=np.int64), axis=1)
+        if boundary_facets is None:
+            boundary_facets = np.zeros((0, self.dim), dtype=np.int64)
+            boundary_bc = np.zeros(0, dtype=np.int32)
+        self.boundary_facets = np.ascontiguousarray(np.sort(np.asarray(boundary_facets, dtype=np.int64), axis=1))
+        self.boundary_bc = np.ascontiguousarray(boundary_bc, dtype=np.int32)
+
+    @property
+    def num_elements(self):
+        return self.elements.shape[0]
+
+    @staticmethod
+    def box(dim, n, scheme=0, bc=BC_DIRICHLET):
+        """[0,1]^dim with n cells per direction (GenMesh tessellation, or Kuhn cubes for scheme 1)."""
+        n = np.ascontiguousarray(np.broadcast_to(np.asarray(n, dtype=np.int64), (dim,)))
+        bcs = np.ascontiguousarray(np.broadcast_to(np.asarray(bc, dtype=np.int32), (2 * dim,)))
+        sizes = np.zeros(3, dtype=np.int64)
+        check(lib().tdg_box_mesh(dim, n, scheme, bcs, sizes, None, None, None, None))
+        nv, ne, nb = (int(s) for s in sizes)
+        v = np.zeros((nv, dim))
+        e = np.zeros((ne, dim + 1), dtype=np.int64)
+        bf = np.zeros((nb, dim), dtype=np.int64)
+        bb = np.zeros(nb, dtype=np.int32)
+        check(lib().tdg_box_mesh(dim, n, scheme, bcs, sizes, ptr(v), ptr(e), ptr(bf), ptr(bb)))
+        return Mesh(v, e, bf, bb)
+
+    def affine_nodes(self, degree):
+        """Warp-and-blend geometry nodes of the affine map, (elements, nbf, dim)."""
+        ref = reference_nodes(self.dim, degree)
+        v = self.vertices[self.elements]  # (E, D+1, D)
+        lam0 = 1.0 - ref.sum(axis=1)
+        return lam0[None, :, None] * v[:, None, 0, :] + np.einsum("pd,edi->epi", ref, v[:, 1:, :])
+
+
+def reference_nodes(dim, degree):
+    out = np.zeros(comb(degree + dim, dim) * dim)
+    check(lib().tdg_reference_nodes(dim, degree, out))
+    return out.reshape(-1, dim)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class DGOperator:
+    """Matrix-free SIPG operator y = A x on one GPU."""
+
+    def __init__(self, mesh, kind, degree, geometry_nodes=None):
+        self.mesh = mesh
+        self.kind = kind
+        self.degree = degree
+        nodes = None
+        if geometry_nodes is not None:
+            nodes = np.ascontiguousarray(geometry_nodes, dtype=np.float64)
+            assert nodes.size == mesh.num_elements * comb(degree + mesh.dim, mesh.dim) * mesh.dim
+        desc = _lib.MeshDesc(mesh.dim, degree, mesh.vertices.shape[0], mesh.vertices.ctypes.data,
+                             mesh.num_elements, mesh.elements.ctypes.data, mesh.boundary_bc.shape[0],
+                             mesh.boundary_facets.ctypes.data if mesh.boundary_bc.shape[0] else None,
+                             mesh.boundary_bc.ctypes.data if mesh.boundary_bc.shape[0] else None,
+                             nodes.ctypes.data if nodes is not None else None)
+        h = C.c_void_p()
+        check(lib().tdg_create(C.byref(desc), kind, C.byref(h)))
+        self._h = h
+        s = np.zeros(7, dtype=np.int64)
+        check(lib().tdg_sizes(h, s))
+        self.bs, self.E, self.n, self.F, self.Nq, self.nq, self.dim = (int(x) for x in s)
+        self.Q = self.dim if kind == ELASTICITY else 1
+        self._material = False
+
+    # --- AbstractDGOperator interface
+    def block_size(self):
+        return self.bs
+
+    def num_local_elements(self):
+        return self.E
+
+    def number_of_local_dofs(self):
+        return self.n
+
+    # --- coefficient plumbing
+    def volume_points(self):
+        out = np.zeros(self.E * self.Nq * self.dim)
+        check(lib().tdg_volume_points(self._h, out))
+        return out.reshape(self.E, self.Nq, self.dim)
+
+    def facet_points(self):
+        out = np.zeros(self.F * self.nq * self.dim)
+        check(lib().tdg_facet_points(self._h, out))
+        return out.reshape(self.F, self.nq, self.dim)
+
+    def facets(self):
+        out = np.zeros(self.F * 5, dtype=np.int64)
+        check(lib().tdg_facets(self._h, out))
+        return out.reshape(self.F, 5)
+
+    def set_material(self, c0_Q, c1_Q=None):
+        c0 = np.ascontiguousarray(c0_Q, dtype=np.float64).reshape(self.E * self.Nq)
+        c1 = None if c1_Q is None else np.ascontiguousarray(c1_Q, dtype=np.float64).reshape(self.E * self.Nq)
+        check(lib().tdg_set_material(self._h, ptr(c0), ptr(c1)))
+        self._material = True
+
+    def set_stream(self, stream):
+        """stream: a torch.cuda.Stream (or None for the default stream)."""
+        check(lib().tdg_set_stream(self._h, None if stream is None else C.c_void_p(stream.cuda_stream)))
+
+    # --- apply
+    def apply(self, x, y):
+        """y = A x on GPU tensors (float64 or float32, contiguous, number_of_local_dofs() long)."""
+        torch = _torch()
+        assert x.is_cuda and y.is_cuda and x.dtype == y.dtype and x.is_contiguous() and y.is_contiguous()
+        assert x.numel() >= self.n and y.numel() >= self.n
+        if x.dtype == torch.float64:
+            check(lib().tdg_mult(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
+        elif x.dtype == torch.float32:
+            check(lib().tdg_mult_f32(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
+        else:
+            raise TypeError("apply: float64 or float32 tensors required")
+        return y
+
+    def apply_batched(self, X, Y):
+        """Y[:, c] = A X[:, c]; X, Y of shape (ncols, ld) row-major (each column contiguous)."""
+        torch = _torch()
+        assert X.shape == Y.shape and X.dim() == 2 and X.shape[1] >= self.n
+        assert X.is_contiguous() and Y.is_contiguous() and X.dtype == Y.dtype
+        f = lib().tdg_mult_batched if X.dtype == torch.float64 else lib().tdg_mult_batched_f32
+        check(f(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), X.shape[0], X.shape[1]))
+        return Y
+
+    def rhs(self, force_Q=None, dirichlet_q=None):
+        """FP64 right-hand side on the host (force: (E, Q, Nq); Dirichlet data: (F, Q, nq))."""
+        b = np.zeros(self.n)
+        f = None if force_Q is None else np.ascontiguousarray(force_Q, dtype=np.float64)
+        d = None if dirichlet_q is None else np.ascontiguousarray(dirichlet_q, dtype=np.float64)
+        check(lib().tdg_rhs(self._h, ptr(f), ptr(d), b))
+        return b
+
+    def apply_counts(self, precision_bytes=8):
+        b, f = C.c_double(), C.c_double()
+        check(lib().tdg_apply_counts(self._h, precision_bytes, C.byref(b), C.byref(f)))
+        return b.value, f.value
+
+    def host_array(self, name):
+        n = C.c_int64()
+        check(lib().tdg_host_array(self._h, name.encode(), None, C.byref(n)))
+        out = np.zeros(n.value)
+        check(lib().tdg_host_array(self._h, name.encode(), C.c_void_p(out.ctypes.data), C.byref(n)))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.tdg_destroy(h)
+            self._h = None
+
+
+class Elasticity(DGOperator):
+    """Linear elasticity (tandem Elasticity): lam_Q, mu_Q at the volume points, or callables f(x)."""
+
+    def __init__(self, mesh, degree, lam=None, mu=None, geometry_nodes=None):
+        super().__init__(mesh, ELASTICITY, degree, geometry_nodes)
+        if lam is not None:
+            X = self.volume_points()
+            lq = lam(X) if callable(lam) else lam
+            mq = mu(X) if callable(mu) else mu
+            self.set_material(np.broadcast_to(lq, X.shape[:2]), np.broadcast_to(mq, X.shape[:2]))
+
+
+class Poisson(DGOperator):
+    """Scalar diffusion (tandem Poisson, modal Dubiner basis): K_Q at the volume points or f(x)."""
+
+    def __init__(self, mesh, degree, K=None, geometry_nodes=None):
+        super().__init__(mesh, POISSON, degree, geometry_nodes)
+        if K is not None:
+            X = self.volume_points()
+            kq = K(X) if callable(K) else K
+            self.set_material(np.broadcast_to(kq, X.shape[:2]))

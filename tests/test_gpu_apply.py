@@ -1,0 +1,89 @@
+"""Parity of the HIP operator apply (through the C ABI) with the CPU FP64 oracle.
+
+Bar: relative L2 difference <= 1e-12 in FP64 (the north-star tolerance for the plain apply);
+<= 1e-5 for the FP32 kernels (FP32 copies of the FP64 precompute, FP32 arithmetic).
+"""
+import numpy as np
+import pytest
+
+from cases import BC_D, BC_N, Case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (dim, cells, scheme, degree, kind, bc)
+    (3, 2, 1, 4, 0, BC_D),          # P4 tet elasticity, Kuhn cubes (the c1 element type)
+    (3, 2, 0, 4, 0, [BC_D, BC_N, BC_N, BC_D, BC_D, BC_N]),  # tandem GenMesh 5-tet cubes, mixed BCs
+    (3, 2, 1, 1, 0, BC_D),
+    (3, 2, 1, 2, 0, BC_D),
+    (3, 2, 1, 3, 0, BC_D),
+    (2, 4, 0, 2, 0, BC_D),
+    (2, 3, 0, 6, 0, [BC_D, BC_N, BC_D, BC_N]),  # P6 triangles (the c3 element type)
+    (2, 32, 0, 2, 1, BC_D),         # c0: Poisson P2 on the 32x32x2 unit square
+    (2, 3, 0, 4, 1, [BC_D, BC_N, BC_N, BC_D]),
+    (3, 2, 1, 2, 1, BC_D),
+    (3, 2, 0, 4, 1, BC_D),
+]
+
+
+def _ids(c):
+    return "D%d-n%s-s%d-P%d-%s" % (c[0], c[1], c[2], c[3], "elast" if c[4] == 0 else "poisson")
+
+
+@pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
+def test_apply_matches_oracle(cuda, case):
+    import torch
+
+    D, n, scheme, N, kind, bc = case
+    c = Case(D, n, scheme, N, kind, bc=bc, seed=7)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-1.0, 1.0, c.n)
+    y_ref = c.oracle.apply(x, nthreads=4)
+    print("case", _ids(case), "n", c.n, flush=True)
+    xd = torch.from_numpy(x).to(cuda)
+    yd = torch.full_like(xd, float("nan"))
+    c.op.apply(xd, yd)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy()
+    assert np.all(np.isfinite(y))
+    err64 = rel_l2(y, y_ref)
+    xf = xd.float()
+    yf = torch.full_like(xf, float("nan"))
+    c.op.apply(xf, yf)
+    torch.cuda.synchronize()
+    err32 = rel_l2(yf.double().cpu().numpy(), y_ref)
+    print("  fp64 rel L2 %.3e  fp32 rel L2 %.3e" % (err64, err32), flush=True)
+    assert err64 <= 1e-12
+    assert err32 <= 1e-5
+
+
+def test_batched_apply_equals_columns(cuda):
+    import torch
+
+    c = Case(3, 2, 1, 4, 0, seed=3)
+    rng = np.random.default_rng(5)
+    ncols, ld = 5, c.n + 7
+    X = torch.from_numpy(rng.uniform(-1, 1, (ncols, ld))).to(cuda)
+    Y = torch.zeros_like(X)
+    c.op.apply_batched(X, Y)
+    for j in range(ncols):
+        y = torch.zeros(c.n, dtype=torch.float64, device=cuda)
+        c.op.apply(X[j, :c.n].contiguous(), y)
+        assert torch.equal(y, Y[j, :c.n])
+    torch.cuda.synchronize()
+    Xf, Yf = X.float(), torch.zeros(ncols, ld, dtype=torch.float32, device=cuda)
+    c.op.apply_batched(Xf, Yf)
+    y = torch.zeros(c.n, dtype=torch.float32, device=cuda)
+    c.op.apply(Xf[2, :c.n].contiguous(), y)
+    assert torch.equal(y, Yf[2, :c.n])
+
+
+def test_apply_is_deterministic(cuda):
+    import torch
+
+    c = Case(3, 2, 1, 4, 0, seed=4)
+    x = torch.from_numpy(np.random.default_rng(1).uniform(-1, 1, c.n)).to(cuda)
+    y1, y2 = torch.zeros_like(x), torch.zeros_like(x)
+    c.op.apply(x, y1)
+    c.op.apply(x, y2)
+    assert torch.equal(y1, y2)
