@@ -95,6 +95,31 @@ int tdg_set_stream(tdg_op* op, void* stream);
  * Elasticity::rhs_volume / rhs_boundary (Elasticity.cpp:521-696). */
 int tdg_rhs(tdg_op const* op, double const* force_Q, double const* dirichlet_q, double* b);
 
+/* Linear solve A x = b on the GPU (b, x device FP64, x overwritten, initial guess 0); replaces
+ * PetscLinearSolver::solve = KSPSolve with KSPCG on the shell matrix
+ * (app/common/PetscLinearSolver.h:42, PetscLinearSolver.cpp:22-32), without preconditioner.
+ * mode 0: FP64 CG.  mode 1: FP32 CG inner solves (to inner_rtol) inside an FP64 iterative
+ * refinement loop whose residual b - A x and update x += d are FP64.  rtol applies to the FP64
+ * true residual. */
+enum { TDG_SOLVE_FP64_CG = 0, TDG_SOLVE_FP32_CG_FP64_IR = 1 };
+typedef struct {
+    double rtol;
+    int max_iter;
+    int mode;
+    double inner_rtol;
+    int max_outer;
+    int check_every;
+} tdg_solve_opts;
+typedef struct {
+    int iterations;
+    int outer_iterations;
+    int inner_iterations;
+    double rel_residual;
+    double seconds;
+    int converged;
+} tdg_solve_result;
+int tdg_solve(tdg_op* op, double const* b, double* x, tdg_solve_opts const* opts, tdg_solve_result* result);
+
 /* Algorithmic bytes (for the given bytes per real) and flops of one apply. */
 int tdg_apply_counts(tdg_op const* op, int precision_bytes, double* bytes, double* flops);
 

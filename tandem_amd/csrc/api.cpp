@@ -4,6 +4,7 @@
 #include "host/mesh.h"
 #include "host/operator.h"
 #include "host/refelem.h"
+#include "solver/krylov.h"
 
 #include <hip/hip_runtime.h>
 
@@ -192,6 +193,31 @@ int tdg_rhs(tdg_op const* op, double const* force_Q, double const* dirichlet_q, 
     return guarded([&] {
         require(op && op->impl && b, "null argument");
         op->impl->rhs(force_Q, dirichlet_q, b);
+    });
+}
+
+int tdg_solve(tdg_op* op, double const* b, double* x, tdg_solve_opts const* opts, tdg_solve_result* result) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(b && x && opts && result && b != x, "bad solve arguments");
+        tdg::SolveOptions so;
+        so.rtol = opts->rtol;
+        so.max_iter = opts->max_iter;
+        so.mode = opts->mode;
+        so.inner_rtol = opts->inner_rtol;
+        so.max_outer = opts->max_outer;
+        so.check_every = opts->check_every;
+        require(so.mode == TDG_SOLVE_FP64_CG || so.mode == TDG_SOLVE_FP32_CG_FP64_IR, "unknown solve mode");
+        require(so.rtol > 0 && so.max_iter > 0 && so.check_every > 0 && so.max_outer > 0 &&
+                    (so.mode == TDG_SOLVE_FP64_CG || (so.inner_rtol > 0 && so.inner_rtol < 1)),
+                "bad solver options");
+        auto r = tdg::solve(o, b, x, so);
+        result->iterations = r.iterations;
+        result->outer_iterations = r.outer_iterations;
+        result->inner_iterations = r.inner_iterations;
+        result->rel_residual = r.rel_residual;
+        result->seconds = r.seconds;
+        result->converged = r.converged;
     });
 }
 

@@ -1,0 +1,31 @@
+// GPU Krylov solvers over the matrix-free operator.
+#pragma once
+
+namespace tdg {
+
+class Operator;
+
+enum SolveMode : int { Solve_FP64_CG = 0, Solve_FP32_CG_FP64_IR = 1 };
+
+struct SolveOptions {
+    double rtol = 1e-10;      // on the FP64 true residual ||b - A x|| / ||b||
+    int max_iter = 100000;    // total (inner) CG iterations
+    int mode = Solve_FP64_CG;
+    double inner_rtol = 1e-4; // FP32 inner solve tolerance (refinement mode)
+    int max_outer = 50;
+    int check_every = 10;     // host reads the residual every check_every iterations
+};
+
+struct SolveResult {
+    int iterations = 0;       // total CG iterations (inner for refinement)
+    int outer_iterations = 0;
+    int inner_iterations = 0;
+    double rel_residual = 0.0;
+    double seconds = 0.0;
+    int converged = 0;
+};
+
+// b, x: device FP64 vectors of op.local_size(); x is overwritten (initial guess 0).
+SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& opts);
+
+} // namespace tdg

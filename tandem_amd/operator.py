@@ -156,6 +156,21 @@ class DGOperator:
         check(f(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), X.shape[0], X.shape[1]))
         return Y
 
+    def upload(self):
+        """Make the operator device-resident now (otherwise the first apply does it)."""
+        check(lib().tdg_upload(self._h))
+
+    def solve(self, b, x, rtol=1e-10, mode="fp64", inner_rtol=1e-4, max_iter=200000, max_outer=60,
+              check_every=10):
+        """Solve A x = b (float64 GPU tensors) with CG ("fp64") or FP32 CG inside FP64 iterative
+        refinement ("fp32-ir").  Returns the solver statistics as a dict."""
+        assert b.is_cuda and x.is_cuda and b.dtype == x.dtype and b.numel() == self.n == x.numel()
+        opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ir": 1}[mode], inner_rtol, max_outer, check_every)
+        res = _lib.SolveResult()
+        check(lib().tdg_solve(self._h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), C.byref(opts),
+                              C.byref(res)))
+        return {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
+
     def rhs(self, force_Q=None, dirichlet_q=None):
         """FP64 right-hand side on the host (force: (E, Q, Nq); Dirichlet data: (F, Q, nq))."""
         b = np.zeros(self.n)
