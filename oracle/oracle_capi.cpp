@@ -217,6 +217,27 @@ int oracle_op_traction(void* h, int64_t fct, double const* x, double const* f_q,
         o->el->traction(fct, x, f_q, t);
     });
 }
+// Discrete solution at the volume quadrature points: out (E x Q x Nq) = sum_k E_Q(k,q) x(k,p).
+// Also returns the quadrature weights times |det J| (E x Nq) in wj when non-null.
+int oracle_op_eval_volume(void* h, double const* x, double* out, double* wj) {
+    return guard([&] {
+        auto o = static_cast<OpHandle*>(h);
+        auto& c = o->c();
+        unsigned Nbf = o->kind == 0 ? o->el->Nbf : o->po->Nbf;
+        unsigned Q = o->kind == 0 ? o->el->Q : 1;
+        auto const& E = o->kind == 0 ? o->el->E_Q : o->po->E_Q;
+        std::size_t Nq = c.volRule.size(), ne = c.nE(), bs = Nbf * Q;
+        for (std::size_t e = 0; e < ne; ++e)
+            for (unsigned p = 0; p < Q; ++p)
+                for (std::size_t q = 0; q < Nq; ++q) {
+                    double s = 0.0;
+                    for (unsigned k = 0; k < Nbf; ++k) s += E[k + Nbf * q] * x[e * bs + k + p * Nbf];
+                    out[(e * Q + p) * Nq + q] = s;
+                    if (wj && p == 0) wj[e * Nq + q] = c.volRule.wts[q] * c.absDetJ[e * Nq + q];
+                }
+    });
+}
+
 // facet table: F x 5 (up0, up1, localNo0, localNo1, bc)
 void oracle_op_facets(void* h, int64_t* out) {
     auto& c = static_cast<OpHandle*>(h)->c();

@@ -64,6 +64,7 @@ def lib():
     L.oracle_op_rhs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, dp]
     L.oracle_op_traction.argtypes = [C.c_void_p, C.c_int64, dp, C.c_void_p, dp]
     L.oracle_op_facets.argtypes = [C.c_void_p, ip64]
+    L.oracle_op_eval_volume.argtypes = [C.c_void_p, dp, dp, C.c_void_p]
     L.oracle_op_array.argtypes = [C.c_void_p, C.c_char_p, dp]
     L.oracle_op_array_size.restype = C.c_int64
     L.oracle_op_array_size.argtypes = [C.c_void_p, C.c_char_p]
@@ -261,6 +262,16 @@ class Operator:
         if lib().oracle_op_traction(self.h, fct, np.ascontiguousarray(x, dtype=np.float64), _ptr(fq), t) != 0:
             _err()
         return t.reshape(self.D, self.nq)
+
+    def eval_volume(self, x):
+        """Discrete solution at the volume points (E, Q, Nq) and the weights W|J| (E, Nq)."""
+        Q = self.D if self.kind == 0 else 1
+        out = np.zeros(self.E * Q * self.Nq)
+        wj = np.zeros(self.E * self.Nq)
+        if lib().oracle_op_eval_volume(self.h, np.ascontiguousarray(x, dtype=np.float64), out,
+                                       wj.ctypes.data_as(C.c_void_p)) != 0:
+            _err()
+        return out.reshape(self.E, Q, self.Nq), wj.reshape(self.E, self.Nq)
 
     def facets(self):
         out = np.zeros(self.F * 5, dtype=np.int64)
