@@ -122,15 +122,16 @@ def solve_comparison(op, mesh, nodes, fields, rtol=1e-10):
     bd = torch.from_numpy(b).cuda()
     out = {}
     xs = {}
-    for mode in ("fp64", "fp32-ir"):
+    for mode in ("fp64", "fp32-ru"):
         x = torch.zeros_like(bd)
-        r = op.solve(bd, x, rtol=rtol, mode=mode, inner_rtol=1e-4, max_iter=30000)
+        r = op.solve(bd, x, rtol=rtol, mode=mode, max_iter=40000)
         log("solve", mode, r)
-        out[mode] = {"seconds": r["seconds"], "iterations": r["iterations"], "outer": r["outer_iterations"],
+        out[mode] = {"seconds": r["seconds"], "iterations": r["iterations"], "fp64_replacements": r["outer_iterations"],
                      "rel_residual": r["rel_residual"], "converged": bool(r["converged"])}
         xs[mode] = x.cpu().numpy()
-    out["speedup_fp32ir_vs_fp64"] = out["fp64"]["seconds"] / out["fp32-ir"]["seconds"]
-    out["solution_rel_diff_fp32ir_vs_fp64"] = float(np.linalg.norm(xs["fp32-ir"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
+    out["speedup_fp32ru_vs_fp64"] = out["fp64"]["seconds"] / out["fp32-ru"]["seconds"]
+    out["solution_rel_diff_fp32ru_vs_fp64"] = float(np.linalg.norm(xs["fp32-ru"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
+    # discretisation error of the FP64 solution against the manufactured solution at the volume points
     return out
 
 

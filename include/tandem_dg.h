@@ -89,6 +89,13 @@ int tdg_mult_batched_f32(tdg_op* op, float const* X, float* Y, int ncols, int64_
 /* Stream all device work of this operator is queued on (hipStream_t; NULL = default stream). */
 int tdg_set_stream(tdg_op* op, void* stream);
 
+/* Kernel choice: TDG_VARIANT_AUTO (default) runs the matrix-core (MFMA) kernel where one is
+ * compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6) and the thread-per-point kernel
+ * otherwise; TDG_VARIANT_POINTS always runs the thread-per-point kernel.  Both compute the same
+ * operator (the reference's element-centric apply); they differ in summation order only. */
+enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1 };
+int tdg_set_variant(tdg_op* op, int variant);
+
 /* Right-hand side b (host, local size) from a body force at the volume points (elements x Q x Nq,
  * may be NULL) and Dirichlet data at the facet points (facets x Q x nq, only boundary Dirichlet
  * facets are read; may be NULL).  Replaces DGOperator::rhs (src/form/DGOperator.h:189-237) with
@@ -99,9 +106,11 @@ int tdg_rhs(tdg_op const* op, double const* force_Q, double const* dirichlet_q, 
  * PetscLinearSolver::solve = KSPSolve with KSPCG on the shell matrix
  * (app/common/PetscLinearSolver.h:42, PetscLinearSolver.cpp:22-32), without preconditioner.
  * mode 0: FP64 CG.  mode 1: FP32 CG inner solves (to inner_rtol) inside an FP64 iterative
- * refinement loop whose residual b - A x and update x += d are FP64.  rtol applies to the FP64
- * true residual. */
-enum { TDG_SOLVE_FP64_CG = 0, TDG_SOLVE_FP32_CG_FP64_IR = 1 };
+ * refinement loop whose residual b - A x and update x += d are FP64 (restarted each refinement).
+ * mode 2: FP32 CG whose residual is replaced by the FP64 true residual, and whose accumulated FP32
+ * update is added to the FP64 x, each time the residual drops by reliable_delta (refinement
+ * without restarting the Krylov direction).  rtol applies to the FP64 true residual. */
+enum { TDG_SOLVE_FP64_CG = 0, TDG_SOLVE_FP32_CG_FP64_IR = 1, TDG_SOLVE_FP32_CG_FP64_RELIABLE = 2 };
 typedef struct {
     double rtol;
     int max_iter;
@@ -109,6 +118,7 @@ typedef struct {
     double inner_rtol;
     int max_outer;
     int check_every;
+    double reliable_delta;
 } tdg_solve_opts;
 typedef struct {
     int iterations;

@@ -24,7 +24,7 @@ class TdgError(RuntimeError):
 
 class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("max_iter", C.c_int), ("mode", C.c_int), ("inner_rtol", C.c_double),
-                ("max_outer", C.c_int), ("check_every", C.c_int)]
+                ("max_outer", C.c_int), ("check_every", C.c_int), ("reliable_delta", C.c_double)]
 
 
 class SolveResult(C.Structure):
@@ -50,7 +50,7 @@ class MeshDesc(C.Structure):
 EXPORTS = [
     "tdg_create", "tdg_destroy", "tdg_last_error", "tdg_sizes", "tdg_volume_points", "tdg_facet_points",
     "tdg_facets", "tdg_set_material", "tdg_upload", "tdg_mult", "tdg_mult_f32", "tdg_mult_batched", "tdg_mult_batched_f32",
-    "tdg_set_stream", "tdg_solve", "tdg_rhs", "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
+    "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_rhs", "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
 ]
 
 
@@ -77,6 +77,7 @@ def lib():
     L.tdg_mult_batched.argtypes = [vp, vp, vp, C.c_int, C.c_int64]
     L.tdg_mult_batched_f32.argtypes = [vp, vp, vp, C.c_int, C.c_int64]
     L.tdg_set_stream.argtypes = [vp, vp]
+    L.tdg_set_variant.argtypes = [vp, C.c_int]
     L.tdg_rhs.argtypes = [vp, vp, vp, _dp]
     L.tdg_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]
     L.tdg_apply_counts.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]

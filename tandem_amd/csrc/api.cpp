@@ -189,6 +189,14 @@ int tdg_set_stream(tdg_op* op, void* stream) {
     });
 }
 
+int tdg_set_variant(tdg_op* op, int variant) {
+    return guarded([&] {
+        require(op && op->impl, "null operator");
+        require(variant == TDG_VARIANT_AUTO || variant == TDG_VARIANT_POINTS, "unknown kernel variant");
+        op->impl->set_variant(variant);
+    });
+}
+
 int tdg_rhs(tdg_op const* op, double const* force_Q, double const* dirichlet_q, double* b) {
     return guarded([&] {
         require(op && op->impl && b, "null argument");
@@ -207,9 +215,13 @@ int tdg_solve(tdg_op* op, double const* b, double* x, tdg_solve_opts const* opts
         so.inner_rtol = opts->inner_rtol;
         so.max_outer = opts->max_outer;
         so.check_every = opts->check_every;
-        require(so.mode == TDG_SOLVE_FP64_CG || so.mode == TDG_SOLVE_FP32_CG_FP64_IR, "unknown solve mode");
+        so.reliable_delta = opts->reliable_delta;
+        require(so.mode == TDG_SOLVE_FP64_CG || so.mode == TDG_SOLVE_FP32_CG_FP64_IR ||
+                    so.mode == TDG_SOLVE_FP32_CG_FP64_RELIABLE,
+                "unknown solve mode");
         require(so.rtol > 0 && so.max_iter > 0 && so.check_every > 0 && so.max_outer > 0 &&
-                    (so.mode == TDG_SOLVE_FP64_CG || (so.inner_rtol > 0 && so.inner_rtol < 1)),
+                    (so.mode != TDG_SOLVE_FP32_CG_FP64_IR || (so.inner_rtol > 0 && so.inner_rtol < 1)) &&
+                    (so.mode != TDG_SOLVE_FP32_CG_FP64_RELIABLE || (so.reliable_delta > 0 && so.reliable_delta < 1)),
                 "bad solver options");
         auto r = tdg::solve(o, b, x, so);
         result->iterations = r.iterations;

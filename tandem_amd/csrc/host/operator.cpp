@@ -9,11 +9,15 @@
 // are rounded from it.
 #include "operator.h"
 
+#include "../kernels/kernels.h"
+#include "../kernels/mfma_layout.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
+#include <type_traits>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -369,19 +373,138 @@ void Operator::set_material(double const* c0_Q, double const* c1_Q) {
     if (E_ >= (std::size_t(1) << 31) || F >= (std::size_t(1) << 31)) throw std::runtime_error("mesh too large for 32-bit indices");
     d64_.reset();
     d32_.reset();
+    m64_.reset();
+    m32_.reset();
     ready_ = true;
 }
+
+template <class T> static std::vector<T> cast(std::vector<double> const& v);
 
 // Device residency of both precisions (first apply after set_material, or an explicit call).
 void Operator::upload_device() {
     if (!ready_) throw std::logic_error("set_material() must be called before the device upload");
-    if (d64_ && d32_) return;
-    auto a = std::make_unique<DeviceOperator<double>>();
-    auto b = std::make_unique<DeviceOperator<float>>();
-    upload(*a);
-    upload(*b);
-    d64_ = std::move(a);
-    d32_ = std::move(b);
+    if (!d64_ || !d32_) {
+        auto a = std::make_unique<DeviceOperator<double>>();
+        auto b = std::make_unique<DeviceOperator<float>>();
+        upload(*a);
+        upload(*b);
+        d64_ = std::move(a);
+        d32_ = std::move(b);
+    }
+    if (uses_mfma() && (!m64_ || !m32_)) {
+        auto a = std::make_unique<DeviceMfma<double>>();
+        auto b = std::make_unique<DeviceMfma<float>>();
+        upload_mfma(*a);
+        upload_mfma(*b);
+        m64_ = std::move(a);
+        m32_ = std::move(b);
+    }
+}
+
+void Operator::set_variant(int v) {
+    if (v != Variant_Auto && v != Variant_Points) throw std::invalid_argument("unknown kernel variant");
+    variant_ = v;
+}
+
+bool Operator::uses_mfma() const {
+    return variant_ == Variant_Auto && kind_ == Kind_Elasticity && mfma_supported(D_, N_);
+}
+
+// Operands and point data of the MFMA kernel (see kernels/dg_apply_mfma.hip for the layouts).
+template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
+    const unsigned D = D_, DD = D * D, NF = D + 1;
+    const MfmaDims L{int(D), int(Nbf_), int(Nq_), int(nq_)};
+    auto T0 = [&](int c, int jb, int q) -> double {
+        if (c > int(D)) return 0.0;
+        return c == 0 ? E_q_[0][jb + Nbf_ * q] : Dxi_q_[0][jb + Nbf_ * ((c - 1) + D * q)];
+    };
+    std::vector<double> A1(L.a1(), 0.0), A2(L.a2(), 0.0), AF(L.af(), 0.0), AL(L.al(), 0.0);
+    for (int e = 0; e < int(D); ++e)
+        for (int b = 0; b < L.QB; ++b)
+            for (int s = 0; s < L.KS; ++s)
+                for (int lane = 0; lane < 64; ++lane) {
+                    int q = 16 * b + lane % 16, k = 4 * s + lane / 16;
+                    if (q < int(Nq_) && k < int(Nbf_))
+                        A1[((e * L.QB + b) * L.KS + s) * 64 + lane] = Dxi_Q_[k + Nbf_ * (e + D * q)];
+                }
+    for (int m = 0; m < L.MT; ++m)
+        for (int e = 0; e < int(D); ++e)
+            for (int b = 0; b < L.QB; ++b)
+                for (int i = 0; i < 4; ++i)
+                    for (int lane = 0; lane < 64; ++lane) {
+                        // K sub-index lane/16 = the lane group whose register i holds this point
+                        int k = 16 * m + lane % 16, q = 16 * b + mfma_row<T>(lane / 16, i);
+                        if (q < int(Nq_) && k < int(Nbf_))
+                            A2[(m * (int(D) * L.QB * 4) + (e * L.QB + b) * 4 + i) * 64 + lane] =
+                                Dxi_Q_[k + Nbf_ * (e + D * q)];
+                    }
+    for (unsigned f = 0; f < NF; ++f) {
+        for (int t = 0; t < L.FT; ++t)
+            for (int s = 0; s < L.KS; ++s)
+                for (int lane = 0; lane < 64; ++lane) {
+                    // tile row rho lands in lane group mfma_group(rho), register mfma_reg(rho):
+                    // make that (facet point 4t + group, channel = register)
+                    int row = lane % 16, q = 4 * t + mfma_group<T>(row), c = mfma_reg<T>(row), k = 4 * s + lane / 16;
+                    if (q < int(nq_) && k < int(Nbf_))
+                        AF[((f * L.FT + t) * L.KS + s) * 64 + lane] = T0(c, iperm_[f * Nbf_ + k], q);
+                }
+        for (int m = 0; m < L.MT; ++m)
+            for (int t = 0; t < L.FT; ++t)
+                for (int c = 0; c < 4; ++c)
+                    for (int lane = 0; lane < 64; ++lane) {
+                        int k = 16 * m + lane % 16, q = 4 * t + lane / 16;
+                        if (q < int(nq_) && k < int(Nbf_))
+                            AL[((f * L.MT + m) * (L.FT * 4) + t * 4 + c) * 64 + lane] = T0(c, iperm_[f * Nbf_ + k], q);
+                    }
+    }
+    // element-grouped point data
+    const std::size_t ng = (E_ + 15) / 16;
+    std::vector<T> VD(ng * Nq_ * L.NCH * 16, T(0)), FD(ng * NF * nq_ * L.NFC * 16, T(0)), FP(ng * NF * 16, T(0));
+    std::vector<int32_t> FI(ng * NF * 16 * 4, 0);
+#pragma omp parallel for schedule(static)
+    for (std::size_t e = 0; e < E_; ++e) {
+        std::size_t grp = e / 16, j = e % 16;
+        for (unsigned q = 0; q < Nq_; ++q) {
+            T* o = &VD[((grp * Nq_ + q) * L.NCH) * 16 + j];
+            for (unsigned c = 0; c < DD; ++c) o[c * 16] = T(JInv_[(e * Nq_ + q) * DD + c]);
+            o[DD * 16] = T(c0Q_[e * Nq_ + q]);
+            o[(DD + 1) * 16] = T(c1Q_[e * Nq_ + q]);
+        }
+        for (unsigned f = 0; f < NF; ++f) {
+            int32_t const* ef = &ef_[(e * NF + f) * 4];
+            int64_t fct = ef[0];
+            int side = (ef[2] >> 2) & 1;
+            int32_t* fi = &FI[((grp * NF + f) * 16 + j) * 4];
+            fi[0] = (ef[2] & 3) | (((ef[2] >> 4) & 3) << 4);
+            fi[1] = ef[1];
+            FP[(grp * NF + f) * 16 + j] = T(penalty_[fct]);
+            for (unsigned q = 0; q < nq_; ++q) {
+                T* o = &FD[(((grp * NF + f) * nq_ + q) * L.NFC) * 16 + j];
+                double const* g0 = &fGp_[((fct * 2 + side) * nq_ + q) * DD];
+                double const* g1 = &fGp_[((fct * 2 + (1 - side)) * nq_ + q) * DD];
+                for (unsigned c = 0; c < DD; ++c) {
+                    o[c * 16] = T(g0[c]);
+                    o[(DD + 2 + c) * 16] = T(g1[c]);
+                }
+                o[DD * 16] = T(fc0_[(fct * 2 + side) * nq_ + q]);
+                o[(DD + 1) * 16] = T(fc1_[(fct * 2 + side) * nq_ + q]);
+                o[(2 * DD + 2) * 16] = T(fc0_[(fct * 2 + (1 - side)) * nq_ + q]);
+                o[(2 * DD + 3) * 16] = T(fc1_[(fct * 2 + (1 - side)) * nq_ + q]);
+                double sg = side ? -1.0 : 1.0;
+                for (unsigned d = 0; d < D; ++d) o[(2 * DD + 4 + d) * 16] = T(sg * normal_[(fct * nq_ + q) * D + d]);
+            }
+        }
+    }
+    dm.A1.upload(cast<T>(A1));
+    dm.A2.upload(cast<T>(A2));
+    dm.AF.upload(cast<T>(AF));
+    dm.AL.upload(cast<T>(AL));
+    dm.w.upload(cast<T>(fct_.weights));
+    dm.perm.upload(perm_);
+    dm.VD.upload(VD);
+    dm.FD.upload(FD);
+    dm.FP.upload(FP);
+    dm.FI.upload(FI);
 }
 
 template <class T> static std::vector<T> cast(std::vector<double> const& v) {
@@ -537,8 +660,6 @@ double Operator::apply_flops() const {
 
 } // namespace tdg
 
-#include "../kernels/kernels.h"
-
 namespace tdg {
 
 template <class T>
@@ -579,13 +700,41 @@ template <> DeviceOperator<float> const& Operator::device<float>() {
     return *d32_;
 }
 
-void Operator::apply(double const* x, double* y) { launch(device<double>(), x, y, 1, 0); }
-void Operator::apply(float const* x, float* y) { launch(device<float>(), x, y, 1, 0); }
-void Operator::apply_batched(double const* X, double* Y, int ncols, int64_t ld) {
-    launch(device<double>(), X, Y, ncols, ld);
+template <class T>
+void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld) {
+    MfmaArgs<T> a;
+    a.A1 = d.A1.ptr;
+    a.A2 = d.A2.ptr;
+    a.AF = d.AF.ptr;
+    a.AL = d.AL.ptr;
+    a.w = d.w.ptr;
+    a.perm = d.perm.ptr;
+    a.VD = d.VD.ptr;
+    a.FD = d.FD.ptr;
+    a.FP = d.FP.ptr;
+    a.FI = reinterpret_cast<int4 const*>(d.FI.ptr);
+    a.num_elements = int64_t(E_);
+    a.num_groups = int64_t((E_ + 15) / 16);
+    int kq = 0, kf = 0;
+    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
+        throw std::runtime_error("no kernel compiled for this dimension/degree");
+    TDG_HIP(launch_elasticity_mfma<T>(D_, N_, a, x, y, ncols, ld, stream_));
 }
-void Operator::apply_batched(float const* X, float* Y, int ncols, int64_t ld) {
-    launch(device<float>(), X, Y, ncols, ld);
+
+template <class T> void Operator::dispatch(T const* x, T* y, int ncols, int64_t ld) {
+    if (!ready_) throw std::logic_error("set_material() must be called before apply");
+    upload_device();
+    if (uses_mfma()) {
+        if constexpr (std::is_same<T, double>::value) launch_mfma(*m64_, x, y, ncols, ld);
+        else launch_mfma(*m32_, x, y, ncols, ld);
+    } else {
+        launch(device<T>(), x, y, ncols, ld);
+    }
 }
+
+void Operator::apply(double const* x, double* y) { dispatch(x, y, 1, 0); }
+void Operator::apply(float const* x, float* y) { dispatch(x, y, 1, 0); }
+void Operator::apply_batched(double const* X, double* Y, int ncols, int64_t ld) { dispatch(X, Y, ncols, ld); }
+void Operator::apply_batched(float const* X, float* Y, int ncols, int64_t ld) { dispatch(X, Y, ncols, ld); }
 
 } // namespace tdg

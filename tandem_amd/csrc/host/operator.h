@@ -42,6 +42,16 @@ template <class T> struct DeviceOperator {
     DeviceArray<int32_t> ef;                   // per element face: facet, neighbour, flags
 };
 
+// Operand tables and element-grouped point data of the MFMA elasticity kernel (one precision).
+template <class T> struct DeviceMfma {
+    DeviceArray<T> A1, A2, AF, AL, w, VD, FD, FP;
+    DeviceArray<int32_t> perm, FI;
+};
+
+// Kernel selection: Variant_Auto uses the MFMA kernel where one is compiled (elasticity), the
+// thread-per-point kernel otherwise; Variant_Points forces the thread-per-point kernel.
+enum Variant : int { Variant_Auto = 0, Variant_Points = 1 };
+
 class Operator {
 public:
     Operator(SimplexMesh mesh, int kind, unsigned degree, std::vector<double> const* geometry_nodes);
@@ -73,6 +83,9 @@ public:
 
     void set_stream(hipStream_t s) { stream_ = s; }
     hipStream_t stream() const { return stream_; }
+    void set_variant(int v);
+    int variant() const { return variant_; }
+    bool uses_mfma() const;
 
     // y = A x on device buffers (x, y of local_size() entries).
     void apply(double const* x, double* y);
@@ -95,7 +108,10 @@ private:
     void build_tables();
     void build_geometry(std::vector<double> const* geometry_nodes);
     template <class T> void upload(DeviceOperator<T>& d);
+    template <class T> void upload_mfma(DeviceMfma<T>& d);
     template <class T> void launch(DeviceOperator<T> const& d, T const* x, T* y, int ncols, int64_t ld);
+    template <class T> void launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld);
+    template <class T> void dispatch(T const* x, T* y, int ncols, int64_t ld);
 
     int kind_;
     unsigned D_, N_, Q_, Nbf_, Nq_, nq_;
@@ -129,6 +145,9 @@ private:
 
     std::unique_ptr<DeviceOperator<double>> d64_;
     std::unique_ptr<DeviceOperator<float>> d32_;
+    std::unique_ptr<DeviceMfma<double>> m64_;
+    std::unique_ptr<DeviceMfma<float>> m32_;
+    int variant_ = Variant_Auto;
 };
 
 } // namespace tdg

@@ -5,15 +5,16 @@ namespace tdg {
 
 class Operator;
 
-enum SolveMode : int { Solve_FP64_CG = 0, Solve_FP32_CG_FP64_IR = 1 };
+enum SolveMode : int { Solve_FP64_CG = 0, Solve_FP32_CG_FP64_IR = 1, Solve_FP32_CG_FP64_RELIABLE = 2 };
 
 struct SolveOptions {
     double rtol = 1e-10;      // on the FP64 true residual ||b - A x|| / ||b||
     int max_iter = 100000;    // total (inner) CG iterations
     int mode = Solve_FP64_CG;
-    double inner_rtol = 1e-4; // FP32 inner solve tolerance (refinement mode)
+    double inner_rtol = 1e-4; // FP32 inner solve tolerance (restarted refinement mode)
     int max_outer = 50;
     int check_every = 10;     // host reads the residual every check_every iterations
+    double reliable_delta = 0.1; // residual drop that triggers an FP64 replacement (mode 2)
 };
 
 struct SolveResult {

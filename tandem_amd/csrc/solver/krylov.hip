@@ -247,6 +247,65 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
         res.outer_iterations = outer;
         res.inner_iterations = inner_total;
         res.iterations = inner_total;
+    } else if (o.mode == Solve_FP32_CG_FP64_RELIABLE) {
+        // FP32 CG whose residual is replaced by the FP64 true residual b - A x (and the FP32 update
+        // folded into the FP64 x) whenever it has dropped by `reliable_delta` since the last
+        // replacement (reliable updates; Sleijpen & van der Vorst 1996, as used for mixed-precision
+        // CG by Clark et al. 2010).  The search direction is kept across replacements, so the
+        // iteration count stays close to FP64 CG while all operator applies but the replacements
+        // run in FP32.
+        Buf<double> r, Ax;
+        Buf<float> r32, xa32, p32, q32;
+        r.alloc(n);
+        Ax.alloc(n);
+        r32.alloc(n);
+        xa32.alloc(n);
+        p32.alloc(n);
+        q32.alloc(n);
+        hipLaunchKernelGGL((fill<double>), g, bl, 0, st, x, 0.0, n);
+        hipLaunchKernelGGL((fill<double>), g, bl, 0, st, Ax.p, 0.0, n);
+        hipLaunchKernelGGL((fill<float>), g, bl, 0, st, xa32.p, 0.0f, n);
+        hipLaunchKernelGGL(residual, g, bl, 0, st, b, Ax.p, r.p, r32.p, n, partial.p);
+        hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + 0);
+        hipLaunchKernelGGL((copy_to<float>), g, bl, 0, st, p32.p, r32.p, n);
+        KHIP(hipGetLastError());
+        double rr_true = read_scalar(s.p + 0, st);
+        double max_r = std::sqrt(rr_true);
+        int it = 0, cur = 0, updates = 0;
+        while (it < o.max_iter) {
+            op.apply(p32.p, q32.p);
+            hipLaunchKernelGGL((dot_partial<float>), g, bl, 0, st, p32.p, q32.p, n, partial.p);
+            hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + 2);
+            hipLaunchKernelGGL((cg_xr<float, float>), g, bl, 0, st, xa32.p, r32.p, p32.p, q32.p, n, s.p, cur, 2,
+                               partial.p);
+            hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + (1 - cur));
+            ++it;
+            bool replace = false;
+            if (it % o.check_every == 0 || it == o.max_iter) {
+                KHIP(hipGetLastError());
+                double rr_it = read_scalar(s.p + (1 - cur), st);
+                if (!std::isfinite(rr_it)) throw std::runtime_error("CG breakdown (non-finite residual)");
+                replace = std::sqrt(rr_it) < o.reliable_delta * max_r || rr_it <= target || it == o.max_iter;
+            }
+            if (replace) {
+                hipLaunchKernelGGL(add_update, g, bl, 0, st, x, xa32.p, n);
+                hipLaunchKernelGGL((fill<float>), g, bl, 0, st, xa32.p, 0.0f, n);
+                op.apply(x, Ax.p);
+                hipLaunchKernelGGL(residual, g, bl, 0, st, b, Ax.p, r.p, r32.p, n, partial.p);
+                hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + (1 - cur));
+                KHIP(hipGetLastError());
+                rr_true = read_scalar(s.p + (1 - cur), st);
+                max_r = std::sqrt(rr_true);
+                ++updates;
+                if (rr_true <= target) break;
+            }
+            hipLaunchKernelGGL((cg_p<float>), g, bl, 0, st, p32.p, r32.p, n, s.p, 1 - cur, cur);
+            cur = 1 - cur;
+        }
+        res.rel_residual = std::sqrt(rr_true / bb);
+        res.iterations = it;
+        res.inner_iterations = it;
+        res.outer_iterations = updates;
     } else {
         throw std::invalid_argument("unknown solver mode");
     }

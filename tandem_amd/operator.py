@@ -129,6 +129,10 @@ class DGOperator:
         check(lib().tdg_set_material(self._h, ptr(c0), ptr(c1)))
         self._material = True
 
+    def set_variant(self, variant):
+        """"auto" (matrix-core kernel where compiled) or "points" (thread-per-point kernel)."""
+        check(lib().tdg_set_variant(self._h, {"auto": 0, "points": 1}[variant]))
+
     def set_stream(self, stream):
         """stream: a torch.cuda.Stream (or None for the default stream)."""
         check(lib().tdg_set_stream(self._h, None if stream is None else C.c_void_p(stream.cuda_stream)))
@@ -161,11 +165,13 @@ class DGOperator:
         check(lib().tdg_upload(self._h))
 
     def solve(self, b, x, rtol=1e-10, mode="fp64", inner_rtol=1e-4, max_iter=200000, max_outer=60,
-              check_every=10):
-        """Solve A x = b (float64 GPU tensors) with CG ("fp64") or FP32 CG inside FP64 iterative
-        refinement ("fp32-ir").  Returns the solver statistics as a dict."""
+              check_every=10, reliable_delta=0.1):
+        """Solve A x = b (float64 GPU tensors).  mode "fp64": FP64 CG; "fp32-ir": FP32 CG solves
+        restarted inside FP64 iterative refinement; "fp32-ru": FP32 CG with FP64 residual
+        replacement (refinement that keeps the Krylov direction).  Returns the statistics."""
         assert b.is_cuda and x.is_cuda and b.dtype == x.dtype and b.numel() == self.n == x.numel()
-        opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ir": 1}[mode], inner_rtol, max_outer, check_every)
+        opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ir": 1, "fp32-ru": 2}[mode], inner_rtol, max_outer,
+                              check_every, reliable_delta)
         res = _lib.SolveResult()
         check(lib().tdg_solve(self._h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), C.byref(opts),
                               C.byref(res)))

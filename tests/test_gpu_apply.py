@@ -30,16 +30,20 @@ def _ids(c):
     return "D%d-n%s-s%d-P%d-%s" % (c[0], c[1], c[2], c[3], "elast" if c[4] == 0 else "poisson")
 
 
+@pytest.mark.parametrize("variant", ["auto", "points"])
 @pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
-def test_apply_matches_oracle(cuda, case):
+def test_apply_matches_oracle(cuda, case, variant):
+    """variant "auto" runs the matrix-core kernel where one is compiled (elasticity 3D P2-P4,
+    2D P2/P4/P6), "points" the thread-per-point kernel."""
     import torch
 
     D, n, scheme, N, kind, bc = case
     c = Case(D, n, scheme, N, kind, bc=bc, seed=7)
+    c.op.set_variant(variant)
     rng = np.random.default_rng(11)
     x = rng.uniform(-1.0, 1.0, c.n)
     y_ref = c.oracle.apply(x, nthreads=4)
-    print("case", _ids(case), "n", c.n, flush=True)
+    print("case", _ids(case), variant, "n", c.n, flush=True)
     xd = torch.from_numpy(x).to(cuda)
     yd = torch.full_like(xd, float("nan"))
     c.op.apply(xd, yd)

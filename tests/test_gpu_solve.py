@@ -27,7 +27,7 @@ def test_cg_matches_dense_solve(cuda):
     b = np.random.default_rng(3).uniform(-1, 1, c.n)
     A = c.oracle.assemble()
     x_ref = np.linalg.solve(A, b)
-    for mode in ("fp64", "fp32-ir"):
+    for mode in ("fp64", "fp32-ir", "fp32-ru"):
         x, r = _solve(c.op, b, mode, rtol=1e-10)
         print(mode, r, flush=True)
         assert r["converged"] and r["rel_residual"] <= 1e-10
@@ -46,7 +46,8 @@ def test_poisson_c0_solve(cuda):
     g = np.zeros((op.F, 1, op.nq))
     b = op.rhs(f[:, None, :], g)
     x64, r64 = _solve(op, b, "fp64", rtol=1e-10)
-    x32, r32 = _solve(op, b, "fp32-ir", rtol=1e-10)
-    print("fp64", r64, "fp32-ir", r32, flush=True)
-    assert r64["converged"] and r32["converged"]
-    assert rel_l2(x32, x64) <= 1e-8
+    for mode in ("fp32-ir", "fp32-ru"):
+        x32, r32 = _solve(op, b, mode, rtol=1e-10)
+        print("fp64", r64, mode, r32, flush=True)
+        assert r64["converged"] and r32["converged"]
+        assert rel_l2(x32, x64) <= 1e-8
