@@ -30,7 +30,7 @@ SEED = 20261203
 PEAK_FP64 = 78.6e12   # MI355X dense FP64 (matrix and vector), FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12     # bytes/s
-KERNEL = "elasticity_apply_kernel<3, 4, double>"
+KERNEL = "elasticity_mfma_kernel<3, 4, double>"
 
 
 def log(*a):
