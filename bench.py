@@ -143,6 +143,9 @@ def main():
     ap.add_argument("--cells", type=int, default=24)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy kernel (y = -x over BYTES of float64) --steps times; "
+                         "its FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the apply kernel")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the operator is not sharded across GPUs); "
@@ -153,6 +156,15 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU available", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        n = args.copy_probe // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(args.steps):
+            torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": args.steps}))
+        return
     t0 = time.perf_counter()
     op, mesh, nodes, fields = build_workload(args.cells)
     op.upload()

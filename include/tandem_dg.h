@@ -89,11 +89,13 @@ int tdg_mult_batched_f32(tdg_op* op, float const* X, float* Y, int ncols, int64_
 /* Stream all device work of this operator is queued on (hipStream_t; NULL = default stream). */
 int tdg_set_stream(tdg_op* op, void* stream);
 
-/* Kernel choice: TDG_VARIANT_AUTO (default) runs the matrix-core (MFMA) kernel where one is
- * compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6) and the thread-per-point kernel
- * otherwise; TDG_VARIANT_POINTS always runs the thread-per-point kernel.  Both compute the same
- * operator (the reference's element-centric apply); they differ in summation order only. */
-enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1 };
+/* Kernel choice: TDG_VARIANT_AUTO (default) runs the matrix-core (MFMA) kernel with its tables
+ * staged in LDS where one is compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6) and the
+ * thread-per-point kernel otherwise; TDG_VARIANT_POINTS always runs the thread-per-point kernel;
+ * TDG_VARIANT_MFMA_TABLES the MFMA kernel reading pre-swizzled operand tables from memory.  All
+ * compute the same operator (the reference's element-centric apply); they differ in summation
+ * order only. */
+enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1, TDG_VARIANT_MFMA_TABLES = 2 };
 int tdg_set_variant(tdg_op* op, int variant);
 
 /* Right-hand side b (host, local size) from a body force at the volume points (elements x Q x Nq,

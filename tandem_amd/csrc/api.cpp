@@ -192,7 +192,8 @@ int tdg_set_stream(tdg_op* op, void* stream) {
 int tdg_set_variant(tdg_op* op, int variant) {
     return guarded([&] {
         require(op && op->impl, "null operator");
-        require(variant == TDG_VARIANT_AUTO || variant == TDG_VARIANT_POINTS, "unknown kernel variant");
+        require(variant == TDG_VARIANT_AUTO || variant == TDG_VARIANT_POINTS || variant == TDG_VARIANT_MFMA_TABLES,
+                "unknown kernel variant");
         op->impl->set_variant(variant);
     });
 }

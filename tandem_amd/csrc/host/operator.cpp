@@ -402,12 +402,13 @@ void Operator::upload_device() {
 }
 
 void Operator::set_variant(int v) {
-    if (v != Variant_Auto && v != Variant_Points) throw std::invalid_argument("unknown kernel variant");
+    if (v != Variant_Auto && v != Variant_Points && v != Variant_MfmaRegs)
+        throw std::invalid_argument("unknown kernel variant");
     variant_ = v;
 }
 
 bool Operator::uses_mfma() const {
-    return variant_ == Variant_Auto && kind_ == Kind_Elasticity && mfma_supported(D_, N_);
+    return variant_ != Variant_Points && kind_ == Kind_Elasticity && mfma_supported(D_, N_);
 }
 
 // Operands and point data of the MFMA kernel (see kernels/dg_apply_mfma.hip for the layouts).
@@ -419,6 +420,16 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
         return c == 0 ? E_q_[0][jb + Nbf_ * q] : Dxi_q_[0][jb + Nbf_ * ((c - 1) + D * q)];
     };
     std::vector<double> A1(L.a1(), 0.0), A2(L.a2(), 0.0), AF(L.af(), 0.0), AL(L.al(), 0.0);
+    // canonical tables: V[e][q][k] and T0[c][q][k] (c <= D)
+    std::vector<double> V(std::size_t(D) * Nq_ * Nbf_), TT(std::size_t(NF) * nq_ * Nbf_);
+    for (unsigned e = 0; e < D; ++e)
+        for (unsigned q = 0; q < Nq_; ++q)
+            for (unsigned k = 0; k < Nbf_; ++k) V[(e * Nq_ + q) * Nbf_ + k] = Dxi_Q_[k + Nbf_ * (e + D * q)];
+    for (unsigned c = 0; c <= D; ++c)
+        for (unsigned q = 0; q < nq_; ++q)
+            for (unsigned k = 0; k < Nbf_; ++k) TT[(c * nq_ + q) * Nbf_ + k] = T0(int(c), int(k), int(q));
+    std::vector<int32_t> P2(perm_);
+    P2.insert(P2.end(), iperm_.begin(), iperm_.end());
     for (int e = 0; e < int(D); ++e)
         for (int b = 0; b < L.QB; ++b)
             for (int s = 0; s < L.KS; ++s)
@@ -495,6 +506,9 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
             }
         }
     }
+    dm.V.upload(cast<T>(V));
+    dm.T0.upload(cast<T>(TT));
+    dm.perm2.upload(P2);
     dm.A1.upload(cast<T>(A1));
     dm.A2.upload(cast<T>(A2));
     dm.AF.upload(cast<T>(AF));
@@ -718,7 +732,22 @@ void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, 
     int kq = 0, kf = 0;
     if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
         throw std::runtime_error("no kernel compiled for this dimension/degree");
-    TDG_HIP(launch_elasticity_mfma<T>(D_, N_, a, x, y, ncols, ld, stream_));
+    if (variant_ == Variant_MfmaRegs) {
+        TDG_HIP(launch_elasticity_mfma<T>(D_, N_, a, x, y, ncols, ld, stream_));
+        return;
+    }
+    MfmaLdsArgs<T> b;
+    b.V = d.V.ptr;
+    b.T0 = d.T0.ptr;
+    b.perm = d.perm2.ptr;
+    b.w = d.w.ptr;
+    b.VD = d.VD.ptr;
+    b.FD = d.FD.ptr;
+    b.FP = d.FP.ptr;
+    b.FI = a.FI;
+    b.num_elements = a.num_elements;
+    b.num_groups = a.num_groups;
+    TDG_HIP(launch_elasticity_lds<T>(D_, N_, b, x, y, ncols, ld, stream_));
 }
 
 template <class T> void Operator::dispatch(T const* x, T* y, int ncols, int64_t ld) {

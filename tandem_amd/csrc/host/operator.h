@@ -45,12 +45,15 @@ template <class T> struct DeviceOperator {
 // Operand tables and element-grouped point data of the MFMA elasticity kernel (one precision).
 template <class T> struct DeviceMfma {
     DeviceArray<T> A1, A2, AF, AL, w, VD, FD, FP;
+    DeviceArray<T> V, T0;        // canonical tables of the LDS-staged kernel
     DeviceArray<int32_t> perm, FI;
+    DeviceArray<int32_t> perm2;  // perm_f followed by iperm_f
 };
 
-// Kernel selection: Variant_Auto uses the MFMA kernel where one is compiled (elasticity), the
-// thread-per-point kernel otherwise; Variant_Points forces the thread-per-point kernel.
-enum Variant : int { Variant_Auto = 0, Variant_Points = 1 };
+// Kernel selection: Variant_Auto uses the LDS-staged MFMA kernel where one is compiled
+// (elasticity), the thread-per-point kernel otherwise; Variant_Points forces the thread-per-point
+// kernel; Variant_MfmaRegs the MFMA kernel with pre-swizzled operand tables read from memory.
+enum Variant : int { Variant_Auto = 0, Variant_Points = 1, Variant_MfmaRegs = 2 };
 
 class Operator {
 public:

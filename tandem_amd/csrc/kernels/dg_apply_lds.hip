@@ -1,0 +1,382 @@
+// Elasticity SIPG apply on the CDNA4 matrix cores with the reference tables and the element
+// vectors staged in LDS.
+//
+// One wave owns 16 consecutive elements: the 16 columns of every 16x16x4 MFMA tile are these
+// elements.  A block of kWaves waves first copies into LDS
+//   sV[e][q][k]   = d phi_k / d xi_e at volume point q          (gradient and divergence operand)
+//   sT[c][q][k]   = face-0 value (c = 0) / d/dxi_(c-1) at facet point q (trace and lift operand)
+//   sP            = the facet relabellings perm[f][k] and their inverses,
+//   sU[wave]      = the 16 element vectors of each wave (one contiguous slice of x),
+// so each MFMA A operand is one LDS read whatever the tile orientation, and B operands of the own
+// element are LDS reads in any relabelled order.
+//
+//   volume   : C(e,q ; el, r) = sum_k V(e,q ; k) U(k ; el, r)                  rows = 16 points
+//              per point in registers, in place: S(e,u) = G (lamWJ tr(Ju) I + muWJ (Ju + Ju^T))
+//              Y(k ; el, u) += sum_(e,q) V(e,q ; k) S(e,q ; el, u)             K = 4 points
+//   facet f  : own and neighbour traces share the face-0 operand T0; the own vector is read in
+//              the face-f frame, U(perm_f(k')), the neighbour's in its face frame; the point
+//              fluxes give W(c ; el, u) and Y(k ; el, u) += sum_(q,c) T0(c, iperm_f(k), q) W(q,c).
+// Output rows of the trace tiles are ordered so that lane group g holds facet point 4t + g with
+// channel c in register c (see mfma_group / mfma_reg).  Padded rows and K entries read a clamped,
+// finite table entry; their B operand (K) is zero or their output row is dropped.  Loads use
+// clamped, in-bounds addresses and the per-point arithmetic is branch-free (results of inactive
+// points are discarded by selects), so the unrolled facet-point loop is one basic block in which
+// the compiler can overlap the point arithmetic of one tile with the MFMAs of the next.
+#include "kernels.h"
+#include "mfma_layout.h"
+
+#include <type_traits>
+
+namespace tdg {
+
+namespace {
+
+template <class T> struct Mma;
+template <> struct Mma<double> {
+    using acc = double __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mma<float> {
+    using acc = float __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <class T> __device__ __forceinline__ T rsqrt_(T v);
+template <> __device__ __forceinline__ double rsqrt_<double>(double v) { return 1.0 / sqrt(v); }
+template <> __device__ __forceinline__ float rsqrt_<float>(float v) { return 1.0f / sqrtf(v); }
+
+constexpr int kWaves = 4;
+
+// FP32: two waves per SIMD (the LDS footprint allows two blocks per CU); FP64: one.
+template <int D, int N, class T>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+void elasticity_lds_kernel(MfmaLdsArgs<T> a, T const* __restrict__ xs,
+                                                                     T* __restrict__ ys, int64_t ld) {
+    using L = MfmaLayout<D, N>;
+    using acc = typename Mma<T>::acc;
+    constexpr int Nbf = L::Nbf, Nq = L::Nq, nq = L::nq, Q = D, BS = Nbf * Q, DD = D * D;
+    constexpr int KS = L::KS, QB = L::QB, MT = L::MT, FT = L::FT, NF = D + 1;
+    constexpr int NCH = L::NCH, NFC = L::NFC;
+    constexpr int nV = D * Nq * Nbf, nT = NF * nq * Nbf;
+
+    __shared__ T sV[nV];
+    __shared__ T sT[nT];
+    __shared__ int sP[2 * NF * Nbf];
+    __shared__ T sU[kWaves][16 * BS];
+
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, j = lane & 15, g = lane >> 4;
+    const int64_t grp = int64_t(blockIdx.x) * kWaves + wv;
+    const bool live = grp < a.num_groups; // uniform per wave
+    T const* __restrict__ x = xs + int64_t(blockIdx.y) * ld;
+    T* __restrict__ y = ys + int64_t(blockIdx.y) * ld;
+    const int64_t e0 = grp * 16;
+    const int nel = live ? int(min(int64_t(16), a.num_elements - e0)) : 0;
+
+    for (int i = tid; i < nV; i += 64 * kWaves) sV[i] = a.V[i];
+    for (int i = tid; i < nT; i += 64 * kWaves) sT[i] = a.T0[i];
+    for (int i = tid; i < 2 * NF * Nbf; i += 64 * kWaves) sP[i] = a.perm[i];
+    T* const U = sU[wv];
+    {
+        T const* src = x + e0 * BS;
+        const int n = nel * BS;
+        for (int i = lane; i < 16 * BS; i += 64) U[i] = i < n ? src[i] : T(0);
+    }
+    __syncthreads();
+
+    const bool valid = j < nel;
+    const int64_t gj = live ? grp : 0; // point data row of this lane (in bounds for every lane)
+    T const* const Uj = U + j * BS;    // element j: U(k, r) at Uj[k + r Nbf]
+    acc Y[MT][Q];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int u = 0; u < Q; ++u) Y[m][u] = acc{0, 0, 0, 0};
+
+    // ------------------------------------------------------------------ volume
+    // one 16-point block; LAST marks the final block, whose fully padded K steps are skipped
+    auto volume_block = [&](int b, auto last) {
+        constexpr bool LAST = decltype(last)::value;
+        // point data of this lane's points q = 16 b + mfma_row(g, i), issued before the MFMAs
+        T pd[4][NCH];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = min(16 * b + mfma_row<T>(g, i), Nq - 1);
+            T const* vd = a.VD + ((gj * Nq + q) * NCH) * 16 + j;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) pd[i][c] = vd[c * 16];
+        }
+        acc C[D][Q];
+#pragma unroll
+        for (int e = 0; e < D; ++e)
+#pragma unroll
+            for (int r = 0; r < Q; ++r) C[e][r] = acc{0, 0, 0, 0};
+        const int qa = min(16 * b + j, Nq - 1) * Nbf; // A row: point 16 b + j
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = 4 * s + g;
+            const int kc = min(k, Nbf - 1);
+            T bu[Q];
+#pragma unroll
+            for (int r = 0; r < Q; ++r) {
+                const T v = Uj[kc + r * Nbf];
+                bu[r] = (4 * s + 3 < Nbf || k < Nbf) ? v : T(0);
+            }
+#pragma unroll
+            for (int e = 0; e < D; ++e) {
+                const T av = sV[e * Nq * Nbf + qa + kc];
+#pragma unroll
+                for (int r = 0; r < Q; ++r) C[e][r] = Mma<T>::mma(av, bu[r], C[e][r]);
+            }
+        }
+        // stress at each point, in place (C[e][u][i] <- S(e, u) of point i), then that point's
+        // share of the divergence: K step (e, i) covers points 16 b + mfma_row(kk, i), kk = 0..3
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (LAST && 16 * (QB - 1) + mfma_row<T>(0, i) >= Nq) continue; // compile-time: all padded
+            const bool act = valid && 16 * b + mfma_row<T>(g, i) < Nq;
+            T const* G = pd[i];
+            const T lw = pd[i][DD], mw = pd[i][DD + 1];
+            T Ju[D][Q]; // Ju[s][r] = du_r/dx_s
+#pragma unroll
+            for (int s2 = 0; s2 < D; ++s2)
+#pragma unroll
+                for (int r = 0; r < Q; ++r) {
+                    T v = T(0);
+#pragma unroll
+                    for (int e = 0; e < D; ++e) v += G[e + s2 * D] * C[e][r][i];
+                    Ju[s2][r] = v;
+                }
+            T tr = T(0);
+#pragma unroll
+            for (int r = 0; r < D; ++r) tr += Ju[r][r];
+#pragma unroll
+            for (int e = 0; e < D; ++e)
+#pragma unroll
+                for (int u = 0; u < Q; ++u) {
+                    T v = T(0);
+#pragma unroll
+                    for (int jj = 0; jj < D; ++jj) {
+                        T P = mw * (Ju[u][jj] + Ju[jj][u]);
+                        if (u == jj) P += lw * tr;
+                        v += G[e + jj * D] * P;
+                    }
+                    C[e][u][i] = act ? v : T(0);
+                }
+            const int qr = min(16 * b + mfma_row<T>(g, i), Nq - 1) * Nbf;
+#pragma unroll
+            for (int e = 0; e < D; ++e)
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    const T av = sV[e * Nq * Nbf + qr + min(16 * m + j, Nbf - 1)];
+#pragma unroll
+                    for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, C[e][u][i], Y[m][u]);
+                }
+        }
+    };
+
+    if (live) {
+#pragma unroll 1
+        for (int b = 0; b < QB - 1; ++b) volume_block(b, std::false_type{});
+        volume_block(QB - 1, std::true_type{});
+
+        // -------------------------------------------------------------- facets
+        // trace A operand row rho = j: facet point 4 t + mfma_group(rho), channel mfma_reg(rho)
+        const int tq = mfma_group<T>(j);
+        const int tc = min(mfma_reg<T>(j), D);
+#pragma unroll 1
+        for (int f = 0; f < NF; ++f) {
+            const int4 info = a.FI[(gj * NF + f) * 16 + j];
+            const int kind = valid ? (info.x & 3) : 0;
+            const int nbface = (info.x >> 4) & 3;
+            const int64_t nb = kind == 1 ? int64_t(info.y) : 0;
+            const T pen = a.FP[(gj * NF + f) * 16 + j];
+            // own B operand offsets in the face-f frame; neighbour vector in its face frame
+            int po[KS];
+            T Nb[KS][Q];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int k = 4 * s + g;
+                const int kc = min(k, Nbf - 1);
+                po[s] = sP[f * Nbf + kc];
+                const int pk = sP[nbface * Nbf + kc];
+#pragma unroll
+                for (int r = 0; r < Q; ++r) {
+                    const T v = x[nb * BS + pk + r * Nbf];
+                    Nb[s][r] = (4 * s + 3 < Nbf || k < Nbf) ? v : T(0);
+                }
+            }
+            // lift A operand columns: face-0 label of row k = 16 m + j
+            int il[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) il[m] = sP[NF * Nbf + f * Nbf + min(16 * m + j, Nbf - 1)];
+            T const* const fdf = a.FD + ((gj * NF + f) * nq) * NFC * 16 + j;
+#pragma unroll
+            for (int t = 0; t < FT; ++t) {
+                const int q = 4 * t + g;
+                const int qc = min(q, nq - 1);
+                const bool act = kind != 0 && q < nq;
+                T fd[NFC];
+#pragma unroll
+                for (int c = 0; c < NFC; ++c) fd[c] = fdf[(qc * NFC + c) * 16];
+                const T wq = a.w[qc];
+                acc Co[Q], Cn[Q];
+#pragma unroll
+                for (int r = 0; r < Q; ++r) Co[r] = Cn[r] = acc{0, 0, 0, 0};
+                const int ab = (tc * nq + min(4 * t + tq, nq - 1)) * Nbf;
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    const int k = 4 * s + g;
+                    const T av = sT[ab + min(k, Nbf - 1)];
+#pragma unroll
+                    for (int r = 0; r < Q; ++r) {
+                        const T v = Uj[po[s] + r * Nbf];
+                        const T bo = (4 * s + 3 < Nbf || k < Nbf) ? v : T(0);
+                        Co[r] = Mma<T>::mma(av, bo, Co[r]);
+                        Cn[r] = Mma<T>::mma(av, Nb[s][r], Cn[r]);
+                    }
+                }
+                // this lane: facet point q; register c = value (0) / d/dxi_e (1 + e)
+                T const* G0 = fd;
+                T const* G1 = fd + DD + 2;
+                const T l0 = fd[DD], m0 = fd[DD + 1], l1 = fd[2 * DD + 2], m1 = fd[2 * DD + 3];
+                T const* n = fd + 2 * DD + 4;
+                T nn = T(0);
+#pragma unroll
+                for (int d = 0; d < D; ++d) nn += n[d] * n[d];
+                const T inl = rsqrt_<T>(nn);
+                T u0[Q], u1[Q], J0[Q][D], J1[Q][D];
+#pragma unroll
+                for (int r = 0; r < Q; ++r) {
+                    u0[r] = Co[r][0];
+                    u1[r] = Cn[r][0];
+#pragma unroll
+                    for (int s2 = 0; s2 < D; ++s2) {
+                        T v0 = T(0), v1 = T(0);
+#pragma unroll
+                        for (int e = 0; e < D; ++e) {
+                            v0 += G0[e + s2 * D] * Co[r][1 + e];
+                            v1 += G1[e + s2 * D] * Cn[r][1 + e];
+                        }
+                        J0[r][s2] = v0;
+                        J1[r][s2] = v1;
+                    }
+                }
+                T tr0 = T(0), tr1 = T(0);
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    tr0 += J0[r][r];
+                    tr1 += J1[r][r];
+                }
+                T dl[Q], sig[Q][D];
+                const bool skel = kind == 1;
+#pragma unroll
+                for (int i = 0; i < Q; ++i) {
+                    dl[i] = skel ? T(0.5) * (u1[i] - u0[i]) : -u0[i];
+                    const T jump = skel ? (u0[i] - u1[i]) : u0[i];
+#pragma unroll
+                    for (int jj = 0; jj < D; ++jj) {
+                        T s0 = m0 * (J0[i][jj] + J0[jj][i]);
+                        T s1 = m1 * (J1[i][jj] + J1[jj][i]);
+                        if (i == jj) {
+                            s0 += l0 * tr0;
+                            s1 += l1 * tr1;
+                        }
+                        sig[i][jj] = (skel ? T(0.5) * (s0 + s1) : s0) - pen * jump * (n[jj] * inl);
+                    }
+                }
+                T dn = T(0);
+#pragma unroll
+                for (int r = 0; r < D; ++r) dn += dl[r] * n[r];
+                T W[D + 1][Q];
+#pragma unroll
+                for (int u = 0; u < Q; ++u) {
+                    T v = T(0);
+#pragma unroll
+                    for (int jj = 0; jj < D; ++jj) v += n[jj] * sig[u][jj];
+                    W[0][u] = act ? -wq * v : T(0);
+                }
+#pragma unroll
+                for (int e = 0; e < D; ++e)
+#pragma unroll
+                    for (int u = 0; u < Q; ++u) {
+                        T v = T(0);
+#pragma unroll
+                        for (int jj = 0; jj < D; ++jj) {
+                            T Cuj = m0 * (dl[u] * n[jj] + dl[jj] * n[u]);
+                            if (u == jj) Cuj += l0 * dn;
+                            v += G0[e + jj * D] * Cuj;
+                        }
+                        W[1 + e][u] = act ? wq * v : T(0);
+                    }
+#pragma unroll
+                for (int c = 0; c <= D; ++c)
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) {
+                        const T av = sT[(c * nq + qc) * Nbf + il[m]];
+#pragma unroll
+                        for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, W[c][u], Y[m][u]);
+                    }
+            }
+        }
+    }
+
+    // ------------------------------------------------------------------ store through LDS
+    __syncthreads(); // every read of U is complete
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k = 16 * m + mfma_row<T>(g, i);
+                if (k < Nbf) {
+#pragma unroll
+                    for (int u = 0; u < Q; ++u) U[j * BS + k + u * Nbf] = Y[m][u][i];
+                }
+            }
+    }
+    __syncthreads();
+    if (live) {
+        T* dst = y + e0 * BS;
+        const int n = nel * BS;
+        for (int i = lane; i < n; i += 64) dst[i] = U[i];
+    }
+}
+
+template <int D, int N, class T>
+hipError_t run(MfmaLdsArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t s) {
+    dim3 grid(unsigned((a.num_groups + kWaves - 1) / kWaves), unsigned(ncols));
+    hipLaunchKernelGGL((elasticity_lds_kernel<D, N, T>), grid, dim3(64 * kWaves), 0, s, a, x, y, ld);
+    return hipGetLastError();
+}
+
+} // namespace
+
+template <class T>
+hipError_t launch_elasticity_lds(int dim, int degree, MfmaLdsArgs<T> const& a, T const* x, T* y, int ncols,
+                                 int64_t ld, hipStream_t s) {
+    if (dim == 3) {
+        switch (degree) {
+        case 2: return run<3, 2, T>(a, x, y, ncols, ld, s);
+        case 3: return run<3, 3, T>(a, x, y, ncols, ld, s);
+        case 4: return run<3, 4, T>(a, x, y, ncols, ld, s);
+        }
+    } else if (dim == 2) {
+        switch (degree) {
+        case 2: return run<2, 2, T>(a, x, y, ncols, ld, s);
+        case 4: return run<2, 4, T>(a, x, y, ncols, ld, s);
+        case 6: return run<2, 6, T>(a, x, y, ncols, ld, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_elasticity_lds<double>(int, int, MfmaLdsArgs<double> const&, double const*, double*, int,
+                                                  int64_t, hipStream_t);
+template hipError_t launch_elasticity_lds<float>(int, int, MfmaLdsArgs<float> const&, float const*, float*, int,
+                                                 int64_t, hipStream_t);
+
+} // namespace tdg

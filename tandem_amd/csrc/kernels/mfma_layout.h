@@ -70,4 +70,22 @@ hipError_t launch_elasticity_mfma(int dim, int degree, MfmaArgs<T> const& a, T c
                                   hipStream_t stream);
 bool mfma_supported(int dim, int degree);
 
+// Arguments of the LDS-staged MFMA kernel: canonical tables (copied to LDS by every block) and the
+// same element-grouped point data as MfmaArgs.
+template <class T> struct MfmaLdsArgs {
+    T const* V;          // [e][q][k]  d phi_k / d xi_e at volume point q
+    T const* T0;         // [c][q][k]  face-0 value (c = 0) and d/dxi_(c-1) at facet point q
+    int32_t const* perm; // [2][f][k]  facet relabelling perm_f, then its inverse
+    T const* w;          // facet weights
+    T const* VD;         // [group][q][NCH][16]
+    T const* FD;         // [group][f][q][NFC][16]
+    T const* FP;         // [group][f][16]
+    int4 const* FI;      // [group][f][16]
+    int64_t num_elements, num_groups;
+};
+
+template <class T>
+hipError_t launch_elasticity_lds(int dim, int degree, MfmaLdsArgs<T> const& a, T const* x, T* y, int ncols,
+                                 int64_t ld, hipStream_t stream);
+
 } // namespace tdg

@@ -130,8 +130,9 @@ class DGOperator:
         self._material = True
 
     def set_variant(self, variant):
-        """"auto" (matrix-core kernel where compiled) or "points" (thread-per-point kernel)."""
-        check(lib().tdg_set_variant(self._h, {"auto": 0, "points": 1}[variant]))
+        """"auto" (LDS-staged matrix-core kernel where compiled), "points" (thread-per-point
+        kernel) or "mfma-tables" (matrix-core kernel with operand tables read from memory)."""
+        check(lib().tdg_set_variant(self._h, {"auto": 0, "points": 1, "mfma-tables": 2}[variant]))
 
     def set_stream(self, stream):
         """stream: a torch.cuda.Stream (or None for the default stream)."""

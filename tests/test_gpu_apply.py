@@ -30,11 +30,12 @@ def _ids(c):
     return "D%d-n%s-s%d-P%d-%s" % (c[0], c[1], c[2], c[3], "elast" if c[4] == 0 else "poisson")
 
 
-@pytest.mark.parametrize("variant", ["auto", "points"])
+@pytest.mark.parametrize("variant", ["auto", "points", "mfma-tables"])
 @pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
 def test_apply_matches_oracle(cuda, case, variant):
-    """variant "auto" runs the matrix-core kernel where one is compiled (elasticity 3D P2-P4,
-    2D P2/P4/P6), "points" the thread-per-point kernel."""
+    """variant "auto" runs the LDS-staged matrix-core kernel where one is compiled (elasticity 3D
+    P2-P4, 2D P2/P4/P6), "points" the thread-per-point kernel, "mfma-tables" the matrix-core kernel
+    reading pre-swizzled operand tables from memory."""
     import torch
 
     D, n, scheme, N, kind, bc = case
@@ -61,10 +62,14 @@ def test_apply_matches_oracle(cuda, case, variant):
     assert err32 <= 1e-5
 
 
-def test_batched_apply_equals_columns(cuda):
+@pytest.mark.parametrize("case", [(3, 2, 1, 4), (3, 3, 1, 2), (2, 5, 0, 6)], ids=["D3-P4", "D3-P2", "D2-P6"])
+def test_batched_apply_equals_columns(cuda, case):
+    """Batched columns (grid.y) and a ragged last 16-element group: each column of the batched apply
+    equals the single apply bit for bit."""
     import torch
 
-    c = Case(3, 2, 1, 4, 0, seed=3)
+    D, n, scheme, N = case
+    c = Case(D, n, scheme, N, 0, seed=3)
     rng = np.random.default_rng(5)
     ncols, ld = 5, c.n + 7
     X = torch.from_numpy(rng.uniform(-1, 1, (ncols, ld))).to(cuda)
