@@ -99,20 +99,38 @@ enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1, TDG_VARIANT_MFMA_TABLES = 2
 int tdg_set_variant(tdg_op* op, int variant);
 
 /* Right-hand side b (host, local size) from a body force at the volume points (elements x Q x Nq,
- * may be NULL) and Dirichlet data at the facet points (facets x Q x nq, only boundary Dirichlet
- * facets are read; may be NULL).  Replaces DGOperator::rhs (src/form/DGOperator.h:189-237) with
- * Elasticity::rhs_volume / rhs_boundary (Elasticity.cpp:521-696). */
-int tdg_rhs(tdg_op const* op, double const* force_Q, double const* dirichlet_q, double* b);
+ * may be NULL) and facet data at the facet points (facets x Q x nq, may be NULL): the Dirichlet
+ * values on Dirichlet facets and the slip on Fault facets (halved on boundary Fault facets, as
+ * bc_boundary does); other facets are not read.  Replaces DGOperator::rhs
+ * (src/form/DGOperator.h:189-237) with Elasticity/Poisson rhs_volume, rhs_skeleton and
+ * rhs_boundary (Elasticity.cpp:521-696, Poisson.cpp:383-540). */
+int tdg_rhs(tdg_op const* op, double const* force_Q, double const* facet_q, double* b);
+
+/* The same right-hand side on the GPU (elasticity, FP64; device pointers) for ncols columns:
+ * column c reads force_Q + c ld_force and facet_q + c ld_facet and writes b + c ld_b (b is
+ * overwritten).  Asynchronous on the operator's stream.  Replaces DGOperator::rhs as above. */
+int tdg_rhs_device(tdg_op* op, double const* force_Q, double const* facet_q, double* b, int ncols,
+                   int64_t ld_force, int64_t ld_facet, int64_t ld_b);
+
+/* Facet traction t (device, facets x Q x nq, FP64) of the device vector x with facet data facet_q
+ * (device, facets x Q x nq, as for tdg_rhs; may be NULL = zero): on skeleton facets
+ * 1/2 (sigma_0 + sigma_1) n - penalty (u_0 - u_1 - f), on boundary facets sigma_0 n - penalty
+ * (u_0 - f), with the unit normal of side 0.  Replaces Elasticity::traction_skeleton /
+ * traction_boundary (Elasticity.cpp:949-1018; compute_traction*, elasticity.py:242-250). */
+int tdg_traction(tdg_op* op, double const* x, double const* facet_q, double* t);
 
 /* Linear solve A x = b on the GPU (b, x device FP64, x overwritten, initial guess 0); replaces
  * PetscLinearSolver::solve = KSPSolve with KSPCG on the shell matrix
- * (app/common/PetscLinearSolver.h:42, PetscLinearSolver.cpp:22-32), without preconditioner.
+ * (app/common/PetscLinearSolver.h:42, PetscLinearSolver.cpp:22-32); precond = TDG_PRECOND_BLOCK_JACOBI uses the
+ * element-block Jacobi preconditioner (tandem's PetscDGMatrix + PCBJACOBI with one block per
+ * element, PetscLinearSolver.cpp:14-15, PetscDGMatrix.cpp:14-77), built on first use.
  * mode 0: FP64 CG.  mode 1: FP32 CG inner solves (to inner_rtol) inside an FP64 iterative
  * refinement loop whose residual b - A x and update x += d are FP64 (restarted each refinement).
  * mode 2: FP32 CG whose residual is replaced by the FP64 true residual, and whose accumulated FP32
  * update is added to the FP64 x, each time the residual drops by reliable_delta (refinement
  * without restarting the Krylov direction).  rtol applies to the FP64 true residual. */
 enum { TDG_SOLVE_FP64_CG = 0, TDG_SOLVE_FP32_CG_FP64_IR = 1, TDG_SOLVE_FP32_CG_FP64_RELIABLE = 2 };
+enum { TDG_PRECOND_NONE = 0, TDG_PRECOND_BLOCK_JACOBI = 1 };
 typedef struct {
     double rtol;
     int max_iter;
@@ -121,6 +139,7 @@ typedef struct {
     int max_outer;
     int check_every;
     double reliable_delta;
+    int precond;
 } tdg_solve_opts;
 typedef struct {
     int iterations;
@@ -131,6 +150,17 @@ typedef struct {
     int converged;
 } tdg_solve_result;
 int tdg_solve(tdg_op* op, double const* b, double* x, tdg_solve_opts const* opts, tdg_solve_result* result);
+
+/* Element-block Jacobi preconditioner (the inverse of each element's diagonal block of A, probed
+ * with the matrix-free apply on an element colouring; assemble_volume/skeleton/boundary restricted
+ * to the element, Elasticity.cpp:317-519, Poisson.cpp:225-381).  tdg_block_jacobi_build builds it
+ * (also done by the first preconditioned solve) and returns the number of colours used;
+ * tdg_block_jacobi_apply computes z = D^-1 r on device FP64 vectors (asynchronous on the operator's
+ * stream); tdg_block_jacobi_inverse copies the FP64 inverse block of one element (bs x bs,
+ * column-major) to the host. */
+int tdg_block_jacobi_build(tdg_op* op, int* colours);
+int tdg_block_jacobi_apply(tdg_op* op, double const* r, double* z);
+int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block);
 
 /* Algorithmic bytes (for the given bytes per real) and flops of one apply. */
 int tdg_apply_counts(tdg_op const* op, int precision_bytes, double* bytes, double* flops);

@@ -435,9 +435,13 @@ void Elasticity::assemble_dense(double* A) const {
     }
 }
 
-// rhs_volume + rhs_boundary (IP) (Elasticity.cpp:521-696; rhsVolume, rhs_lift_ip, rhsFacet in
-// elasticity.py:151-180).  force_Q: E x Q x Nq; dirichlet_q: F x Q x nq (boundary facets only).
-void Elasticity::rhs(double const* force_Q, double const* dirichlet_q, double* b) const {
+// rhs_volume + rhs_skeleton + rhs_boundary (IP) (Elasticity.cpp:521-696; rhsVolume, rhs_lift_ip,
+// rhsFacet in elasticity.py:151-180).  force_Q: E x Q x Nq.  facet_q: F x Q x nq, the Dirichlet
+// data on Dirichlet facets and the slip on Fault facets (bc_skeleton / bc_boundary,
+// Elasticity.cpp:545-573: a Fault on the boundary takes half the slip); other facets are ignored.
+// Skeleton facets add to both sides with c10 = eps/2 and c20 = +-penalty, the side-0 normal and each
+// side's own traces (Elasticity.cpp:575-643); boundary facets c10 = eps, c20 = penalty (:645-696).
+void Elasticity::rhs(double const* force_Q, double const* facet_q, double* b) const {
     unsigned D = c.D, bs = Nbf * Q;
     std::size_t ne = c.nE();
     std::fill(b, b + bs * ne, 0.0);
@@ -452,40 +456,49 @@ void Elasticity::rhs(double const* force_Q, double const* dirichlet_q, double* b
                     b[el * bs + k + p * Nbf] += s;
                 }
     }
-    if (!dirichlet_q) return;
+    if (!facet_q) return;
+    std::vector<double> fq(Q * nq);
     for (std::size_t f = 0; f < c.nF(); ++f) {
         auto const& info = c.topo.fct[f];
-        if (info.up[0] != info.up[1] || info.bc != BC::Dirichlet) continue;
-        double const* fq = &dirichlet_q[f * Q * nq];
-        auto Dx = physical_grad(D, Nbf, nq, &c.JInv0[f * nq * D * D], Dxi_q[info.localNo[0]]);
-        auto const& Eq = E_q[info.localNo[0]];
+        bool skel = info.up[0] != info.up[1];
+        if (info.bc != BC::Dirichlet && info.bc != BC::Fault) continue;
+        double scale = (!skel && info.bc == BC::Fault) ? 0.5 : 1.0;
+        for (unsigned i = 0; i < Q * nq; ++i) fq[i] = scale * facet_q[f * Q * nq + i];
         double const* n = &c.normal[f * nq * D];
-        double c10 = -1.0, c20 = penalty[f];
-        double* B = b + info.up[0] * bs;
-        for (unsigned q = 0; q < nq; ++q) {
-            double w = c.fctRule.wts[q];
-            double lq = lam_q0[f * nq + q], mq = mu_q0[f * nq + q];
-            double fn = 0.0;
-            for (unsigned i = 0; i < D; ++i) fn += fq[i * nq + q] * n[q * D + i];
-            for (unsigned k = 0; k < Nbf; ++k) {
-                double nd = 0.0;
-                for (unsigned j = 0; j < D; ++j) nd += Dx[k + Nbf * (j + D * q)] * n[q * D + j];
-                double dfd = 0.0;
-                for (unsigned i = 0; i < D; ++i) dfd += Dx[k + Nbf * (i + D * q)] * fq[i * nq + q];
-                for (unsigned p = 0; p < Q; ++p) {
-                    double tt = lq * Dx[k + Nbf * (p + D * q)] * fn +
-                                mq * (nd * fq[p * nq + q] + dfd * n[q * D + p]);
-                    B[k + p * Nbf] += c10 * tt * w +
-                                      c20 * w * Eq[k + Nbf * q] * fq[p * nq + q] * c.normalLength[f * nq + q];
+        for (int side = 0; side < (skel ? 2 : 1); ++side) {
+            double const* g = side == 0 ? &c.JInv0[f * nq * D * D] : &c.JInv1[f * nq * D * D];
+            auto Dx = physical_grad(D, Nbf, nq, g, Dxi_q[info.localNo[side]]);
+            auto const& Eq = E_q[info.localNo[side]];
+            double const* lam = side == 0 ? &lam_q0[f * nq] : &lam_q1[f * nq];
+            double const* mu = side == 0 ? &mu_q0[f * nq] : &mu_q1[f * nq];
+            double c10 = skel ? -0.5 : -1.0, c20 = side == 0 ? penalty[f] : -penalty[f];
+            double* B = b + info.up[side] * bs;
+            for (unsigned q = 0; q < nq; ++q) {
+                double w = c.fctRule.wts[q];
+                double lq = lam[q], mq = mu[q];
+                double fn = 0.0;
+                for (unsigned i = 0; i < D; ++i) fn += fq[i * nq + q] * n[q * D + i];
+                for (unsigned k = 0; k < Nbf; ++k) {
+                    double nd = 0.0;
+                    for (unsigned j = 0; j < D; ++j) nd += Dx[k + Nbf * (j + D * q)] * n[q * D + j];
+                    double dfd = 0.0;
+                    for (unsigned i = 0; i < D; ++i) dfd += Dx[k + Nbf * (i + D * q)] * fq[i * nq + q];
+                    for (unsigned p = 0; p < Q; ++p) {
+                        double tt = lq * Dx[k + Nbf * (p + D * q)] * fn +
+                                    mq * (nd * fq[p * nq + q] + dfd * n[q * D + p]);
+                        B[k + p * Nbf] += c10 * tt * w +
+                                          c20 * w * Eq[k + Nbf * q] * fq[p * nq + q] * c.normalLength[f * nq + q];
+                    }
                 }
             }
         }
     }
 }
 
-// compute_traction / compute_traction_bnd (elasticity.py:242-250; Elasticity.cpp:949-1018) with
-// f_q = 0 for skeleton facets and the Dirichlet data on boundary facets.
-void Elasticity::traction(std::size_t f, double const* x, double const* f_q, double* t) const {
+// compute_traction / compute_traction_bnd (elasticity.py:242-250; Elasticity.cpp:949-1018).  f_q
+// (Q x nq, may be null) is the facet's data as for rhs: read on Dirichlet and Fault facets only
+// (bc_skeleton / bc_boundary), halved on boundary Fault facets; zero elsewhere.
+void Elasticity::traction(std::size_t f, double const* x, double const* f_q_in, double* t) const {
     unsigned D = c.D, bs = Nbf * Q;
     auto const& info = c.topo.fct[f];
     bool skel = info.up[0] != info.up[1];
@@ -520,9 +533,12 @@ void Elasticity::traction(std::size_t f, double const* x, double const* f_q, dou
         }
     }
     double c00 = -penalty[f];
+    bool data = f_q_in && (info.bc == BC::Dirichlet || info.bc == BC::Fault);
+    double scale = (!skel && info.bc == BC::Fault) ? 0.5 : 1.0;
     for (unsigned p = 0; p < D; ++p)
         for (unsigned q = 0; q < nq; ++q) {
-            double jump = uv[p * nq + q] - (skel ? uv[(D + p) * nq + q] : 0.0) - (f_q ? f_q[p * nq + q] : 0.0);
+            double fv = data ? scale * f_q_in[p * nq + q] : 0.0;
+            double jump = uv[p * nq + q] - (skel ? uv[(D + p) * nq + q] : 0.0) - fv;
             t[p * nq + q] = tr[p * nq + q] + c00 * jump;
         }
 }
@@ -764,8 +780,9 @@ void Poisson::assemble_dense(double* A) const {
     }
 }
 
-// rhs_volume + rhs_boundary (IP) (Poisson.cpp:383-540; poisson.py:83-118)
-void Poisson::rhs(double const* force_Q, double const* dirichlet_q, double* b) const {
+// rhs_volume + rhs_skeleton + rhs_boundary (IP) (Poisson.cpp:383-540; poisson.py:79-103), with
+// the facet data conventions of Elasticity::rhs.
+void Poisson::rhs(double const* force_Q, double const* facet_q, double* b) const {
     unsigned D = c.D, DD = D * D, Nm = matSpace.nbf;
     std::size_t ne = c.nE();
     std::fill(b, b + Nbf * ne, 0.0);
@@ -778,25 +795,31 @@ void Poisson::rhs(double const* force_Q, double const* dirichlet_q, double* b) c
                 b[el * Nbf + k] += s;
             }
     }
-    if (!dirichlet_q) return;
+    if (!facet_q) return;
     for (std::size_t f = 0; f < c.nF(); ++f) {
         auto const& info = c.topo.fct[f];
-        if (info.up[0] != info.up[1] || info.bc != BC::Dirichlet) continue;
-        double const* fq = &dirichlet_q[f * nq];
-        auto Dx = physical_grad(D, Nbf, nq, &c.JInv0[f * nq * DD], Dxi_q[info.localNo[0]]);
-        auto const& mT = matE_q_T[info.localNo[0]];
-        auto const& Eq = E_q[info.localNo[0]];
+        bool skel = info.up[0] != info.up[1];
+        if (info.bc != BC::Dirichlet && info.bc != BC::Fault) continue;
+        double scale = (!skel && info.bc == BC::Fault) ? 0.5 : 1.0;
+        double const* fq = &facet_q[f * nq];
         double const* n = &c.normal[f * nq * D];
-        double c10 = -1.0, c20 = penalty[f];
-        for (unsigned q = 0; q < nq; ++q) {
-            double kq = 0.0;
-            for (unsigned m = 0; m < Nm; ++m) kq += K[info.up[0] * Nm + m] * mT[q + m * nq];
-            double w = c.fctRule.wts[q];
-            for (unsigned k = 0; k < Nbf; ++k) {
-                double kn = 0.0;
-                for (unsigned i = 0; i < D; ++i) kn += kq * Dx[k + Nbf * (i + D * q)] * n[q * D + i];
-                b[info.up[0] * Nbf + k] += c10 * w * kn * fq[q] +
-                                           c20 * w * Eq[k + Nbf * q] * c.normalLength[f * nq + q] * fq[q];
+        for (int side = 0; side < (skel ? 2 : 1); ++side) {
+            double const* g = side == 0 ? &c.JInv0[f * nq * DD] : &c.JInv1[f * nq * DD];
+            auto Dx = physical_grad(D, Nbf, nq, g, Dxi_q[info.localNo[side]]);
+            auto const& mT = matE_q_T[info.localNo[side]];
+            auto const& Eq = E_q[info.localNo[side]];
+            double c10 = skel ? -0.5 : -1.0, c20 = side == 0 ? penalty[f] : -penalty[f];
+            std::size_t el = info.up[side];
+            for (unsigned q = 0; q < nq; ++q) {
+                double kq = 0.0;
+                for (unsigned m = 0; m < Nm; ++m) kq += K[el * Nm + m] * mT[q + m * nq];
+                double w = c.fctRule.wts[q];
+                for (unsigned k = 0; k < Nbf; ++k) {
+                    double kn = 0.0;
+                    for (unsigned i = 0; i < D; ++i) kn += kq * Dx[k + Nbf * (i + D * q)] * n[q * D + i];
+                    b[el * Nbf + k] += c10 * w * kn * scale * fq[q] +
+                                       c20 * w * Eq[k + Nbf * q] * c.normalLength[f * nq + q] * scale * fq[q];
+                }
             }
         }
     }

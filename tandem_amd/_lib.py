@@ -24,7 +24,8 @@ class TdgError(RuntimeError):
 
 class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("max_iter", C.c_int), ("mode", C.c_int), ("inner_rtol", C.c_double),
-                ("max_outer", C.c_int), ("check_every", C.c_int), ("reliable_delta", C.c_double)]
+                ("max_outer", C.c_int), ("check_every", C.c_int), ("reliable_delta", C.c_double),
+                ("precond", C.c_int)]
 
 
 class SolveResult(C.Structure):
@@ -50,7 +51,7 @@ class MeshDesc(C.Structure):
 EXPORTS = [
     "tdg_create", "tdg_destroy", "tdg_last_error", "tdg_sizes", "tdg_volume_points", "tdg_facet_points",
     "tdg_facets", "tdg_set_material", "tdg_upload", "tdg_mult", "tdg_mult_f32", "tdg_mult_batched", "tdg_mult_batched_f32",
-    "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_rhs", "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
+    "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_rhs", "tdg_rhs_device", "tdg_traction", "tdg_block_jacobi_build", "tdg_block_jacobi_apply", "tdg_block_jacobi_inverse", "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
 ]
 
 
@@ -79,6 +80,11 @@ def lib():
     L.tdg_set_stream.argtypes = [vp, vp]
     L.tdg_set_variant.argtypes = [vp, C.c_int]
     L.tdg_rhs.argtypes = [vp, vp, vp, _dp]
+    L.tdg_rhs_device.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int64, C.c_int64, C.c_int64]
+    L.tdg_traction.argtypes = [vp, vp, vp, vp]
+    L.tdg_block_jacobi_build.argtypes = [vp, C.POINTER(C.c_int)]
+    L.tdg_block_jacobi_apply.argtypes = [vp, vp, vp]
+    L.tdg_block_jacobi_inverse.argtypes = [vp, C.c_int64, _dp]
     L.tdg_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]
     L.tdg_apply_counts.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.tdg_host_array.argtypes = [vp, C.c_char_p, vp, C.POINTER(C.c_int64)]

@@ -5,6 +5,7 @@
 #include "host/operator.h"
 #include "host/refelem.h"
 #include "solver/krylov.h"
+#include "solver/block_jacobi.h"
 
 #include <hip/hip_runtime.h>
 
@@ -198,10 +199,33 @@ int tdg_set_variant(tdg_op* op, int variant) {
     });
 }
 
-int tdg_rhs(tdg_op const* op, double const* force_Q, double const* dirichlet_q, double* b) {
+int tdg_rhs(tdg_op const* op, double const* force_Q, double const* facet_q, double* b) {
     return guarded([&] {
         require(op && op->impl && b, "null argument");
-        op->impl->rhs(force_Q, dirichlet_q, b);
+        op->impl->rhs(force_Q, facet_q, b);
+    });
+}
+
+int tdg_rhs_device(tdg_op* op, double const* force_Q, double const* facet_q, double* b, int ncols,
+                   int64_t ld_force, int64_t ld_facet, int64_t ld_b) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(b && ncols >= 1, "bad rhs arguments");
+        const int64_t E = int64_t(o.num_elements()), F = int64_t(o.num_facets());
+        const int64_t Q = int64_t(o.block_size() / o.nbf());
+        require(ncols == 1 || (ld_b >= int64_t(o.local_size()) &&
+                               (!force_Q || ld_force >= E * Q * int64_t(o.nq_vol())) &&
+                               (!facet_q || ld_facet >= F * Q * int64_t(o.nq_fct()))),
+                "leading dimensions are smaller than one column");
+        o.rhs_device(force_Q, facet_q, b, ncols, ld_force, ld_facet, ld_b);
+    });
+}
+
+int tdg_traction(tdg_op* op, double const* x, double const* facet_q, double* t) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(x && t, "null argument");
+        o.traction_device(x, facet_q, t);
     });
 }
 
@@ -217,6 +241,8 @@ int tdg_solve(tdg_op* op, double const* b, double* x, tdg_solve_opts const* opts
         so.max_outer = opts->max_outer;
         so.check_every = opts->check_every;
         so.reliable_delta = opts->reliable_delta;
+        so.precond = opts->precond;
+        require(so.precond == TDG_PRECOND_NONE || so.precond == TDG_PRECOND_BLOCK_JACOBI, "unknown preconditioner");
         require(so.mode == TDG_SOLVE_FP64_CG || so.mode == TDG_SOLVE_FP32_CG_FP64_IR ||
                     so.mode == TDG_SOLVE_FP32_CG_FP64_RELIABLE,
                 "unknown solve mode");
@@ -231,6 +257,29 @@ int tdg_solve(tdg_op* op, double const* b, double* x, tdg_solve_opts const* opts
         result->rel_residual = r.rel_residual;
         result->seconds = r.seconds;
         result->converged = r.converged;
+    });
+}
+
+int tdg_block_jacobi_build(tdg_op* op, int* colours) {
+    return guarded([&] {
+        auto& bj = ready(op).block_jacobi();
+        if (colours) *colours = bj.colours();
+    });
+}
+
+int tdg_block_jacobi_apply(tdg_op* op, double const* r, double* z) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(r && z && r != z, "bad block Jacobi arguments");
+        o.block_jacobi().apply(r, z, o.stream());
+    });
+}
+
+int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(block, "null argument");
+        o.block_jacobi().inverse_block(element, block);
     });
 }
 

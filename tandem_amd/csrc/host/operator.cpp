@@ -11,8 +11,10 @@
 
 #include "../kernels/kernels.h"
 #include "../kernels/mfma_layout.h"
+#include "../solver/block_jacobi.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -375,6 +377,8 @@ void Operator::set_material(double const* c0_Q, double const* c1_Q) {
     d32_.reset();
     m64_.reset();
     m32_.reset();
+    rhs64_.reset();
+    bj_.reset();
     ready_ = true;
 }
 
@@ -471,6 +475,7 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
     // element-grouped point data
     const std::size_t ng = (E_ + 15) / 16;
     std::vector<T> VD(ng * Nq_ * L.NCH * 16, T(0)), FD(ng * NF * nq_ * L.NFC * 16, T(0)), FP(ng * NF * 16, T(0));
+    std::vector<T> FS(ng * NF * nq_ * L.NFS * 16, T(0));
     std::vector<int32_t> FI(ng * NF * 16 * 4, 0);
 #pragma omp parallel for schedule(static)
     for (std::size_t e = 0; e < E_; ++e) {
@@ -503,9 +508,15 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
                 o[(2 * DD + 3) * 16] = T(fc1_[(fct * 2 + (1 - side)) * nq_ + q]);
                 double sg = side ? -1.0 : 1.0;
                 for (unsigned d = 0; d < D; ++d) o[(2 * DD + 4 + d) * 16] = T(sg * normal_[(fct * nq_ + q) * D + d]);
+                T* os = &FS[(((grp * NF + f) * nq_ + q) * L.NFS) * 16 + j];
+                for (unsigned c = 0; c < DD; ++c) os[c * 16] = T(g0[c]);
+                os[DD * 16] = T(fc0_[(fct * 2 + side) * nq_ + q]);
+                os[(DD + 1) * 16] = T(fc1_[(fct * 2 + side) * nq_ + q]);
+                for (unsigned d = 0; d < D; ++d) os[(DD + 2 + d) * 16] = T(sg * normal_[(fct * nq_ + q) * D + d]);
             }
         }
     }
+    dm.FS.upload(FS);
     dm.V.upload(cast<T>(V));
     dm.T0.upload(cast<T>(TT));
     dm.perm2.upload(P2);
@@ -568,9 +579,9 @@ template <class T> void Operator::upload(DeviceOperator<T>& d) {
 
 // rhs_volume + rhs_boundary with the interior-penalty lift (Elasticity.cpp:521-696 /
 // Poisson.cpp:383-540; rhsVolume, rhs_lift_ip, rhsFacet).  Dirichlet data on boundary facets only.
-void Operator::rhs(double const* force_Q, double const* dirichlet_q, double* b) const {
+void Operator::rhs(double const* force_Q, double const* facet_q, double* b) const {
     if (!ready_) throw std::runtime_error("set_material() must be called first");
-    unsigned D = D_, DD = D * D, Nm = static_cast<unsigned>(num_basis(D, N_));
+    unsigned D = D_, DD = D * D;
     std::size_t bs = block_size();
     bool el = kind_ == Kind_Elasticity;
     std::fill(b, b + bs * E_, 0.0);
@@ -585,50 +596,126 @@ void Operator::rhs(double const* force_Q, double const* dirichlet_q, double* b) 
                     b[e * bs + k + p * Nbf_] += s;
                 }
     }
-    if (!dirichlet_q) return;
+    if (!facet_q) return;
+    // rhs_skeleton / rhs_boundary (Elasticity.cpp:545-696, Poisson.cpp:405-540): Dirichlet data on
+    // Dirichlet facets, slip on Fault facets (halved on the boundary); skeleton facets add to both
+    // sides with c10 = eps/2, c20 = +-penalty and the side-0 normal.
     std::size_t F = topo_.facets.size();
+    std::vector<double> fq(Q_ * nq_);
     for (std::size_t i = 0; i < F; ++i) {
         Facet const& fc = topo_.facets[i];
-        if (fc.elem[0] != fc.elem[1] || fc.bc != BC_Dirichlet) continue;
-        double const* fq = &dirichlet_q[i * Q_ * nq_];
-        double const* Ji = &fJInv_[0][i * nq_ * DD];
-        auto const& Eq = E_q_[fc.local[0]];
-        auto const& Dq = Dxi_q_[fc.local[0]];
-        double* B = b + fc.elem[0] * bs;
+        if (fc.bc != BC_Dirichlet && fc.bc != BC_Fault) continue;
+        bool skel = fc.elem[0] != fc.elem[1];
+        double scale = (!skel && fc.bc == BC_Fault) ? 0.5 : 1.0;
+        for (unsigned j = 0; j < Q_ * nq_; ++j) fq[j] = scale * facet_q[i * Q_ * nq_ + j];
         double pen = penalty_[i];
-        for (unsigned q = 0; q < nq_; ++q) {
-            double w = fct_.weights[q];
-            double const* n = &normal_[(i * nq_ + q) * D];
-            double nl = nlen_[i * nq_ + q];
-            for (unsigned k = 0; k < Nbf_; ++k) {
-                double dx[3];
-                for (unsigned j = 0; j < D; ++j) {
-                    double v = 0.0;
-                    for (unsigned e = 0; e < D; ++e) v += Ji[q * DD + e + j * D] * Dq[k + Nbf_ * (e + D * q)];
-                    dx[j] = v;
-                }
-                if (el) {
-                    double lq = fc0_[(i * 2) * nq_ + q], mq = fc1_[(i * 2) * nq_ + q];
-                    double fn = 0.0, nd = 0.0, dfd = 0.0;
+        for (int side = 0; side < (skel ? 2 : 1); ++side) {
+            double const* Ji = &fJInv_[side][i * nq_ * DD];
+            auto const& Eq = E_q_[fc.local[side]];
+            auto const& Dq = Dxi_q_[fc.local[side]];
+            double* B = b + fc.elem[side] * bs;
+            double c10 = skel ? -0.5 : -1.0, c20 = side == 0 ? pen : -pen;
+            for (unsigned q = 0; q < nq_; ++q) {
+                double w = fct_.weights[q];
+                double const* n = &normal_[(i * nq_ + q) * D];
+                double nl = nlen_[i * nq_ + q];
+                double c0s = fc0_[(i * 2 + side) * nq_ + q], c1s = fc1_[(i * 2 + side) * nq_ + q];
+                for (unsigned k = 0; k < Nbf_; ++k) {
+                    double dx[3];
                     for (unsigned j = 0; j < D; ++j) {
-                        fn += fq[j * nq_ + q] * n[j];
-                        nd += dx[j] * n[j];
-                        dfd += dx[j] * fq[j * nq_ + q];
+                        double v = 0.0;
+                        for (unsigned e = 0; e < D; ++e) v += Ji[q * DD + e + j * D] * Dq[k + Nbf_ * (e + D * q)];
+                        dx[j] = v;
                     }
-                    for (unsigned p = 0; p < D; ++p) {
-                        double tt = lq * dx[p] * fn + mq * (nd * fq[p * nq_ + q] + dfd * n[p]);
-                        B[k + p * Nbf_] += -tt * w + pen * w * Eq[k + Nbf_ * q] * fq[p * nq_ + q] * nl;
+                    if (el) {
+                        double fn = 0.0, nd = 0.0, dfd = 0.0;
+                        for (unsigned j = 0; j < D; ++j) {
+                            fn += fq[j * nq_ + q] * n[j];
+                            nd += dx[j] * n[j];
+                            dfd += dx[j] * fq[j * nq_ + q];
+                        }
+                        for (unsigned p = 0; p < D; ++p) {
+                            double tt = c0s * dx[p] * fn + c1s * (nd * fq[p * nq_ + q] + dfd * n[p]);
+                            B[k + p * Nbf_] += c10 * tt * w + c20 * w * Eq[k + Nbf_ * q] * fq[p * nq_ + q] * nl;
+                        }
+                    } else {
+                        double kn = 0.0;
+                        for (unsigned j = 0; j < D; ++j) kn += c0s * dx[j] * n[j];
+                        B[k] += c10 * w * kn * fq[q] + c20 * w * Eq[k + Nbf_ * q] * nl * fq[q];
                     }
-                } else {
-                    double kq = fc0_[(i * 2) * nq_ + q];
-                    double kn = 0.0;
-                    for (unsigned j = 0; j < D; ++j) kn += kq * dx[j] * n[j];
-                    B[k] += -w * kn * fq[q] + pen * w * Eq[k + Nbf_ * q] * nl * fq[q];
                 }
             }
         }
     }
-    (void)Nm;
+}
+
+RhsArgs Operator::rhs_args() {
+    if (kind_ != Kind_Elasticity) throw std::invalid_argument("the device rhs / traction is built for elasticity");
+    upload_device();
+    if (!rhs64_) {
+        auto r = std::make_unique<DeviceRhs>();
+        std::vector<double> tabE(std::size_t(Nbf_) * Nq_), jw(E_ * Nq_);
+        for (unsigned k = 0; k < Nbf_; ++k)
+            for (unsigned q = 0; q < Nq_; ++q) tabE[k * Nq_ + q] = E_Q_[k + Nbf_ * q];
+        for (std::size_t e = 0; e < E_; ++e)
+            for (unsigned q = 0; q < Nq_; ++q) jw[e * Nq_ + q] = absDetJ_[e * Nq_ + q] * vol_.weights[q];
+        std::size_t F = topo_.facets.size();
+        std::vector<int32_t> fbc(F), fct(F * 4, 0);
+        for (std::size_t i = 0; i < F; ++i) {
+            Facet const& fc = topo_.facets[i];
+            bool skel = fc.elem[0] != fc.elem[1];
+            fbc[i] = int32_t(fc.bc) | (skel ? 1 << 8 : 0);
+            fct[i * 4 + 0] = int32_t(fc.elem[0]);
+            fct[i * 4 + 1] = int32_t(fc.elem[1]);
+            fct[i * 4 + 2] = int32_t(fc.local[0]) | (int32_t(fc.local[1]) << 4);
+        }
+        r->tabE.upload(tabE);
+        r->jw.upload(jw);
+        r->fbc.upload(fbc);
+        r->fct.upload(fct);
+        rhs64_ = std::move(r);
+    }
+    DeviceOperator<double> const& d = *d64_;
+    RhsArgs a;
+    a.tabE = rhs64_->tabE.ptr;
+    a.jw = rhs64_->jw.ptr;
+    a.tabF = d.tabF.ptr;
+    a.w = d.w.ptr;
+    a.perm = d.perm.ptr;
+    a.iperm = d.iperm.ptr;
+    a.fG = d.fG.ptr;
+    a.fc0 = d.fc0.ptr;
+    a.fc1 = d.fc1.ptr;
+    a.fn = d.fn.ptr;
+    a.fpen = d.fpen.ptr;
+    a.ef = reinterpret_cast<int4 const*>(d.ef.ptr);
+    a.fbc = rhs64_->fbc.ptr;
+    a.fct = reinterpret_cast<int4 const*>(rhs64_->fct.ptr);
+    a.num_elements = int64_t(E_);
+    a.num_facets = int64_t(topo_.facets.size());
+    int kq = 0, kf = 0;
+    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
+        throw std::runtime_error("no kernel compiled for this dimension/degree");
+    return a;
+}
+
+void Operator::rhs_device(double const* force_Q, double const* facet_q, double* b, int ncols, int64_t ld_force,
+                          int64_t ld_facet, int64_t ld_b) {
+    if (!ready_) throw std::logic_error("set_material() must be called first");
+    RhsArgs a = rhs_args();
+    TDG_HIP(launch_elasticity_rhs(D_, N_, a, force_Q, facet_q, b, ncols, ld_force, ld_facet, ld_b, stream_));
+}
+
+BlockJacobi& Operator::block_jacobi() {
+    if (!bj_) bj_ = std::make_unique<BlockJacobi>();
+    if (!bj_->built()) bj_->build(*this);
+    return *bj_;
+}
+
+void Operator::traction_device(double const* x, double const* facet_q, double* t) {
+    if (!ready_) throw std::logic_error("set_material() must be called first");
+    RhsArgs a = rhs_args();
+    TDG_HIP(launch_elasticity_traction(D_, N_, a, x, facet_q, t, stream_));
 }
 
 std::vector<double> const* Operator::host_array(std::string const& n) const {
@@ -742,11 +829,12 @@ void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, 
     b.perm = d.perm2.ptr;
     b.w = d.w.ptr;
     b.VD = d.VD.ptr;
-    b.FD = d.FD.ptr;
+    b.FS = d.FS.ptr;
     b.FP = d.FP.ptr;
     b.FI = a.FI;
     b.num_elements = a.num_elements;
     b.num_groups = a.num_groups;
+    b.xcd_remap = std::getenv("TDG_XCD_REMAP") ? std::atoi(std::getenv("TDG_XCD_REMAP")) : 1;
     TDG_HIP(launch_elasticity_lds<T>(D_, N_, b, x, y, ncols, ld, stream_));
 }
 

@@ -4,6 +4,7 @@
 
 #include "mesh.h"
 #include "refelem.h"
+#include "../kernels/kernels.h"
 
 #include <hip/hip_runtime.h>
 
@@ -45,15 +46,23 @@ template <class T> struct DeviceOperator {
 // Operand tables and element-grouped point data of the MFMA elasticity kernel (one precision).
 template <class T> struct DeviceMfma {
     DeviceArray<T> A1, A2, AF, AL, w, VD, FD, FP;
-    DeviceArray<T> V, T0;        // canonical tables of the LDS-staged kernel
+    DeviceArray<T> V, T0, FS;    // canonical tables and one-side facet records of the LDS-staged kernel
     DeviceArray<int32_t> perm, FI;
     DeviceArray<int32_t> perm2;  // perm_f followed by iperm_f
+};
+
+// Extra FP64 data of the right-hand side and traction kernels (elasticity).
+struct DeviceRhs {
+    DeviceArray<double> tabE, jw;
+    DeviceArray<int32_t> fbc, fct; // fct: int4 per facet
 };
 
 // Kernel selection: Variant_Auto uses the LDS-staged MFMA kernel where one is compiled
 // (elasticity), the thread-per-point kernel otherwise; Variant_Points forces the thread-per-point
 // kernel; Variant_MfmaRegs the MFMA kernel with pre-swizzled operand tables read from memory.
 enum Variant : int { Variant_Auto = 0, Variant_Points = 1, Variant_MfmaRegs = 2 };
+
+class BlockJacobi;
 
 class Operator {
 public:
@@ -97,8 +106,18 @@ public:
     void apply_batched(double const* X, double* Y, int ncols, int64_t ld);
     void apply_batched(float const* X, float* Y, int ncols, int64_t ld);
 
-    // right-hand side (force at volume points, Dirichlet data at facet points), FP64, host.
-    void rhs(double const* force_Q, double const* dirichlet_q, double* b_host) const;
+    // Right-hand side, FP64, from the force at the volume points (E x Q x Nq) and facet data at the
+    // facet points (F x Q x nq): Dirichlet values on Dirichlet facets, slip on Fault facets (half
+    // of it on boundary Fault facets).  Host version and device version (device buffers, ncols
+    // columns with leading dimensions ld_*; force or facet data may be null).
+    void rhs(double const* force_Q, double const* facet_q, double* b_host) const;
+    void rhs_device(double const* force_Q, double const* facet_q, double* b, int ncols, int64_t ld_force,
+                    int64_t ld_facet, int64_t ld_b);
+    // Facet traction (F x Q x nq) of the device vector x with facet data (may be null), FP64.
+    void traction_device(double const* x, double const* facet_q, double* t);
+
+    // Element-block Jacobi preconditioner (built on first use; rebuilt after set_material).
+    BlockJacobi& block_jacobi();
 
     // named host arrays for inspection
     std::vector<double> const* host_array(std::string const& name) const;
@@ -150,6 +169,9 @@ private:
     std::unique_ptr<DeviceOperator<float>> d32_;
     std::unique_ptr<DeviceMfma<double>> m64_;
     std::unique_ptr<DeviceMfma<float>> m32_;
+    std::unique_ptr<DeviceRhs> rhs64_;
+    std::unique_ptr<BlockJacobi> bj_;
+    RhsArgs rhs_args();
     int variant_ = Variant_Auto;
 };
 

@@ -60,7 +60,7 @@ void elasticity_lds_kernel(MfmaLdsArgs<T> a, T const* __restrict__ xs,
     using acc = typename Mma<T>::acc;
     constexpr int Nbf = L::Nbf, Nq = L::Nq, nq = L::nq, Q = D, BS = Nbf * Q, DD = D * D;
     constexpr int KS = L::KS, QB = L::QB, MT = L::MT, FT = L::FT, NF = D + 1;
-    constexpr int NCH = L::NCH, NFC = L::NFC;
+    constexpr int NCH = L::NCH, NFS = L::NFS;
     constexpr int nV = D * Nq * Nbf, nT = NF * nq * Nbf;
 
     __shared__ T sV[nV];
@@ -69,7 +69,11 @@ void elasticity_lds_kernel(MfmaLdsArgs<T> a, T const* __restrict__ xs,
     __shared__ T sU[kWaves][16 * BS];
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, j = lane & 15, g = lane >> 4;
-    const int64_t grp = int64_t(blockIdx.x) * kWaves + wv;
+    // Workgroups are dealt round-robin to the 8 XCDs; renumber them so that each XCD works through
+    // one contiguous range of element groups and neighbouring elements share its L2.  The grid
+    // is a multiple of 8 blocks.
+    const unsigned lb = a.xcd_remap ? (blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u : blockIdx.x;
+    const int64_t grp = int64_t(lb) * kWaves + wv;
     const bool live = grp < a.num_groups; // uniform per wave
     T const* __restrict__ x = xs + int64_t(blockIdx.y) * ld;
     T* __restrict__ y = ys + int64_t(blockIdx.y) * ld;
@@ -213,15 +217,22 @@ void elasticity_lds_kernel(MfmaLdsArgs<T> a, T const* __restrict__ xs,
             int il[MT];
 #pragma unroll
             for (int m = 0; m < MT; ++m) il[m] = sP[NF * Nbf + f * Nbf + min(16 * m + j, Nbf - 1)];
-            T const* const fdf = a.FD + ((gj * NF + f) * nq) * NFC * 16 + j;
+            // facet records: this side's, and the other side's from the neighbour's own record of
+            // the shared facet (this element's own record on boundary faces, where it is unused)
+            T const* const fso = a.FS + ((gj * NF + f) * nq) * NFS * 16 + j;
+            const int64_t oe = kind == 1 ? nb : gj * 16 + j;
+            const int ofc = kind == 1 ? nbface : f;
+            T const* const fsn = a.FS + (((oe >> 4) * NF + ofc) * nq) * NFS * 16 + (oe & 15);
 #pragma unroll
             for (int t = 0; t < FT; ++t) {
                 const int q = 4 * t + g;
                 const int qc = min(q, nq - 1);
                 const bool act = kind != 0 && q < nq;
-                T fd[NFC];
+                T fo[NFS], fn[DD + 2];
 #pragma unroll
-                for (int c = 0; c < NFC; ++c) fd[c] = fdf[(qc * NFC + c) * 16];
+                for (int c = 0; c < NFS; ++c) fo[c] = fso[(qc * NFS + c) * 16];
+#pragma unroll
+                for (int c = 0; c < DD + 2; ++c) fn[c] = fsn[(qc * NFS + c) * 16];
                 const T wq = a.w[qc];
                 acc Co[Q], Cn[Q];
 #pragma unroll
@@ -240,10 +251,10 @@ void elasticity_lds_kernel(MfmaLdsArgs<T> a, T const* __restrict__ xs,
                     }
                 }
                 // this lane: facet point q; register c = value (0) / d/dxi_e (1 + e)
-                T const* G0 = fd;
-                T const* G1 = fd + DD + 2;
-                const T l0 = fd[DD], m0 = fd[DD + 1], l1 = fd[2 * DD + 2], m1 = fd[2 * DD + 3];
-                T const* n = fd + 2 * DD + 4;
+                T const* G0 = fo;
+                T const* G1 = fn;
+                const T l0 = fo[DD], m0 = fo[DD + 1], l1 = fn[DD], m1 = fn[DD + 1];
+                T const* n = fo + DD + 2;
                 T nn = T(0);
 #pragma unroll
                 for (int d = 0; d < D; ++d) nn += n[d] * n[d];
@@ -348,7 +359,8 @@ void elasticity_lds_kernel(MfmaLdsArgs<T> a, T const* __restrict__ xs,
 
 template <int D, int N, class T>
 hipError_t run(MfmaLdsArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t s) {
-    dim3 grid(unsigned((a.num_groups + kWaves - 1) / kWaves), unsigned(ncols));
+    const int64_t blocks = (a.num_groups + kWaves - 1) / kWaves;
+    dim3 grid(unsigned((blocks + 7) / 8 * 8), unsigned(ncols));
     hipLaunchKernelGGL((elasticity_lds_kernel<D, N, T>), grid, dim3(64 * kWaves), 0, s, a, x, y, ld);
     return hipGetLastError();
 }

@@ -40,4 +40,32 @@ hipError_t launch_poisson(int dim, int degree, ApplyArgs<T> const& a, T const* x
 // Volume / facet rule sizes the kernels are compiled for (2N+1 rules of tandem's autorule).
 int kernel_rule_sizes(int dim, int degree, int* Nq, int* nq);
 
+// Device views for the right-hand side and facet traction kernels (FP64).
+struct RhsArgs {
+    double const* tabE;   // [k][q]  basis values at the volume points
+    double const* jw;     // [E][Nq] |det J| W
+    double const* tabF;   // [l][c][q] face-0 facet table (elasticity)
+    double const* w;      // facet weights
+    int32_t const* perm;  // [f][l]  facet node relabelling
+    int32_t const* iperm; // [f][k]
+    double const* fG;     // [F][2][nq][D*D] per side facet geometry
+    double const* fc0;    // [F][2][nq] lambda traces
+    double const* fc1;    // [F][2][nq] mu traces
+    double const* fn;     // [F][nq][D] area-scaled normal of side 0
+    double const* fpen;   // [F] penalty
+    int4 const* ef;       // [E][D+1] facet, neighbour, flags
+    int32_t const* fbc;   // [F] BC | skeleton << 8
+    int4 const* fct;      // [F] element 0, element 1, local face 0 | local face 1 << 4
+    int64_t num_elements, num_facets;
+};
+
+// b(:, c) = rhs of force(:, c) (E x Q x Nq, may be null) and facet data(:, c) (F x Q x nq, may be
+// null) for ncols columns.
+hipError_t launch_elasticity_rhs(int dim, int degree, RhsArgs const& a, double const* force, double const* fdata,
+                                 double* b, int ncols, int64_t ld_force, int64_t ld_fdata, int64_t ld_b,
+                                 hipStream_t stream);
+// t (F x Q x nq) = facet traction of x with facet data fdata (may be null).
+hipError_t launch_elasticity_traction(int dim, int degree, RhsArgs const& a, double const* x, double const* fdata,
+                                      double* t, hipStream_t stream);
+
 } // namespace tdg

@@ -1,0 +1,52 @@
+// Element-block Jacobi preconditioner of the DG operator: the inverse of every element's diagonal
+// block A_KK (bs x bs), in FP64 and FP32.
+//
+// The blocks are probed with the matrix-free apply on a colouring of the elements (no two elements
+// sharing a facet have the same colour): with x = e_j on every element of one colour and zero
+// elsewhere, y restricted to those elements is column j of their diagonal blocks.  This is the
+// element-diagonal part of tandem's assembled matrix (assemble_volume / assemble_skeleton /
+// assemble_boundary, Elasticity.cpp:317-519, Poisson.cpp:225-381; PetscDGMatrix.cpp:14-77) that a
+// PCBJACOBI with one block per element uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace tdg {
+
+class Operator;
+
+class BlockJacobi {
+public:
+    BlockJacobi() = default;
+    BlockJacobi(BlockJacobi const&) = delete;
+    BlockJacobi& operator=(BlockJacobi const&) = delete;
+    ~BlockJacobi();
+
+    // Probes, inverts and stores the blocks (FP64 and FP32 copies) on op's stream.
+    void build(Operator& op);
+    bool built() const { return inv64_ != nullptr; }
+    int block_size() const { return bs_; }
+    int colours() const { return colours_; }
+
+    // z = D^-1 r on device vectors (element-blocked), asynchronous on stream s.
+    void apply(double const* r, double* z, hipStream_t s) const;
+    void apply(float const* r, float* z, hipStream_t s) const;
+
+    // FP64 inverse block of element e (bs x bs, column-major) copied to the host, for checks.
+    void inverse_block(int64_t e, double* out) const;
+
+private:
+    void release();
+    int bs_ = 0, colours_ = 0;
+    int64_t E_ = 0;
+    double* inv64_ = nullptr; // [E][bs][bs] column-major blocks
+    float* inv32_ = nullptr;
+};
+
+// Greedy colouring of the elements over their facet neighbours (colour per element).
+std::vector<int32_t> colour_elements(Operator const& op, int* ncolours);
+
+} // namespace tdg

@@ -1,0 +1,205 @@
+// Element-block Jacobi preconditioner: probing of the diagonal blocks with the matrix-free apply,
+// in-LDS Gauss-Jordan inversion (one workgroup per element), and the block application.
+#include "block_jacobi.h"
+
+#include "../host/operator.h"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace tdg {
+
+namespace {
+
+#define BHIP(call)                                                                                 \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kThreads = 256;
+
+// X[j][e bs + i] = 1 if element e has colour c and i = c0 + j, else 0 (j < nc)
+__global__ __launch_bounds__(kThreads) void probe_columns(double* __restrict__ X, int32_t const* __restrict__ colour,
+                                                          int c, int c0, int nc, int64_t n, int bs) {
+    const int64_t total = int64_t(nc) * n;
+    for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < total; t += int64_t(gridDim.x) * kThreads) {
+        const int64_t j = t / n, k = t % n, e = k / bs;
+        const int i = int(k % bs);
+        X[t] = (colour[e] == c && i == c0 + int(j)) ? 1.0 : 0.0;
+    }
+}
+
+// D[e][row + (c0 + j) bs] = Y[j][e bs + row] for the elements of colour c
+__global__ __launch_bounds__(kThreads) void gather_columns(double const* __restrict__ Y, int32_t const* __restrict__ colour,
+                                                           int c, int c0, int nc, int64_t n, int bs,
+                                                           double* __restrict__ Dblk) {
+    const int64_t total = int64_t(nc) * n;
+    for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < total; t += int64_t(gridDim.x) * kThreads) {
+        const int64_t j = t / n, k = t % n, e = k / bs;
+        const int row = int(k % bs);
+        if (colour[e] == c) Dblk[e * int64_t(bs) * bs + row + (c0 + j) * bs] = Y[t];
+    }
+}
+
+// In-place inverse of each (symmetrised) SPD block by Gauss-Jordan elimination without pivoting,
+// the block held in LDS; writes the FP64 inverse back and an FP32 copy.
+template <int BSMAX>
+__global__ __launch_bounds__(kThreads) void invert_blocks(double* __restrict__ Dblk, float* __restrict__ inv32, int bs) {
+    __shared__ double A[BSMAX * BSMAX];
+    __shared__ double colk[BSMAX];
+    const int64_t e = blockIdx.x;
+    double* g = Dblk + e * int64_t(bs) * bs;
+    const int nn = bs * bs;
+    for (int t = threadIdx.x; t < nn; t += kThreads) {
+        const int i = t % bs, j = t / bs;
+        A[t] = 0.5 * (g[i + j * bs] + g[j + i * bs]);
+    }
+    __syncthreads();
+    for (int k = 0; k < bs; ++k) {
+        for (int i = threadIdx.x; i < bs; i += kThreads) colk[i] = A[i + k * bs];
+        __syncthreads();
+        const double p = 1.0 / colk[k];
+        for (int j = threadIdx.x; j < bs; j += kThreads) A[k + j * bs] = j == k ? p : A[k + j * bs] * p;
+        __syncthreads();
+        for (int t = threadIdx.x; t < nn; t += kThreads) {
+            const int i = t % bs, j = t / bs;
+            if (i == k) continue;
+            A[t] = j == k ? -colk[i] * p : A[t] - colk[i] * A[k + j * bs];
+        }
+        __syncthreads();
+    }
+    float* g32 = inv32 + e * int64_t(bs) * bs;
+    for (int t = threadIdx.x; t < nn; t += kThreads) {
+        g[t] = A[t];
+        g32[t] = float(A[t]);
+    }
+}
+
+// z_e = inv_e r_e, one workgroup per element (column-major blocks: lanes read consecutive rows)
+template <class T>
+__global__ __launch_bounds__(128) void apply_blocks(T const* __restrict__ inv, T const* __restrict__ r, T* __restrict__ z,
+                                                    int bs) {
+    __shared__ T rs[128];
+    const int64_t e = blockIdx.x;
+    const int i = threadIdx.x;
+    if (i < bs) rs[i] = r[e * bs + i];
+    __syncthreads();
+    if (i >= bs) return;
+    T const* B = inv + e * int64_t(bs) * bs;
+    T acc = T(0);
+    for (int j = 0; j < bs; ++j) acc += B[i + j * bs] * rs[j];
+    z[e * bs + i] = acc;
+}
+
+unsigned grid_for(int64_t items) { return unsigned(std::min<int64_t>((items + kThreads - 1) / kThreads, 65536)); }
+
+} // namespace
+
+std::vector<int32_t> colour_elements(Operator const& op, int* ncolours) {
+    const std::size_t E = op.num_elements();
+    std::vector<std::vector<int64_t>> nb(E);
+    for (auto const& f : op.topology().facets)
+        if (f.elem[0] != f.elem[1]) {
+            nb[f.elem[0]].push_back(f.elem[1]);
+            nb[f.elem[1]].push_back(f.elem[0]);
+        }
+    std::vector<int32_t> col(E, -1);
+    int nc = 0;
+    for (std::size_t e = 0; e < E; ++e) {
+        uint64_t used = 0;
+        for (auto o : nb[e])
+            if (col[o] >= 0 && col[o] < 64) used |= uint64_t(1) << col[o];
+        int c = 0;
+        while (used & (uint64_t(1) << c)) ++c;
+        col[e] = c;
+        nc = std::max(nc, c + 1);
+    }
+    *ncolours = nc;
+    return col;
+}
+
+BlockJacobi::~BlockJacobi() { release(); }
+
+void BlockJacobi::release() {
+    if (inv64_) (void)hipFree(inv64_);
+    if (inv32_) (void)hipFree(inv32_);
+    inv64_ = nullptr;
+    inv32_ = nullptr;
+}
+
+void BlockJacobi::build(Operator& op) {
+    if (!op.ready()) throw std::logic_error("set_material() must be called before building the preconditioner");
+    release();
+    op.upload_device();
+    hipStream_t st = op.stream();
+    const int bs = int(op.block_size());
+    const int64_t E = int64_t(op.num_elements()), n = int64_t(op.local_size());
+    if (bs > 112) throw std::invalid_argument("block size above 112 is not supported by the block inversion");
+    int nc = 0;
+    std::vector<int32_t> col = colour_elements(op, &nc);
+    int32_t* dcol = nullptr;
+    double *X = nullptr, *Y = nullptr;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(bs, (int64_t(1) << 30) / (n * 8)));
+    try {
+        BHIP(hipMalloc(reinterpret_cast<void**>(&dcol), col.size() * sizeof(int32_t)));
+        BHIP(hipMemcpy(dcol, col.data(), col.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        BHIP(hipMalloc(reinterpret_cast<void**>(&inv64_), std::size_t(E) * bs * bs * sizeof(double)));
+        BHIP(hipMalloc(reinterpret_cast<void**>(&inv32_), std::size_t(E) * bs * bs * sizeof(float)));
+        BHIP(hipMalloc(reinterpret_cast<void**>(&X), std::size_t(chunk) * n * sizeof(double)));
+        BHIP(hipMalloc(reinterpret_cast<void**>(&Y), std::size_t(chunk) * n * sizeof(double)));
+        for (int c = 0; c < nc; ++c)
+            for (int c0 = 0; c0 < bs; c0 += int(chunk)) {
+                const int m = int(std::min<int64_t>(chunk, bs - c0));
+                hipLaunchKernelGGL(probe_columns, dim3(grid_for(m * n)), dim3(kThreads), 0, st, X, dcol, c, c0, m, n,
+                                   bs);
+                BHIP(hipGetLastError());
+                op.apply_batched(X, Y, m, n);
+                hipLaunchKernelGGL(gather_columns, dim3(grid_for(m * n)), dim3(kThreads), 0, st, Y, dcol, c, c0, m, n,
+                                   bs, inv64_);
+                BHIP(hipGetLastError());
+            }
+        if (bs <= 32)
+            hipLaunchKernelGGL((invert_blocks<32>), dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, inv32_, bs);
+        else if (bs <= 64)
+            hipLaunchKernelGGL((invert_blocks<64>), dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, inv32_, bs);
+        else
+            hipLaunchKernelGGL((invert_blocks<112>), dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, inv32_, bs);
+        BHIP(hipGetLastError());
+        BHIP(hipStreamSynchronize(st));
+    } catch (...) {
+        if (dcol) (void)hipFree(dcol);
+        if (X) (void)hipFree(X);
+        if (Y) (void)hipFree(Y);
+        release();
+        throw;
+    }
+    (void)hipFree(dcol);
+    (void)hipFree(X);
+    (void)hipFree(Y);
+    bs_ = bs;
+    E_ = E;
+    colours_ = nc;
+}
+
+void BlockJacobi::apply(double const* r, double* z, hipStream_t s) const {
+    if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
+    hipLaunchKernelGGL((apply_blocks<double>), dim3(unsigned(E_)), dim3(128), 0, s, inv64_, r, z, bs_);
+    BHIP(hipGetLastError());
+}
+
+void BlockJacobi::apply(float const* r, float* z, hipStream_t s) const {
+    if (!inv32_) throw std::logic_error("block Jacobi preconditioner not built");
+    hipLaunchKernelGGL((apply_blocks<float>), dim3(unsigned(E_)), dim3(128), 0, s, inv32_, r, z, bs_);
+    BHIP(hipGetLastError());
+}
+
+void BlockJacobi::inverse_block(int64_t e, double* out) const {
+    if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
+    if (e < 0 || e >= E_) throw std::out_of_range("element index out of range");
+    BHIP(hipMemcpy(out, inv64_ + e * int64_t(bs_) * bs_, std::size_t(bs_) * bs_ * sizeof(double),
+                   hipMemcpyDeviceToHost));
+}
+
+} // namespace tdg

@@ -9,6 +9,7 @@
 #include "krylov.h"
 
 #include "../host/operator.h"
+#include "block_jacobi.h"
 
 #include <hip/hip_runtime.h>
 
@@ -129,18 +130,27 @@ double read_scalar(double const* d, hipStream_t s) {
     return h;
 }
 
-// Plain CG on A z = rhs in precision T from z = 0; returns iterations.  s: device scalars [4].
+// (Preconditioned) CG on A z = rhs in precision T from z = 0; returns iterations.  s: device
+// scalars [6]: s[0], s[1] alternate rho = r.(M r), s[2] = p.Ap, s[4] = r.r when preconditioned.
+// M = null is plain CG (rho = r.r).  w: work vector for M r.
 template <class T, class TX>
-int cg(Operator& op, T const* rhs, TX* z, T* r, T* p, T* q, double* s, double* partial, int64_t n, double target_rr,
-       int max_it, int check_every, hipStream_t st, double* rr_out) {
+int cg(Operator& op, BlockJacobi const* M, T const* rhs, TX* z, T* r, T* p, T* q, T* w, double* s, double* partial,
+       int64_t n, double target_rr, int max_it, int check_every, hipStream_t st, double* rr_out) {
     const dim3 g(kGrid), b(kBlock);
     hipLaunchKernelGGL((fill<TX>), g, b, 0, st, z, TX(0), n);
     hipLaunchKernelGGL((copy_to<T>), g, b, 0, st, r, rhs, n);
-    hipLaunchKernelGGL((copy_to<T>), g, b, 0, st, p, rhs, n);
     hipLaunchKernelGGL((dot_partial<T>), g, b, 0, st, r, r, n, partial);
-    hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + 0);
+    hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (M ? 4 : 0));
+    if (M) {
+        M->apply(r, w, st);
+        hipLaunchKernelGGL((copy_to<T>), g, b, 0, st, p, w, n);
+        hipLaunchKernelGGL((dot_partial<T>), g, b, 0, st, r, w, n, partial);
+        hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + 0);
+    } else {
+        hipLaunchKernelGGL((copy_to<T>), g, b, 0, st, p, rhs, n);
+    }
     KHIP(hipGetLastError());
-    double rr = read_scalar(s + 0, st);
+    double rr = read_scalar(s + (M ? 4 : 0), st);
     if (rr <= target_rr) {
         *rr_out = rr;
         return 0;
@@ -151,18 +161,23 @@ int cg(Operator& op, T const* rhs, TX* z, T* r, T* p, T* q, double* s, double* p
         hipLaunchKernelGGL((dot_partial<T>), g, b, 0, st, p, q, n, partial);
         hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + 2);
         hipLaunchKernelGGL((cg_xr<T, TX>), g, b, 0, st, z, r, p, q, n, s, cur, 2, partial);
-        hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (1 - cur));
-        hipLaunchKernelGGL((cg_p<T>), g, b, 0, st, p, r, n, s, 1 - cur, cur);
+        hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (M ? 4 : 1 - cur));
+        if (M) {
+            M->apply(r, w, st);
+            hipLaunchKernelGGL((dot_partial<T>), g, b, 0, st, r, w, n, partial);
+            hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (1 - cur));
+        }
+        hipLaunchKernelGGL((cg_p<T>), g, b, 0, st, p, M ? w : r, n, s, 1 - cur, cur);
         cur = 1 - cur;
         ++it;
         if (it % check_every == 0 || it == max_it) {
             KHIP(hipGetLastError());
-            rr = read_scalar(s + cur, st);
+            rr = read_scalar(s + (M ? 4 : cur), st);
             if (!std::isfinite(rr)) throw std::runtime_error("CG breakdown (non-finite residual)");
             if (rr <= target_rr) break;
         }
     }
-    *rr_out = read_scalar(s + cur, st);
+    *rr_out = read_scalar(s + (M ? 4 : cur), st);
     return it;
 }
 
@@ -176,7 +191,10 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
     const int64_t n = int64_t(op.local_size());
     const dim3 g(kGrid), bl(kBlock);
     Buf<double> s, partial;
-    s.alloc(4);
+    s.alloc(6);
+    BlockJacobi const* M = nullptr;
+    if (o.precond == Precond_BlockJacobi) M = &op.block_jacobi();
+    else if (o.precond != Precond_None) throw std::invalid_argument("unknown preconditioner");
     partial.alloc(kGrid);
     SolveResult res{};
     KHIP(hipStreamSynchronize(st));
@@ -193,12 +211,13 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
     }
     const double target = o.rtol * o.rtol * bb;
     if (o.mode == Solve_FP64_CG) {
-        Buf<double> r, p, q;
+        Buf<double> r, p, q, w;
         r.alloc(n);
         p.alloc(n);
         q.alloc(n);
+        if (M) w.alloc(n);
         double rr = 0.0;
-        res.iterations = cg<double, double>(op, b, x, r.p, p.p, q.p, s.p, partial.p, n, target, o.max_iter,
+        res.iterations = cg<double, double>(op, M, b, x, r.p, p.p, q.p, w.p, s.p, partial.p, n, target, o.max_iter,
                                             o.check_every, st, &rr);
         // report the true residual ||b - A x|| / ||b||
         op.apply(x, q.p);
@@ -210,7 +229,7 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
         res.inner_iterations = res.iterations;
     } else if (o.mode == Solve_FP32_CG_FP64_IR) {
         Buf<double> r, Ax;
-        Buf<float> r32, d32, p32, q32, w32;
+        Buf<float> r32, d32, p32, q32, w32, z32;
         r.alloc(n);
         Ax.alloc(n);
         r32.alloc(n);
@@ -218,6 +237,7 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
         p32.alloc(n);
         q32.alloc(n);
         w32.alloc(n);
+        if (M) z32.alloc(n);
         // x = 0, r = b, r32 = float(b)
         hipLaunchKernelGGL((fill<double>), g, bl, 0, st, x, 0.0, n);
         hipLaunchKernelGGL((fill<double>), g, bl, 0, st, Ax.p, 0.0, n);
@@ -229,8 +249,8 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
             // FP32 CG on A d = r to relative tolerance inner_rtol, then the FP64 update x += d
             double inner_target = o.inner_rtol * o.inner_rtol * rr;
             double rr_in = 0.0;
-            int its = cg<float, float>(op, r32.p, d32.p, w32.p, p32.p, q32.p, s.p, partial.p, n, inner_target,
-                                       o.max_iter - inner_total, o.check_every, st, &rr_in);
+            int its = cg<float, float>(op, M, r32.p, d32.p, w32.p, p32.p, q32.p, z32.p, s.p, partial.p, n,
+                                       inner_target, o.max_iter - inner_total, o.check_every, st, &rr_in);
             inner_total += its;
             hipLaunchKernelGGL(add_update, g, bl, 0, st, x, d32.p, n);
             ++outer;
@@ -255,35 +275,45 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
         // iteration count stays close to FP64 CG while all operator applies but the replacements
         // run in FP32.
         Buf<double> r, Ax;
-        Buf<float> r32, xa32, p32, q32;
+        // With a preconditioner M the recursion is preconditioned CG: rho = r.(M r) in s[cur],
+        // r.r in s[4] for the replacement and stopping tests.
+        Buf<float> r32, xa32, p32, q32, z32;
         r.alloc(n);
         Ax.alloc(n);
         r32.alloc(n);
         xa32.alloc(n);
         p32.alloc(n);
         q32.alloc(n);
+        if (M) z32.alloc(n);
+        auto precondition = [&](int irho) {
+            M->apply(r32.p, z32.p, st);
+            hipLaunchKernelGGL((dot_partial<float>), g, bl, 0, st, r32.p, z32.p, n, partial.p);
+            hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + irho);
+        };
         hipLaunchKernelGGL((fill<double>), g, bl, 0, st, x, 0.0, n);
         hipLaunchKernelGGL((fill<double>), g, bl, 0, st, Ax.p, 0.0, n);
         hipLaunchKernelGGL((fill<float>), g, bl, 0, st, xa32.p, 0.0f, n);
         hipLaunchKernelGGL(residual, g, bl, 0, st, b, Ax.p, r.p, r32.p, n, partial.p);
-        hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + 0);
-        hipLaunchKernelGGL((copy_to<float>), g, bl, 0, st, p32.p, r32.p, n);
+        hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + (M ? 4 : 0));
+        if (M) precondition(0);
+        hipLaunchKernelGGL((copy_to<float>), g, bl, 0, st, p32.p, M ? z32.p : r32.p, n);
         KHIP(hipGetLastError());
-        double rr_true = read_scalar(s.p + 0, st);
+        double rr_true = read_scalar(s.p + (M ? 4 : 0), st);
         double max_r = std::sqrt(rr_true);
         int it = 0, cur = 0, updates = 0;
         while (it < o.max_iter) {
+            const int irr = M ? 4 : 1 - cur; // where r.r lands
             op.apply(p32.p, q32.p);
             hipLaunchKernelGGL((dot_partial<float>), g, bl, 0, st, p32.p, q32.p, n, partial.p);
             hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + 2);
             hipLaunchKernelGGL((cg_xr<float, float>), g, bl, 0, st, xa32.p, r32.p, p32.p, q32.p, n, s.p, cur, 2,
                                partial.p);
-            hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + (1 - cur));
+            hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + irr);
             ++it;
             bool replace = false;
             if (it % o.check_every == 0 || it == o.max_iter) {
                 KHIP(hipGetLastError());
-                double rr_it = read_scalar(s.p + (1 - cur), st);
+                double rr_it = read_scalar(s.p + irr, st);
                 if (!std::isfinite(rr_it)) throw std::runtime_error("CG breakdown (non-finite residual)");
                 replace = std::sqrt(rr_it) < o.reliable_delta * max_r || rr_it <= target || it == o.max_iter;
             }
@@ -292,14 +322,15 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
                 hipLaunchKernelGGL((fill<float>), g, bl, 0, st, xa32.p, 0.0f, n);
                 op.apply(x, Ax.p);
                 hipLaunchKernelGGL(residual, g, bl, 0, st, b, Ax.p, r.p, r32.p, n, partial.p);
-                hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + (1 - cur));
+                hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + irr);
                 KHIP(hipGetLastError());
-                rr_true = read_scalar(s.p + (1 - cur), st);
+                rr_true = read_scalar(s.p + irr, st);
                 max_r = std::sqrt(rr_true);
                 ++updates;
                 if (rr_true <= target) break;
             }
-            hipLaunchKernelGGL((cg_p<float>), g, bl, 0, st, p32.p, r32.p, n, s.p, 1 - cur, cur);
+            if (M) precondition(1 - cur);
+            hipLaunchKernelGGL((cg_p<float>), g, bl, 0, st, p32.p, M ? z32.p : r32.p, n, s.p, 1 - cur, cur);
             cur = 1 - cur;
         }
         res.rel_residual = std::sqrt(rr_true / bb);

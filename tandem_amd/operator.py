@@ -166,25 +166,74 @@ class DGOperator:
         check(lib().tdg_upload(self._h))
 
     def solve(self, b, x, rtol=1e-10, mode="fp64", inner_rtol=1e-4, max_iter=200000, max_outer=60,
-              check_every=10, reliable_delta=0.1):
+              check_every=10, reliable_delta=0.1, precond=None):
         """Solve A x = b (float64 GPU tensors).  mode "fp64": FP64 CG; "fp32-ir": FP32 CG solves
         restarted inside FP64 iterative refinement; "fp32-ru": FP32 CG with FP64 residual
-        replacement (refinement that keeps the Krylov direction).  Returns the statistics."""
+        replacement (refinement that keeps the Krylov direction).  precond: None or
+        "block-jacobi".  Returns the statistics."""
         assert b.is_cuda and x.is_cuda and b.dtype == x.dtype and b.numel() == self.n == x.numel()
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ir": 1, "fp32-ru": 2}[mode], inner_rtol, max_outer,
-                              check_every, reliable_delta)
+                              check_every, reliable_delta, {None: 0, "block-jacobi": 1}[precond])
         res = _lib.SolveResult()
         check(lib().tdg_solve(self._h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), C.byref(opts),
                               C.byref(res)))
         return {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
 
-    def rhs(self, force_Q=None, dirichlet_q=None):
-        """FP64 right-hand side on the host (force: (E, Q, Nq); Dirichlet data: (F, Q, nq))."""
+    def rhs(self, force_Q=None, facet_q=None):
+        """FP64 right-hand side on the host.  force: (E, Q, Nq); facet data (F, Q, nq): Dirichlet
+        values on Dirichlet facets, slip on Fault facets."""
         b = np.zeros(self.n)
         f = None if force_Q is None else np.ascontiguousarray(force_Q, dtype=np.float64)
-        d = None if dirichlet_q is None else np.ascontiguousarray(dirichlet_q, dtype=np.float64)
+        d = None if facet_q is None else np.ascontiguousarray(facet_q, dtype=np.float64)
         check(lib().tdg_rhs(self._h, ptr(f), ptr(d), b))
         return b
+
+    def rhs_device(self, b, force=None, facet=None):
+        """FP64 right-hand side on the GPU into b (elasticity).  Tensors on the GPU: force (E*Q*Nq)
+        or (ncols, >= E*Q*Nq), facet data (F*Q*nq) or (ncols, ...), b (n) or (ncols, >= n)."""
+        torch = _torch()
+        ncols = 1 if b.dim() == 1 else b.shape[0]
+
+        def view(t, width):
+            if t is None:
+                return None, 0
+            assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+            if ncols == 1:
+                assert t.numel() >= width
+                return C.c_void_p(t.data_ptr()), width
+            assert t.dim() == 2 and t.shape[0] == ncols and t.shape[1] >= width
+            return C.c_void_p(t.data_ptr()), t.shape[1]
+
+        fp, lf = view(force, self.E * self.Q * self.Nq)
+        dp, ld = view(facet, self.F * self.Q * self.nq)
+        bp, lb = view(b, self.n)
+        check(lib().tdg_rhs_device(self._h, fp, dp, bp, ncols, lf, ld, lb))
+        return b
+
+    def traction(self, x, facet=None, t=None):
+        """Facet traction (F, Q, nq) of the GPU vector x (float64) with facet data (may be None)."""
+        torch = _torch()
+        if t is None:
+            t = torch.empty(self.F * self.Q * self.nq, dtype=torch.float64, device=x.device)
+        fp = None if facet is None else C.c_void_p(facet.data_ptr())
+        check(lib().tdg_traction(self._h, C.c_void_p(x.data_ptr()), fp, C.c_void_p(t.data_ptr())))
+        return t.view(self.F, self.Q, self.nq)
+
+    def block_jacobi_build(self):
+        """Build the element-block Jacobi preconditioner; returns the number of element colours."""
+        c = C.c_int()
+        check(lib().tdg_block_jacobi_build(self._h, C.byref(c)))
+        return c.value
+
+    def block_jacobi_apply(self, r, z):
+        """z = D^-1 r (float64 GPU tensors)."""
+        check(lib().tdg_block_jacobi_apply(self._h, C.c_void_p(r.data_ptr()), C.c_void_p(z.data_ptr())))
+        return z
+
+    def block_jacobi_inverse(self, element):
+        out = np.zeros(self.bs * self.bs)
+        check(lib().tdg_block_jacobi_inverse(self._h, element, out))
+        return out.reshape(self.bs, self.bs, order="F")
 
     def apply_counts(self, precision_bytes=8):
         b, f = C.c_double(), C.c_double()

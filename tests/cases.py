@@ -47,3 +47,20 @@ class Case:
 
 def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def fault_plane_mesh(D, n, scheme, bc=BC_D, fault_bc=3, x0=0.5):
+    """Box mesh (product and oracle) whose interior facets on the plane x = x0 carry fault_bc
+    (tandem tags faults through the facet BC list, DGOperatorTopo.cpp:13-109)."""
+    pm = T.Mesh.box(D, n, scheme, bc)
+    v = pm.vertices
+    faces = set()
+    for el in pm.elements:
+        for skip in range(D + 1):
+            f = tuple(sorted(np.delete(el, skip)))
+            if np.all(np.abs(v[list(f), 0] - x0) < 1e-12):
+                faces.add(f)
+    faces = np.array(sorted(faces), dtype=np.int64).reshape(-1, D)
+    bf = np.concatenate([pm.boundary_facets, faces])
+    bbc = np.concatenate([pm.boundary_bc, np.full(len(faces), fault_bc, dtype=np.int32)])
+    return T.Mesh(v, pm.elements, bf, bbc), O.Mesh.from_arrays(v, pm.elements, bf, bbc)

@@ -96,3 +96,27 @@ def test_boundary_errors():
     bb = T.Mesh(m.vertices, m.elements, m.boundary_facets, np.full(len(m.boundary_bc), 4, dtype=np.int32))
     with pytest.raises(L.TdgError):
         T.DGOperator(bb, 0, 2)
+
+
+@pytest.mark.parametrize("kind,D,N,scheme", [(0, 3, 2, 1), (0, 2, 4, 0), (1, 2, 2, 0), (1, 3, 2, 0)])
+def test_rhs_with_fault_and_boundary_fault_matches_oracle(kind, D, N, scheme):
+    """Host rhs (rhs_volume + rhs_skeleton + rhs_boundary) on a mesh with an interior fault plane and
+    Fault/Dirichlet/natural box faces against the oracle."""
+    from cases import fault_plane_mesh
+    bcs = [5, 3, 1, 5, 5, 3][:2 * D]
+    pm, om = fault_plane_mesh(D, 2 if D == 3 else 4, scheme, bc=bcs)
+    nodes = warp_nodes(pm.affine_nodes(N), 0.02)
+    op = T.DGOperator(pm, kind, N, nodes)
+    vc = op.volume_points()
+    c0 = 2 + 0.3 * np.sin(3 * vc[..., 0]) * np.cos(2 * vc[..., 1])
+    c1 = 1 + 0.2 * np.cos(2 * vc[..., D - 1])
+    op.set_material(c0, c1 if kind == 0 else None)
+    oo = O.Operator(kind, om, N, c0, c1 if kind == 0 else None, nodes=nodes)
+    assert np.array_equal(op.facets(), oo.facets())
+    assert np.sum(oo.facets()[:, 4] == 3) > 0
+    Q = D if kind == 0 else 1
+    rng = np.random.default_rng(2)
+    f = rng.uniform(-1, 1, (op.E, Q, op.Nq))
+    g = rng.uniform(-1, 1, (op.F, Q, op.nq))
+    rb, ro = op.rhs(f, g), oo.rhs(f, g)
+    assert np.abs(rb - ro).max() <= 1e-12 * np.abs(ro).max()

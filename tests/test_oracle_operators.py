@@ -193,3 +193,120 @@ def test_elasticity_mms_convergence():
     h = 1.0 / np.array([2, 3, 4])
     rates = np.log(np.array(errs[:-1]) / np.array(errs[1:])) / np.log(h[:-1] / h[1:])
     assert rates[-1] > 2.4 and rates[-1] < 3.6, rates
+
+
+def _fault_mesh(D, cells, scheme, fault_bc=3):
+    """Box mesh with the interior plane x = 1/2 tagged with fault_bc (tandem tags faults through
+    the facet BC list, as for boundaries; DGOperatorTopo.cpp:13-109)."""
+    m = O.Mesh.box(D, [cells] * D, scheme, [BC_D] * (2 * D))
+    v = m.verts
+    faces = []
+    for el in m.elems:
+        for skip in range(D + 1):
+            f = np.delete(el, skip)
+            if np.all(np.abs(v[f, 0] - 0.5) < 1e-12):
+                faces.append(np.sort(f))
+    faces = np.unique(np.array(faces), axis=0)
+    bf = np.concatenate([m.bfacets, faces])
+    bbc = np.concatenate([m.bbc, np.full(len(faces), fault_bc, dtype=np.int32)])
+    return O.Mesh.from_arrays(v, m.elems, bf, bbc)
+
+
+@pytest.mark.parametrize("kind,D,N", [(0, 3, 2), (0, 2, 3), (1, 2, 2), (1, 3, 1)])
+def test_fault_slip_rhs_reproduces_jump(kind, D, N):
+    """rhs_skeleton / bc_skeleton with slip data (Elasticity.cpp:545-643, Poisson.cpp:405-495):
+    a field that is constant on each side of a fault plane, with jump s = u(side 0) - u(side 1)
+    imposed as slip on the fault and as Dirichlet data on the box, solves the homogeneous problem
+    (stress vanishes, tractions are continuous), so SIPG reproduces it exactly.  Pins the skeleton
+    factors eps/2, the +-penalty of the two sides and the use of the side-0 normal on both."""
+    m = _fault_mesh(D, 2 if D == 3 else 4, 1 if D == 3 else 0)
+    vc, fcoord = m.coords(N)
+    Q = D if kind == 0 else 1
+    c0 = np.full(vc.shape[:2], 2.0 if kind == 0 else 1.5)
+    c1 = np.full(vc.shape[:2], 1.0) if kind == 0 else None
+    op = O.Operator(kind, m, N, c0, c1)
+    s = np.array([0.3, -0.2, 0.7][:Q])
+    # element-wise constants: 0 left of the fault, s right of it
+    right = vc[:, :, 0].mean(axis=1) > 0.5
+    uel = np.where(right[:, None], s[None, :], 0.0)            # (E, Q)
+    fct = op.facets()
+    fq = np.zeros((op.F, Q, op.nq))
+    nfault = 0
+    for f, (e0, e1, l0, l1, bc) in enumerate(fct):
+        if bc == 3 and e0 != e1:
+            fq[f] = (uel[e0] - uel[e1])[:, None]
+            nfault += 1
+        elif bc == BC_D:
+            fq[f] = uel[e0][:, None]
+    assert nfault > 0
+    b = op.rhs(None, fq)
+    x = np.linalg.solve(op.assemble(), b)
+    u, _ = op.eval_volume(x)                                   # (E, Q, Nq)
+    err = np.abs(u - uel[:, :, None]).max()
+    assert err < 1e-10, err
+    # a wrong sign of the jump must not reproduce the field
+    fq_bad = fq.copy()
+    for f, (e0, e1, l0, l1, bc) in enumerate(fct):
+        if bc == 3 and e0 != e1:
+            fq_bad[f] *= -1.0
+    xb = np.linalg.solve(op.assemble(), op.rhs(None, fq_bad))
+    ub, _ = op.eval_volume(xb)
+    assert np.abs(ub - uel[:, :, None]).max() > 1e-2
+
+
+@pytest.mark.parametrize("kind,D,N", [(0, 3, 2), (1, 2, 2)])
+def test_boundary_fault_takes_half_slip(kind, D, N):
+    """bc_boundary (Elasticity.cpp:559-573, Poisson.cpp:418-431): on a boundary Fault facet the slip
+    is halved, so its rhs equals the Dirichlet rhs of half the data."""
+    mD = O.Mesh.box(D, [2] * D, 0, [BC_D] * (2 * D))
+    mF = O.Mesh.box(D, [2] * D, 0, [3] * (2 * D))
+    vc, _ = mD.coords(N)
+    c0 = np.full(vc.shape[:2], 2.0)
+    c1 = np.full(vc.shape[:2], 1.0) if kind == 0 else None
+    oD, oF = O.Operator(kind, mD, N, c0, c1), O.Operator(kind, mF, N, c0, c1)
+    Q = D if kind == 0 else 1
+    fq = np.random.default_rng(4).uniform(-1, 1, (oD.F, Q, oD.nq))
+    np.testing.assert_allclose(oF.rhs(None, fq), oD.rhs(None, 0.5 * fq), rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("D,N,scheme", [(3, 2, 1), (2, 3, 0)])
+def test_traction_of_linear_field_is_sigma_n(D, N, scheme):
+    """compute_traction / compute_traction_bnd (elasticity.py:242-250): for u = A x (in the P_N space,
+    constant stress sigma = lam tr(A) I + mu (A + A^T)) with matching Dirichlet data, the jump and the
+    data term vanish and the traction is sigma n_unit on every facet, skeleton or boundary (affine
+    and curved meshes alike, since u is interpolated exactly)."""
+    m = O.Mesh.box(D, [2] * D, scheme, [BC_D] * (2 * D))
+    nodes = warp_nodes(m.affine_nodes(N), 0.02)
+    vc, fc = m.coords(N, nodes)
+    lam, mu = 2.0, 1.0
+    op = O.Operator(0, m, N, np.full(vc.shape[:2], lam), np.full(vc.shape[:2], mu), nodes=nodes)
+    A = np.random.default_rng(3).uniform(-1, 1, (D, D))
+    # nodal interpolation of u = A x on the warped element nodes (the geometry nodes are the
+    # solution nodes, same degree)
+    x = np.einsum("ij,epj->eip", A, nodes).reshape(-1)
+    sig = lam * np.trace(A) * np.eye(D) + mu * (A + A.T)
+    nu = op.array("unitNormal").reshape(op.F, op.nq, D)
+    facet_data = np.einsum("ij,fqj->fiq", A, fc)
+    for f in range(op.F):
+        t = op.traction(f, x, facet_data[f])
+        np.testing.assert_allclose(t, (nu[f] @ sig.T).T, atol=1e-10)
+
+
+def test_traction_vanishes_for_slip_field():
+    """A field constant on each side of a fault with the jump given as slip: zero stress and zero
+    penalty term, so the fault traction is zero (pins the sign of the slip in the jump)."""
+    m = _fault_mesh(3, 2, 1)
+    vc, _ = m.coords(2)
+    op = O.Operator(0, m, 2, np.full(vc.shape[:2], 2.0), np.full(vc.shape[:2], 1.0))
+    s = np.array([0.3, -0.2, 0.7])
+    uel = np.where((vc[:, :, 0].mean(axis=1) > 0.5)[:, None], s[None, :], 0.0)
+    x = np.repeat(uel[:, :, None], op.bs // 3, axis=2).reshape(-1)
+    fct = op.facets()
+    nf = 0
+    for f, (e0, e1, l0, l1, bc) in enumerate(fct):
+        if bc == 3 and e0 != e1:
+            fq = np.repeat((uel[e0] - uel[e1])[:, None], op.nq, axis=1)
+            assert np.abs(op.traction(f, x, fq)).max() < 1e-11
+            assert np.abs(op.traction(f, x, -fq)).max() > 1e-3
+            nf += 1
+    assert nf > 0
