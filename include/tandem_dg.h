@@ -89,13 +89,13 @@ int tdg_mult_batched_f32(tdg_op* op, float const* X, float* Y, int ncols, int64_
 /* Stream all device work of this operator is queued on (hipStream_t; NULL = default stream). */
 int tdg_set_stream(tdg_op* op, void* stream);
 
-/* Kernel choice: TDG_VARIANT_AUTO (default) runs the matrix-core (MFMA) kernel with its tables
- * staged in LDS where one is compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6) and the
- * thread-per-point kernel otherwise; TDG_VARIANT_POINTS always runs the thread-per-point kernel;
- * TDG_VARIANT_MFMA_TABLES the MFMA kernel reading pre-swizzled operand tables from memory.  All
- * compute the same operator (the reference's element-centric apply); they differ in summation
- * order only. */
-enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1, TDG_VARIANT_MFMA_TABLES = 2 };
+/* Kernel choice: TDG_VARIANT_AUTO (default) runs the matrix-core (MFMA) kernel where one is
+ * compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6), with workgroups renumbered so
+ * that each XCD takes one contiguous range of elements, and the thread-per-point kernel otherwise;
+ * TDG_VARIANT_POINTS always runs the thread-per-point kernel; TDG_VARIANT_MFMA_LINEAR runs the MFMA
+ * kernel with workgroups in launch order.  All compute the same operator (the reference's
+ * element-centric apply); the two kernels differ in summation order only. */
+enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1, TDG_VARIANT_MFMA_LINEAR = 2 };
 int tdg_set_variant(tdg_op* op, int variant);
 
 /* Right-hand side b (host, local size) from a body force at the volume points (elements x Q x Nq,
@@ -150,6 +150,16 @@ typedef struct {
     int converged;
 } tdg_solve_result;
 int tdg_solve(tdg_op* op, double const* b, double* x, tdg_solve_opts const* opts, tdg_solve_result* result);
+
+/* ncols right-hand sides solved together (device FP64, column c at B + c ld, ld >= local size):
+ * ncols CG recursions in lockstep, each iteration one batched apply over all columns (the
+ * discrete Green's function's slip right-hand sides, solved one by one in
+ * SeasQDDiscreteGreenOperator.cpp:540-549).  opts->mode TDG_SOLVE_FP64_CG or
+ * TDG_SOLVE_FP32_CG_FP64_RELIABLE (per-column FP64 residual replacement); opts->precond as for
+ * tdg_solve.  Per column, iterations[c] and rel_residual[c] (host arrays of ncols, may be NULL);
+ * result->iterations = lockstep iterations, result->outer_iterations = replacements. */
+int tdg_solve_multi(tdg_op* op, double const* B, double* X, int ncols, int64_t ld, tdg_solve_opts const* opts,
+                    tdg_solve_result* result, int* iterations, double* rel_residual);
 
 /* Element-block Jacobi preconditioner (the inverse of each element's diagonal block of A, probed
  * with the matrix-free apply on an element colouring; assemble_volume/skeleton/boundary restricted

@@ -193,7 +193,7 @@ int tdg_set_stream(tdg_op* op, void* stream) {
 int tdg_set_variant(tdg_op* op, int variant) {
     return guarded([&] {
         require(op && op->impl, "null operator");
-        require(variant == TDG_VARIANT_AUTO || variant == TDG_VARIANT_POINTS || variant == TDG_VARIANT_MFMA_TABLES,
+        require(variant == TDG_VARIANT_AUTO || variant == TDG_VARIANT_POINTS || variant == TDG_VARIANT_MFMA_LINEAR,
                 "unknown kernel variant");
         op->impl->set_variant(variant);
     });
@@ -229,32 +229,60 @@ int tdg_traction(tdg_op* op, double const* x, double const* facet_q, double* t) 
     });
 }
 
+static tdg::SolveOptions solve_options(tdg_solve_opts const* opts) {
+    tdg::SolveOptions so;
+    so.rtol = opts->rtol;
+    so.max_iter = opts->max_iter;
+    so.mode = opts->mode;
+    so.inner_rtol = opts->inner_rtol;
+    so.max_outer = opts->max_outer;
+    so.check_every = opts->check_every;
+    so.reliable_delta = opts->reliable_delta;
+    so.precond = opts->precond;
+    require(so.precond == TDG_PRECOND_NONE || so.precond == TDG_PRECOND_BLOCK_JACOBI, "unknown preconditioner");
+    require(so.mode == TDG_SOLVE_FP64_CG || so.mode == TDG_SOLVE_FP32_CG_FP64_IR ||
+                so.mode == TDG_SOLVE_FP32_CG_FP64_RELIABLE,
+            "unknown solve mode");
+    require(so.rtol > 0 && so.max_iter > 0 && so.check_every > 0 && so.max_outer > 0 &&
+                (so.mode != TDG_SOLVE_FP32_CG_FP64_IR || (so.inner_rtol > 0 && so.inner_rtol < 1)) &&
+                (so.mode != TDG_SOLVE_FP32_CG_FP64_RELIABLE || (so.reliable_delta > 0 && so.reliable_delta < 1)),
+            "bad solver options");
+    return so;
+}
+
 int tdg_solve(tdg_op* op, double const* b, double* x, tdg_solve_opts const* opts, tdg_solve_result* result) {
     return guarded([&] {
         auto& o = ready(op);
         require(b && x && opts && result && b != x, "bad solve arguments");
-        tdg::SolveOptions so;
-        so.rtol = opts->rtol;
-        so.max_iter = opts->max_iter;
-        so.mode = opts->mode;
-        so.inner_rtol = opts->inner_rtol;
-        so.max_outer = opts->max_outer;
-        so.check_every = opts->check_every;
-        so.reliable_delta = opts->reliable_delta;
-        so.precond = opts->precond;
-        require(so.precond == TDG_PRECOND_NONE || so.precond == TDG_PRECOND_BLOCK_JACOBI, "unknown preconditioner");
-        require(so.mode == TDG_SOLVE_FP64_CG || so.mode == TDG_SOLVE_FP32_CG_FP64_IR ||
-                    so.mode == TDG_SOLVE_FP32_CG_FP64_RELIABLE,
-                "unknown solve mode");
-        require(so.rtol > 0 && so.max_iter > 0 && so.check_every > 0 && so.max_outer > 0 &&
-                    (so.mode != TDG_SOLVE_FP32_CG_FP64_IR || (so.inner_rtol > 0 && so.inner_rtol < 1)) &&
-                    (so.mode != TDG_SOLVE_FP32_CG_FP64_RELIABLE || (so.reliable_delta > 0 && so.reliable_delta < 1)),
-                "bad solver options");
-        auto r = tdg::solve(o, b, x, so);
+        auto r = tdg::solve(o, b, x, solve_options(opts));
         result->iterations = r.iterations;
         result->outer_iterations = r.outer_iterations;
         result->inner_iterations = r.inner_iterations;
         result->rel_residual = r.rel_residual;
+        result->seconds = r.seconds;
+        result->converged = r.converged;
+    });
+}
+
+int tdg_solve_multi(tdg_op* op, double const* B, double* X, int ncols, int64_t ld, tdg_solve_opts const* opts,
+                    tdg_solve_result* result, int* iterations, double* rel_residual) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(B && X && opts && result && B != X && ncols >= 1 && ld >= int64_t(o.local_size()),
+                "bad multi-column solve arguments");
+        auto so = solve_options(opts);
+        require(so.mode != TDG_SOLVE_FP32_CG_FP64_IR, "multi-column solves support modes 0 and 2");
+        auto r = tdg::solve_multi(o, B, X, ncols, ld, so);
+        result->iterations = r.total_iterations;
+        result->outer_iterations = r.replacements;
+        result->inner_iterations = r.total_iterations;
+        double worst = 0.0;
+        for (int c = 0; c < ncols; ++c) {
+            worst = std::max(worst, r.rel_residual[c]);
+            if (iterations) iterations[c] = r.iterations[c];
+            if (rel_residual) rel_residual[c] = r.rel_residual[c];
+        }
+        result->rel_residual = worst;
         result->seconds = r.seconds;
         result->converged = r.converged;
     });

@@ -406,7 +406,7 @@ void Operator::upload_device() {
 }
 
 void Operator::set_variant(int v) {
-    if (v != Variant_Auto && v != Variant_Points && v != Variant_MfmaRegs)
+    if (v != Variant_Auto && v != Variant_Points && v != Variant_MfmaLinear)
         throw std::invalid_argument("unknown kernel variant");
     variant_ = v;
 }
@@ -415,16 +415,16 @@ bool Operator::uses_mfma() const {
     return variant_ != Variant_Points && kind_ == Kind_Elasticity && mfma_supported(D_, N_);
 }
 
-// Operands and point data of the MFMA kernel (see kernels/dg_apply_mfma.hip for the layouts).
+// Tables and element-grouped point data of the MFMA kernel (kernels/dg_apply_lds.hip): canonical
+// reference tables V[e][q][k] and T0[c][q][k] (face 0, c = 0 value, 1 + e gradient), the facet
+// relabellings, and per group of 16 elements (element innermost) the volume point data VD, the
+// one-side facet records FS, penalties FP and face info FI.
 template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
     const unsigned D = D_, DD = D * D, NF = D + 1;
     const MfmaDims L{int(D), int(Nbf_), int(Nq_), int(nq_)};
     auto T0 = [&](int c, int jb, int q) -> double {
-        if (c > int(D)) return 0.0;
         return c == 0 ? E_q_[0][jb + Nbf_ * q] : Dxi_q_[0][jb + Nbf_ * ((c - 1) + D * q)];
     };
-    std::vector<double> A1(L.a1(), 0.0), A2(L.a2(), 0.0), AF(L.af(), 0.0), AL(L.al(), 0.0);
-    // canonical tables: V[e][q][k] and T0[c][q][k] (c <= D)
     std::vector<double> V(std::size_t(D) * Nq_ * Nbf_), TT(std::size_t(NF) * nq_ * Nbf_);
     for (unsigned e = 0; e < D; ++e)
         for (unsigned q = 0; q < Nq_; ++q)
@@ -434,47 +434,9 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
             for (unsigned k = 0; k < Nbf_; ++k) TT[(c * nq_ + q) * Nbf_ + k] = T0(int(c), int(k), int(q));
     std::vector<int32_t> P2(perm_);
     P2.insert(P2.end(), iperm_.begin(), iperm_.end());
-    for (int e = 0; e < int(D); ++e)
-        for (int b = 0; b < L.QB; ++b)
-            for (int s = 0; s < L.KS; ++s)
-                for (int lane = 0; lane < 64; ++lane) {
-                    int q = 16 * b + lane % 16, k = 4 * s + lane / 16;
-                    if (q < int(Nq_) && k < int(Nbf_))
-                        A1[((e * L.QB + b) * L.KS + s) * 64 + lane] = Dxi_Q_[k + Nbf_ * (e + D * q)];
-                }
-    for (int m = 0; m < L.MT; ++m)
-        for (int e = 0; e < int(D); ++e)
-            for (int b = 0; b < L.QB; ++b)
-                for (int i = 0; i < 4; ++i)
-                    for (int lane = 0; lane < 64; ++lane) {
-                        // K sub-index lane/16 = the lane group whose register i holds this point
-                        int k = 16 * m + lane % 16, q = 16 * b + mfma_row<T>(lane / 16, i);
-                        if (q < int(Nq_) && k < int(Nbf_))
-                            A2[(m * (int(D) * L.QB * 4) + (e * L.QB + b) * 4 + i) * 64 + lane] =
-                                Dxi_Q_[k + Nbf_ * (e + D * q)];
-                    }
-    for (unsigned f = 0; f < NF; ++f) {
-        for (int t = 0; t < L.FT; ++t)
-            for (int s = 0; s < L.KS; ++s)
-                for (int lane = 0; lane < 64; ++lane) {
-                    // tile row rho lands in lane group mfma_group(rho), register mfma_reg(rho):
-                    // make that (facet point 4t + group, channel = register)
-                    int row = lane % 16, q = 4 * t + mfma_group<T>(row), c = mfma_reg<T>(row), k = 4 * s + lane / 16;
-                    if (q < int(nq_) && k < int(Nbf_))
-                        AF[((f * L.FT + t) * L.KS + s) * 64 + lane] = T0(c, iperm_[f * Nbf_ + k], q);
-                }
-        for (int m = 0; m < L.MT; ++m)
-            for (int t = 0; t < L.FT; ++t)
-                for (int c = 0; c < 4; ++c)
-                    for (int lane = 0; lane < 64; ++lane) {
-                        int k = 16 * m + lane % 16, q = 4 * t + lane / 16;
-                        if (q < int(nq_) && k < int(Nbf_))
-                            AL[((f * L.MT + m) * (L.FT * 4) + t * 4 + c) * 64 + lane] = T0(c, iperm_[f * Nbf_ + k], q);
-                    }
-    }
     // element-grouped point data
     const std::size_t ng = (E_ + 15) / 16;
-    std::vector<T> VD(ng * Nq_ * L.NCH * 16, T(0)), FD(ng * NF * nq_ * L.NFC * 16, T(0)), FP(ng * NF * 16, T(0));
+    std::vector<T> VD(ng * Nq_ * L.NCH * 16, T(0)), FP(ng * NF * 16, T(0));
     std::vector<T> FS(ng * NF * nq_ * L.NFS * 16, T(0));
     std::vector<int32_t> FI(ng * NF * 16 * 4, 0);
 #pragma omp parallel for schedule(static)
@@ -495,19 +457,8 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
             fi[1] = ef[1];
             FP[(grp * NF + f) * 16 + j] = T(penalty_[fct]);
             for (unsigned q = 0; q < nq_; ++q) {
-                T* o = &FD[(((grp * NF + f) * nq_ + q) * L.NFC) * 16 + j];
                 double const* g0 = &fGp_[((fct * 2 + side) * nq_ + q) * DD];
-                double const* g1 = &fGp_[((fct * 2 + (1 - side)) * nq_ + q) * DD];
-                for (unsigned c = 0; c < DD; ++c) {
-                    o[c * 16] = T(g0[c]);
-                    o[(DD + 2 + c) * 16] = T(g1[c]);
-                }
-                o[DD * 16] = T(fc0_[(fct * 2 + side) * nq_ + q]);
-                o[(DD + 1) * 16] = T(fc1_[(fct * 2 + side) * nq_ + q]);
-                o[(2 * DD + 2) * 16] = T(fc0_[(fct * 2 + (1 - side)) * nq_ + q]);
-                o[(2 * DD + 3) * 16] = T(fc1_[(fct * 2 + (1 - side)) * nq_ + q]);
                 double sg = side ? -1.0 : 1.0;
-                for (unsigned d = 0; d < D; ++d) o[(2 * DD + 4 + d) * 16] = T(sg * normal_[(fct * nq_ + q) * D + d]);
                 T* os = &FS[(((grp * NF + f) * nq_ + q) * L.NFS) * 16 + j];
                 for (unsigned c = 0; c < DD; ++c) os[c * 16] = T(g0[c]);
                 os[DD * 16] = T(fc0_[(fct * 2 + side) * nq_ + q]);
@@ -520,14 +471,8 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
     dm.V.upload(cast<T>(V));
     dm.T0.upload(cast<T>(TT));
     dm.perm2.upload(P2);
-    dm.A1.upload(cast<T>(A1));
-    dm.A2.upload(cast<T>(A2));
-    dm.AF.upload(cast<T>(AF));
-    dm.AL.upload(cast<T>(AL));
     dm.w.upload(cast<T>(fct_.weights));
-    dm.perm.upload(perm_);
     dm.VD.upload(VD);
-    dm.FD.upload(FD);
     dm.FP.upload(FP);
     dm.FI.upload(FI);
 }
@@ -803,26 +748,9 @@ template <> DeviceOperator<float> const& Operator::device<float>() {
 
 template <class T>
 void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld) {
-    MfmaArgs<T> a;
-    a.A1 = d.A1.ptr;
-    a.A2 = d.A2.ptr;
-    a.AF = d.AF.ptr;
-    a.AL = d.AL.ptr;
-    a.w = d.w.ptr;
-    a.perm = d.perm.ptr;
-    a.VD = d.VD.ptr;
-    a.FD = d.FD.ptr;
-    a.FP = d.FP.ptr;
-    a.FI = reinterpret_cast<int4 const*>(d.FI.ptr);
-    a.num_elements = int64_t(E_);
-    a.num_groups = int64_t((E_ + 15) / 16);
     int kq = 0, kf = 0;
     if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
         throw std::runtime_error("no kernel compiled for this dimension/degree");
-    if (variant_ == Variant_MfmaRegs) {
-        TDG_HIP(launch_elasticity_mfma<T>(D_, N_, a, x, y, ncols, ld, stream_));
-        return;
-    }
     MfmaLdsArgs<T> b;
     b.V = d.V.ptr;
     b.T0 = d.T0.ptr;
@@ -831,10 +759,10 @@ void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, 
     b.VD = d.VD.ptr;
     b.FS = d.FS.ptr;
     b.FP = d.FP.ptr;
-    b.FI = a.FI;
-    b.num_elements = a.num_elements;
-    b.num_groups = a.num_groups;
-    b.xcd_remap = std::getenv("TDG_XCD_REMAP") ? std::atoi(std::getenv("TDG_XCD_REMAP")) : 1;
+    b.FI = reinterpret_cast<int4 const*>(d.FI.ptr);
+    b.num_elements = int64_t(E_);
+    b.num_groups = int64_t((E_ + 15) / 16);
+    b.xcd_remap = variant_ == Variant_MfmaLinear ? 0 : 1;
     TDG_HIP(launch_elasticity_lds<T>(D_, N_, b, x, y, ncols, ld, stream_));
 }
 

@@ -45,10 +45,9 @@ template <class T> struct DeviceOperator {
 
 // Operand tables and element-grouped point data of the MFMA elasticity kernel (one precision).
 template <class T> struct DeviceMfma {
-    DeviceArray<T> A1, A2, AF, AL, w, VD, FD, FP;
-    DeviceArray<T> V, T0, FS;    // canonical tables and one-side facet records of the LDS-staged kernel
-    DeviceArray<int32_t> perm, FI;
-    DeviceArray<int32_t> perm2;  // perm_f followed by iperm_f
+    DeviceArray<T> V, T0, w;       // canonical reference tables, facet weights
+    DeviceArray<T> VD, FS, FP;     // volume point data, one-side facet records, penalties
+    DeviceArray<int32_t> perm2, FI; // perm_f followed by iperm_f; face info
 };
 
 // Extra FP64 data of the right-hand side and traction kernels (elasticity).
@@ -57,10 +56,10 @@ struct DeviceRhs {
     DeviceArray<int32_t> fbc, fct; // fct: int4 per facet
 };
 
-// Kernel selection: Variant_Auto uses the LDS-staged MFMA kernel where one is compiled
-// (elasticity), the thread-per-point kernel otherwise; Variant_Points forces the thread-per-point
-// kernel; Variant_MfmaRegs the MFMA kernel with pre-swizzled operand tables read from memory.
-enum Variant : int { Variant_Auto = 0, Variant_Points = 1, Variant_MfmaRegs = 2 };
+// Kernel selection: Variant_Auto uses the MFMA kernel where one is compiled (elasticity; workgroups
+// renumbered per XCD), the thread-per-point kernel otherwise; Variant_Points forces the
+// thread-per-point kernel; Variant_MfmaLinear the MFMA kernel with workgroups in launch order.
+enum Variant : int { Variant_Auto = 0, Variant_Points = 1, Variant_MfmaLinear = 2 };
 
 class BlockJacobi;
 

@@ -386,6 +386,10 @@ hipError_t launch_elasticity_lds(int dim, int degree, MfmaLdsArgs<T> const& a, T
     return hipErrorInvalidValue;
 }
 
+bool mfma_supported(int dim, int degree) {
+    return (dim == 3 && degree >= 2 && degree <= 4) || (dim == 2 && (degree == 2 || degree == 4 || degree == 6));
+}
+
 template hipError_t launch_elasticity_lds<double>(int, int, MfmaLdsArgs<double> const&, double const*, double*, int,
                                                   int64_t, hipStream_t);
 template hipError_t launch_elasticity_lds<float>(int, int, MfmaLdsArgs<float> const&, float const*, float*, int,

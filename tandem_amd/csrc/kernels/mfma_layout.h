@@ -12,18 +12,12 @@ namespace tdg {
 //   KS = ceil(Nbf / 4)   K-steps over the basis,         MT = ceil(Nbf / 16) basis tiles,
 //   QB = ceil(Nq / 16)   16-point volume blocks,          FT = ceil(nq / 4)   4-point facet tiles,
 //   NCH = D*D + 2        volume channels (JInv, lamWJ, muWJ),
-//   NFC = 2 D*D + 4 + D  facet channels (own G', own lam, mu, neighbour G', lam, mu, normal),
 //   NFS = D*D + 2 + D    facet channels of one side (G', lam, mu, oriented normal).
 struct MfmaDims {
-    int D, Nbf, Nq, nq, KS, QB, MT, FT, NCH, NFC, NFS;
+    int D, Nbf, Nq, nq, KS, QB, MT, FT, NCH, NFS;
     __host__ __device__ constexpr MfmaDims(int D_, int Nbf_, int Nq_, int nq_)
         : D(D_), Nbf(Nbf_), Nq(Nq_), nq(nq_), KS((Nbf_ + 3) / 4), QB((Nq_ + 15) / 16), MT((Nbf_ + 15) / 16),
-          FT((nq_ + 3) / 4), NCH(D_ * D_ + 2), NFC(2 * D_ * D_ + 4 + D_), NFS(D_ * D_ + 2 + D_) {}
-    // table sizes (entries)
-    __host__ __device__ constexpr long a1() const { return long(D) * QB * KS * 64; }
-    __host__ __device__ constexpr long a2() const { return long(MT) * D * QB * 4 * 64; }
-    __host__ __device__ constexpr long af() const { return long(D + 1) * FT * KS * 64; }
-    __host__ __device__ constexpr long al() const { return long(D + 1) * MT * FT * 4 * 64; }
+          FT((nq_ + 3) / 4), NCH(D_ * D_ + 2), NFS(D_ * D_ + 2 + D_) {}
 };
 
 namespace detail {
@@ -40,8 +34,7 @@ template <int D_, int N_> struct MfmaLayout {
     static constexpr int Nq = detail::vol_points(D_, N_);
     static constexpr int nq = detail::fct_points(D_, N_);
     static constexpr MfmaDims dims{D_, Nbf, Nq, nq};
-    static constexpr int KS = dims.KS, QB = dims.QB, MT = dims.MT, FT = dims.FT, NCH = dims.NCH, NFC = dims.NFC,
-                         NFS = dims.NFS;
+    static constexpr int KS = dims.KS, QB = dims.QB, MT = dims.MT, FT = dims.FT, NCH = dims.NCH, NFS = dims.NFS;
 };
 
 // Output row of a 16x16x4 MFMA tile held in register i (0..3) of lane group g = lane / 16:
@@ -53,27 +46,10 @@ template <class T> __host__ __device__ constexpr int mfma_row(int g, int i) {
 template <class T> __host__ __device__ constexpr int mfma_group(int rho) { return sizeof(T) == 8 ? rho % 4 : rho / 4; }
 template <class T> __host__ __device__ constexpr int mfma_reg(int rho) { return sizeof(T) == 8 ? rho / 4 : rho % 4; }
 
-template <class T> struct MfmaArgs {
-    T const* A1;    // [e][b][s][lane]      volume gradient operands
-    T const* A2;    // [m][(e,b,i)][lane]   volume lift operands
-    T const* AF;    // [f][t][s][lane]      facet trace operands (face relabelling folded in)
-    T const* AL;    // [f][m][(t,c)][lane]  facet lift operands
-    T const* w;     // facet weights
-    int32_t const* perm; // [f][k] facet node relabelling
-    T const* VD;    // [group][q][NCH][16]
-    T const* FD;    // [group][f][q][NFC][16]
-    T const* FP;    // [group][f][16]  penalty
-    int4 const* FI; // [group][f][16]  {kind | neighbour face << 4, neighbour element, -, -}
-    int64_t num_elements, num_groups;
-};
-
-template <class T>
-hipError_t launch_elasticity_mfma(int dim, int degree, MfmaArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld,
-                                  hipStream_t stream);
 bool mfma_supported(int dim, int degree);
 
-// Arguments of the LDS-staged MFMA kernel: canonical tables (copied to LDS by every block) and the
-// same element-grouped point data as MfmaArgs.
+// Arguments of the MFMA kernel: canonical tables (copied to LDS by every block) and the
+// element-grouped point data.
 template <class T> struct MfmaLdsArgs {
     T const* V;          // [e][q][k]  d phi_k / d xi_e at volume point q
     T const* T0;         // [c][q][k]  face-0 value (c = 0) and d/dxi_(c-1) at facet point q

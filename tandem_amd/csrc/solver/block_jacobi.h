@@ -31,9 +31,10 @@ public:
     int block_size() const { return bs_; }
     int colours() const { return colours_; }
 
-    // z = D^-1 r on device vectors (element-blocked), asynchronous on stream s.
-    void apply(double const* r, double* z, hipStream_t s) const;
-    void apply(float const* r, float* z, hipStream_t s) const;
+    // z = D^-1 r on device vectors (element-blocked), asynchronous on stream s; ncols columns with
+    // leading dimension ld.
+    void apply(double const* r, double* z, hipStream_t s, int ncols = 1, int64_t ld = 0) const;
+    void apply(float const* r, float* z, hipStream_t s, int ncols = 1, int64_t ld = 0) const;
 
     // FP64 inverse block of element e (bs x bs, column-major) copied to the host, for checks.
     void inverse_block(int64_t e, double* out) const;

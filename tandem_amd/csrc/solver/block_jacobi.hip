@@ -79,11 +79,13 @@ __global__ __launch_bounds__(kThreads) void invert_blocks(double* __restrict__ D
 
 // z_e = inv_e r_e, one workgroup per element (column-major blocks: lanes read consecutive rows)
 template <class T>
-__global__ __launch_bounds__(128) void apply_blocks(T const* __restrict__ inv, T const* __restrict__ r, T* __restrict__ z,
-                                                    int bs) {
+__global__ __launch_bounds__(128) void apply_blocks(T const* __restrict__ inv, T const* __restrict__ rs_, T* __restrict__ zs,
+                                                    int bs, int64_t ld) {
     __shared__ T rs[128];
     const int64_t e = blockIdx.x;
     const int i = threadIdx.x;
+    T const* __restrict__ r = rs_ + int64_t(blockIdx.y) * ld;
+    T* __restrict__ z = zs + int64_t(blockIdx.y) * ld;
     if (i < bs) rs[i] = r[e * bs + i];
     __syncthreads();
     if (i >= bs) return;
@@ -183,15 +185,17 @@ void BlockJacobi::build(Operator& op) {
     colours_ = nc;
 }
 
-void BlockJacobi::apply(double const* r, double* z, hipStream_t s) const {
+void BlockJacobi::apply(double const* r, double* z, hipStream_t s, int ncols, int64_t ld) const {
     if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
-    hipLaunchKernelGGL((apply_blocks<double>), dim3(unsigned(E_)), dim3(128), 0, s, inv64_, r, z, bs_);
+    hipLaunchKernelGGL((apply_blocks<double>), dim3(unsigned(E_), unsigned(ncols)), dim3(128), 0, s, inv64_, r, z, bs_,
+                       ld);
     BHIP(hipGetLastError());
 }
 
-void BlockJacobi::apply(float const* r, float* z, hipStream_t s) const {
+void BlockJacobi::apply(float const* r, float* z, hipStream_t s, int ncols, int64_t ld) const {
     if (!inv32_) throw std::logic_error("block Jacobi preconditioner not built");
-    hipLaunchKernelGGL((apply_blocks<float>), dim3(unsigned(E_)), dim3(128), 0, s, inv32_, r, z, bs_);
+    hipLaunchKernelGGL((apply_blocks<float>), dim3(unsigned(E_), unsigned(ncols)), dim3(128), 0, s, inv32_, r, z, bs_,
+                       ld);
     BHIP(hipGetLastError());
 }
 

@@ -1,6 +1,9 @@
 // GPU Krylov solvers over the matrix-free operator.
 #pragma once
 
+#include <cstdint>
+#include <vector>
+
 namespace tdg {
 
 class Operator;
@@ -30,5 +33,18 @@ struct SolveResult {
 
 // b, x: device FP64 vectors of op.local_size(); x is overwritten (initial guess 0).
 SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& opts);
+
+struct MultiSolveResult {
+    int total_iterations = 0;          // lockstep iterations (fused applies)
+    int replacements = 0;              // FP64 residual replacements summed over columns
+    std::vector<int> iterations;       // per column: iterations until it converged
+    std::vector<double> rel_residual;  // per column: FP64 true relative residual
+    double seconds = 0.0;
+    int converged = 0;                 // all columns
+};
+
+// ncols right-hand sides B (column c at B + c ld) solved together, each iteration one batched
+// apply; modes Solve_FP64_CG and Solve_FP32_CG_FP64_RELIABLE.  X overwritten (initial guess 0).
+MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols, int64_t ld, SolveOptions const& o);
 
 } // namespace tdg

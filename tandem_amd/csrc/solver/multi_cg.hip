@@ -1,0 +1,369 @@
+// Multi-right-hand-side conjugate gradients: ncols independent CG recursions advanced in lockstep,
+// so every iteration costs one fused (batched) operator apply over all columns.  Used for the
+// discrete Green's function, whose right-hand sides are the slip basis functions of the fault
+// (tandem's SeasQDDiscreteGreenOperator solves them one at a time, app/common/
+// SeasQDDiscreteGreenOperator.cpp:540-549).
+//
+// Per column: its own scalars (rho, p.Ap, r.r), its own convergence (a converged column is frozen:
+// its step lengths are zero), and in the FP32 mode its own FP64 residual replacements (the
+// reliable-update scheme of the single-vector solver, applied to the columns whose residual
+// dropped by reliable_delta at a check).  Reductions are two-stage with a fixed grid (bitwise
+// reproducible).  Vectors are column-major: column c at offset c ld.
+#include "krylov.h"
+
+#include "../host/operator.h"
+#include "block_jacobi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace tdg {
+
+namespace {
+
+#define MHIP(call)                                                                                 \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kB = 256;
+constexpr int kG = 256;  // partial sums per column
+constexpr int kS = 8;    // scalars per column: 0,1 rho; 2 p.Ap; 3 b.b; 4 r.r; 5 active; 6 replace
+
+__device__ inline double bsum(double v) {
+    __shared__ double s[kB / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kB / 64; ++i) r += s[i];
+    return r;
+}
+
+template <class A, class B>
+__global__ __launch_bounds__(kB) void mdot(A const* __restrict__ a, B const* __restrict__ b, int64_t n, int64_t ld,
+                                           double* __restrict__ partial) {
+    const int64_t c = blockIdx.y;
+    A const* ac = a + c * ld;
+    B const* bc = b + c * ld;
+    double v = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
+        v += double(ac[i]) * double(bc[i]);
+    v = bsum(v);
+    if (threadIdx.x == 0) partial[c * kG + blockIdx.x] = v;
+}
+
+// S[c][slot] = sum of the column's partials (only where mask slot is set, if msk >= 0)
+__global__ __launch_bounds__(kB) void mfinal(double const* __restrict__ partial, double* __restrict__ S, int slot,
+                                             int msk) {
+    const int c = blockIdx.x;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < kG; i += kB) v += partial[c * kG + i];
+    v = bsum(v);
+    if (threadIdx.x == 0 && (msk < 0 || S[c * kS + msk] != 0.0)) S[c * kS + slot] = v;
+}
+
+// active columns: x += alpha p, r -= alpha q, alpha = rho / pAp; partial sums of r.r
+template <class T, class TX>
+__global__ __launch_bounds__(kB) void mxr(TX* __restrict__ x, T* __restrict__ r, T const* __restrict__ p,
+                                          T const* __restrict__ q, int64_t n, int64_t ld, double const* __restrict__ S,
+                                          int irho, double* __restrict__ partial) {
+    const int64_t c = blockIdx.y;
+    double const* s = S + c * kS;
+    const bool act = s[5] != 0.0;
+    const T a = act ? T(s[irho] / s[2]) : T(0);
+    TX* xc = x + c * ld;
+    T* rc = r + c * ld;
+    T const* pc = p + c * ld;
+    T const* qc = q + c * ld;
+    double v = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) {
+        T ri = rc[i];
+        if (act) {
+            xc[i] += TX(a) * TX(pc[i]);
+            ri = ri - a * qc[i];
+            rc[i] = ri;
+        }
+        v += double(ri) * double(ri);
+    }
+    v = bsum(v);
+    if (threadIdx.x == 0) partial[c * kG + blockIdx.x] = v;
+}
+
+// active columns: p = z + (rho_new / rho_old) p
+template <class T>
+__global__ __launch_bounds__(kB) void mp(T* __restrict__ p, T const* __restrict__ z, int64_t n, int64_t ld,
+                                         double const* __restrict__ S, int inew, int iold) {
+    const int64_t c = blockIdx.y;
+    double const* s = S + c * kS;
+    if (s[5] == 0.0) return;
+    const T beta = T(s[inew] / s[iold]);
+    T* pc = p + c * ld;
+    T const* zc = z + c * ld;
+    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
+        pc[i] = zc[i] + beta * pc[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(kB) void mcopy(T* __restrict__ d, T const* __restrict__ s, int64_t n, int64_t ld) {
+    const int64_t c = blockIdx.y;
+    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
+        d[c * ld + i] = s[c * ld + i];
+}
+template <class T>
+__global__ __launch_bounds__(kB) void mfill(T* __restrict__ d, T v, int64_t n, int64_t ld) {
+    const int64_t c = blockIdx.y;
+    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) d[c * ld + i] = v;
+}
+
+// columns flagged for replacement (S[c][6]): x += xa (FP64), xa = 0
+__global__ __launch_bounds__(kB) void mfold(double* __restrict__ x, float* __restrict__ xa, int64_t n, int64_t ld,
+                                            double const* __restrict__ S) {
+    const int64_t c = blockIdx.y;
+    if (S[c * kS + 6] == 0.0) return;
+    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) {
+        x[c * ld + i] += double(xa[c * ld + i]);
+        xa[c * ld + i] = 0.0f;
+    }
+}
+
+// flagged columns (or all if all != 0): r = b - Ax in FP64, r32 = float(r); partial sums of r.r
+__global__ __launch_bounds__(kB) void mresidual(double const* __restrict__ b, double const* __restrict__ Ax,
+                                                double* __restrict__ r, float* __restrict__ r32, int64_t n, int64_t ld,
+                                                double const* __restrict__ S, int all, double* __restrict__ partial) {
+    const int64_t c = blockIdx.y;
+    const bool on = all || S[c * kS + 6] != 0.0;
+    double v = 0.0;
+    if (on)
+        for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) {
+            const double ri = b[c * ld + i] - Ax[c * ld + i];
+            r[c * ld + i] = ri;
+            if (r32) r32[c * ld + i] = float(ri);
+            v += ri * ri;
+        }
+    v = bsum(v);
+    if (threadIdx.x == 0) partial[c * kG + blockIdx.x] = v;
+}
+
+template <class T> struct DBuf {
+    T* p = nullptr;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(std::size_t n) { MHIP(hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T))); }
+};
+
+} // namespace
+
+MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols, int64_t ld, SolveOptions const& o) {
+    if (!op.ready()) throw std::logic_error("set_material() must be called before solve");
+    if (ncols < 1 || ld < int64_t(op.local_size())) throw std::invalid_argument("bad column layout");
+    if (o.mode != Solve_FP64_CG && o.mode != Solve_FP32_CG_FP64_RELIABLE)
+        throw std::invalid_argument("multi-column solves support FP64 CG and FP32 CG with FP64 replacement");
+    op.upload_device();
+    BlockJacobi const* M = nullptr;
+    if (o.precond == Precond_BlockJacobi) M = &op.block_jacobi();
+    else if (o.precond != Precond_None) throw std::invalid_argument("unknown preconditioner");
+    hipStream_t st = op.stream();
+    const int64_t n = int64_t(op.local_size());
+    const std::size_t tot = std::size_t(ncols) * ld;
+    const dim3 g(kG, ncols), bl(kB);
+    DBuf<double> S, partial;
+    S.alloc(std::size_t(ncols) * kS);
+    partial.alloc(std::size_t(ncols) * kG);
+    std::vector<double> hs(std::size_t(ncols) * kS, 0.0);
+    MultiSolveResult res;
+    res.rel_residual.assign(ncols, 0.0);
+    res.iterations.assign(ncols, 0);
+    MHIP(hipStreamSynchronize(st));
+    auto t0 = std::chrono::steady_clock::now();
+    auto pull = [&] {
+        MHIP(hipMemcpyAsync(hs.data(), S.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        MHIP(hipStreamSynchronize(st));
+    };
+    auto push = [&] {
+        MHIP(hipMemcpyAsync(S.p, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    };
+    // b.b per column, all columns active
+    for (int c = 0; c < ncols; ++c) hs[c * kS + 5] = 1.0;
+    push();
+    hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, B, B, n, ld, partial.p);
+    hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 3, -1);
+    MHIP(hipGetLastError());
+    pull();
+    std::vector<double> target(ncols);
+    for (int c = 0; c < ncols; ++c) {
+        target[c] = o.rtol * o.rtol * hs[c * kS + 3];
+        if (hs[c * kS + 3] == 0.0) hs[c * kS + 5] = 0.0;
+    }
+    hipLaunchKernelGGL((mfill<double>), g, bl, 0, st, X, 0.0, n, ld);
+
+    if (o.mode == Solve_FP64_CG) {
+        DBuf<double> r, p, q, z;
+        r.alloc(tot);
+        p.alloc(tot);
+        q.alloc(tot);
+        if (M) z.alloc(tot);
+        push();
+        hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, r.p, B, n, ld);
+        hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, n, ld, partial.p);
+        hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
+        auto prec = [&](int slot) {
+            if (M) {
+                M->apply(r.p, z.p, st, ncols, ld);
+                hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, z.p, n, ld, partial.p);
+            } else {
+                hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, n, ld, partial.p);
+            }
+            hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, slot, -1);
+        };
+        prec(0);
+        hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, p.p, M ? z.p : r.p, n, ld);
+        MHIP(hipGetLastError());
+        int it = 0, cur = 0;
+        while (it < o.max_iter) {
+            op.apply_batched(p.p, q.p, ncols, ld);
+            hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, p.p, q.p, n, ld, partial.p);
+            hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 2, -1);
+            hipLaunchKernelGGL((mxr<double, double>), g, bl, 0, st, X, r.p, p.p, q.p, n, ld, S.p, cur, partial.p);
+            hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
+            prec(1 - cur);
+            hipLaunchKernelGGL((mp<double>), g, bl, 0, st, p.p, M ? z.p : r.p, n, ld, S.p, 1 - cur, cur);
+            cur = 1 - cur;
+            ++it;
+            if (it % o.check_every == 0 || it == o.max_iter) {
+                MHIP(hipGetLastError());
+                pull();
+                int active = 0;
+                for (int c = 0; c < ncols; ++c) {
+                    if (hs[c * kS + 5] == 0.0) continue;
+                    const double rr = hs[c * kS + 4];
+                    if (!std::isfinite(rr)) throw std::runtime_error("CG breakdown (non-finite residual)");
+                    res.iterations[c] = it;
+                    if (rr <= target[c]) hs[c * kS + 5] = 0.0;
+                    else ++active;
+                }
+                push();
+                if (!active) break;
+            }
+        }
+        res.total_iterations = it;
+        // true residuals
+        op.apply_batched(X, q.p, ncols, ld);
+        hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, q.p, r.p, (float*)nullptr, n, ld, S.p, 1, partial.p);
+        hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
+        MHIP(hipGetLastError());
+        pull();
+    } else {
+        DBuf<double> r, Ax;
+        DBuf<float> r32, xa, p, q, z;
+        r.alloc(tot);
+        Ax.alloc(tot);
+        r32.alloc(tot);
+        xa.alloc(tot);
+        p.alloc(tot);
+        q.alloc(tot);
+        if (M) z.alloc(tot);
+        push();
+        hipLaunchKernelGGL((mfill<float>), g, bl, 0, st, xa.p, 0.0f, n, ld);
+        hipLaunchKernelGGL((mfill<double>), g, bl, 0, st, Ax.p, 0.0, n, ld);
+        hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, Ax.p, r.p, r32.p, n, ld, S.p, 1, partial.p);
+        hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
+        // rho into slot `slot` of the columns flagged in slot msk (all if msk < 0)
+        auto prec = [&](int slot, int msk) {
+            if (M) {
+                M->apply(r32.p, z.p, st, ncols, ld);
+                hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, z.p, n, ld, partial.p);
+            } else {
+                hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, r32.p, n, ld, partial.p);
+            }
+            hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, slot, msk);
+        };
+        prec(0, -1);
+        hipLaunchKernelGGL((mcopy<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, n, ld);
+        MHIP(hipGetLastError());
+        pull();
+        std::vector<double> maxr(ncols);
+        for (int c = 0; c < ncols; ++c) maxr[c] = std::sqrt(hs[c * kS + 4]);
+        int it = 0, cur = 0;
+        res.replacements = 0;
+        while (it < o.max_iter) {
+            op.apply_batched(p.p, q.p, ncols, ld);
+            hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, p.p, q.p, n, ld, partial.p);
+            hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 2, -1);
+            hipLaunchKernelGGL((mxr<float, float>), g, bl, 0, st, xa.p, r32.p, p.p, q.p, n, ld, S.p, cur, partial.p);
+            hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
+            ++it;
+            bool any_replace = false;
+            int active = 0;
+            if (it % o.check_every == 0 || it == o.max_iter) {
+                MHIP(hipGetLastError());
+                pull();
+                for (int c = 0; c < ncols; ++c) {
+                    hs[c * kS + 6] = 0.0;
+                    if (hs[c * kS + 5] == 0.0) continue;
+                    const double rr = hs[c * kS + 4];
+                    if (!std::isfinite(rr)) throw std::runtime_error("CG breakdown (non-finite residual)");
+                    res.iterations[c] = it;
+                    if (std::sqrt(rr) < o.reliable_delta * maxr[c] || rr <= target[c] || it == o.max_iter) {
+                        hs[c * kS + 6] = 1.0;
+                        any_replace = true;
+                    }
+                }
+                push();
+            }
+            if (any_replace) {
+                // FP64 replacement of the flagged columns: x += xa, r = b - A x
+                hipLaunchKernelGGL(mfold, g, bl, 0, st, X, xa.p, n, ld, S.p);
+                op.apply_batched(X, Ax.p, ncols, ld);
+                hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, Ax.p, r.p, r32.p, n, ld, S.p, 0, partial.p);
+                hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, 6);
+                MHIP(hipGetLastError());
+                pull();
+                for (int c = 0; c < ncols; ++c) {
+                    if (hs[c * kS + 6] == 0.0) continue;
+                    ++res.replacements;
+                    const double rr = hs[c * kS + 4];
+                    maxr[c] = std::sqrt(rr);
+                    if (rr <= target[c] || it == o.max_iter) hs[c * kS + 5] = 0.0;
+                }
+                push();
+            }
+            for (int c = 0; c < ncols; ++c)
+                if (hs[c * kS + 5] != 0.0) ++active;
+            if ((any_replace || it % o.check_every == 0) && !active) break;
+            prec(1 - cur, -1);
+            hipLaunchKernelGGL((mp<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, n, ld, S.p, 1 - cur, cur);
+            cur = 1 - cur;
+        }
+        res.total_iterations = it;
+        // fold any remaining update and report the true residuals
+        for (int c = 0; c < ncols; ++c) hs[c * kS + 6] = 1.0;
+        push();
+        hipLaunchKernelGGL(mfold, g, bl, 0, st, X, xa.p, n, ld, S.p);
+        op.apply_batched(X, Ax.p, ncols, ld);
+        hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, Ax.p, r.p, (float*)nullptr, n, ld, S.p, 1, partial.p);
+        hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
+        MHIP(hipGetLastError());
+        pull();
+    }
+    res.converged = 1;
+    for (int c = 0; c < ncols; ++c) {
+        const double bb = hs[c * kS + 3];
+        res.rel_residual[c] = bb > 0.0 ? std::sqrt(hs[c * kS + 4] / bb) : 0.0;
+        if (!(res.rel_residual[c] <= o.rtol)) res.converged = 0;
+    }
+    MHIP(hipStreamSynchronize(st));
+    res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return res;
+}
+
+} // namespace tdg

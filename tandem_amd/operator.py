@@ -130,9 +130,9 @@ class DGOperator:
         self._material = True
 
     def set_variant(self, variant):
-        """"auto" (LDS-staged matrix-core kernel where compiled), "points" (thread-per-point
-        kernel) or "mfma-tables" (matrix-core kernel with operand tables read from memory)."""
-        check(lib().tdg_set_variant(self._h, {"auto": 0, "points": 1, "mfma-tables": 2}[variant]))
+        """"auto" (matrix-core kernel where compiled, workgroups renumbered per XCD), "points"
+        (thread-per-point kernel) or "mfma-linear" (matrix-core kernel, workgroups in launch order)."""
+        check(lib().tdg_set_variant(self._h, {"auto": 0, "points": 1, "mfma-linear": 2}[variant]))
 
     def set_stream(self, stream):
         """stream: a torch.cuda.Stream (or None for the default stream)."""
@@ -178,6 +178,26 @@ class DGOperator:
         check(lib().tdg_solve(self._h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), C.byref(opts),
                               C.byref(res)))
         return {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
+
+    def solve_multi(self, B, X, rtol=1e-10, mode="fp64", max_iter=200000, check_every=10, reliable_delta=0.1,
+                    precond=None):
+        """Solve A X[c] = B[c] for the rows c of (ncols, ld) float64 GPU tensors together (one batched
+        apply per iteration).  mode "fp64" or "fp32-ru".  Returns the statistics with per-column
+        iterations and relative residuals."""
+        assert B.is_cuda and X.is_cuda and B.shape == X.shape and B.dim() == 2 and B.shape[1] >= self.n
+        assert B.dtype == X.dtype and B.is_contiguous() and X.is_contiguous()
+        nc, ld = B.shape
+        opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ru": 2}[mode], 1e-4, 1, check_every, reliable_delta,
+                              {None: 0, "block-jacobi": 1}[precond])
+        res = _lib.SolveResult()
+        its = np.zeros(nc, dtype=np.int32)
+        rel = np.zeros(nc)
+        check(lib().tdg_solve_multi(self._h, C.c_void_p(B.data_ptr()), C.c_void_p(X.data_ptr()), nc, ld,
+                                    C.byref(opts), C.byref(res), its, rel))
+        out = {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
+        out["column_iterations"] = its
+        out["column_rel_residual"] = rel
+        return out
 
     def rhs(self, force_Q=None, facet_q=None):
         """FP64 right-hand side on the host.  force: (E, Q, Nq); facet data (F, Q, nq): Dirichlet

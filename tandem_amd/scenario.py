@@ -144,3 +144,103 @@ def poisson_sine(X):
     D = X.shape[-1]
     u = np.prod(np.sin(np.pi * X), axis=-1)
     return u, D * np.pi ** 2 * u
+
+
+class FaultRectangle:
+    """2D plane-strain Green's-function problem: the rectangle [0, W] x [-H, 0] with a planar fault
+    dipping at `dip` degrees from the surface point (x0, 0), made of `nfault` straight facets.
+
+    Triangulation: a seeded, jittered point cloud of spacing h (exact points on the rectangle's
+    sides and on the fault, none closer than `band` h to the fault) triangulated by Delaunay; the
+    fault segments are then edges of the triangulation (checked).  BCs: the surface y = 0 is free
+    (natural), the other sides are Dirichlet, the fault facets are Fault.  (Mesh and BC choices are
+    this project's; the reference reads such meshes from gmsh.)
+    """
+
+    def __init__(self, W=400.0, H=200.0, h=0.9, nfault=512, fault_h=0.5, dip=30.0, x0=80.0, seed=20261205,
+                 band=0.6):
+        from scipy.spatial import Delaunay
+
+        rng = np.random.default_rng(seed)
+        d = np.array([np.cos(np.radians(dip)), -np.sin(np.radians(dip))])
+        fault = np.array([x0, 0.0])[None, :] + fault_h * np.arange(nfault + 1)[:, None] * d[None, :]
+        if fault[-1, 1] <= -H or fault[-1, 0] >= W:
+            raise ValueError("fault leaves the rectangle")
+        nx, ny = int(round(W / h)), int(round(H / h))
+        xs, ys = np.linspace(0.0, W, nx + 1), np.linspace(-H, 0.0, ny + 1)
+        side = [np.stack([xs, np.full_like(xs, -H)], 1), np.stack([xs, np.zeros_like(xs)], 1),
+                np.stack([np.zeros_like(ys), ys], 1), np.stack([np.full_like(ys, W), ys], 1)]
+        gx, gy = np.meshgrid(xs[1:-1], ys[1:-1], indexing="ij")
+        inner = np.stack([gx.ravel(), gy.ravel()], 1) + rng.uniform(-0.3 * h, 0.3 * h, (gx.size, 2))
+        # drop points close to the fault segment (distance to the segment, not the line)
+        rel = inner - fault[0]
+        t = np.clip(rel @ d, 0.0, fault_h * nfault)
+        dist = np.linalg.norm(rel - t[:, None] * d[None, :], axis=1)
+        inner = inner[dist > band * h]
+        surf = side[1]
+        surf = surf[np.abs(surf[:, 0] - x0) > 0.5 * h]  # the fault's surface point replaces its neighbour
+        pts = np.concatenate([side[0], surf, side[2], side[3], inner, fault])
+        pts = np.unique(np.round(pts, 12), axis=0)
+        tri = Delaunay(pts)
+        elems = np.sort(tri.simplices.astype(np.int64), axis=1)
+        # keep non-degenerate triangles
+        p = pts[elems]
+        area = 0.5 * np.abs((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1]) -
+                            (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+        elems = elems[area > 1e-12 * h * h]
+        index = {tuple(np.round(q, 12)): i for i, q in enumerate(pts)}
+        fv = np.array([index[tuple(np.round(q, 12))] for q in fault])
+        fedges = np.sort(np.stack([fv[:-1], fv[1:]], 1), axis=1)
+        edges = np.sort(np.concatenate([elems[:, [0, 1]], elems[:, [0, 2]], elems[:, [1, 2]]]), axis=1)
+        eset = set(map(tuple, edges))
+        missing = [tuple(e) for e in fedges if tuple(e) not in eset]
+        if missing:
+            raise RuntimeError("%d fault segments are not triangulation edges" % len(missing))
+        # boundary edges (appear once) with their BC
+        uniq, cnt = np.unique(edges, axis=0, return_counts=True)
+        bnd = uniq[cnt == 1]
+        mid = pts[bnd].mean(axis=1)
+        bc = np.where(np.abs(mid[:, 1]) < 1e-9, 1, 5).astype(np.int32)
+        self.vertices, self.elements = pts, elems
+        self.boundary_facets = np.concatenate([bnd, fedges])
+        self.boundary_bc = np.concatenate([bc, np.full(len(fedges), 3, dtype=np.int32)])
+        self.fault_vertices = fv
+        self.tangent = d
+
+    def mesh(self):
+        from .operator import Mesh
+        return Mesh(self.vertices, self.elements, self.boundary_facets, self.boundary_bc)
+
+    def slip_basis(self, op, columns=None):
+        """Facet data (ncols, F, 2, nq) of the slip basis functions: facet-nodal Lagrange polynomials
+        of degree N on each fault facet (nodes at the facet's warp-and-blend = Gauss-Lobatto points),
+        times the along-fault unit tangent, oriented so that the slip is the hanging wall's
+        displacement minus the footwall's ([[u]] = u(side 0) - u(side 1), hence the sign by side).
+        Column c = (fault facet c // (N+1), node c % (N+1)); `columns` selects a subset."""
+        N = op.degree
+        fct = op.facets()                        # (F, 5): elem0, elem1, local0, local1, bc
+        Xf = op.facet_points()                   # (F, nq, 2)
+        faults = np.nonzero(fct[:, 4] == 3)[0]
+        # Gauss-Lobatto points on [0, 1]: the end points and the roots of P_N'
+        lg = np.polynomial.legendre.Legendre.basis(N).deriv().roots()
+        nodes1d = np.sort(np.concatenate([[0.0, 1.0], 0.5 * (np.real(lg) + 1.0)]))
+        ncols_all = len(faults) * (N + 1)
+        cols = np.arange(ncols_all) if columns is None else np.asarray(columns)
+        out = np.zeros((len(cols), op.F, 2, op.nq))
+        cent = self.vertices[self.elements].mean(axis=1)
+        nrm = np.array([-self.tangent[1], self.tangent[0]])  # points to the hanging wall (up)
+        for j, c in enumerate(cols):
+            f = faults[c // (N + 1)]
+            i = c % (N + 1)
+            # facet parameter of the quadrature points along the segment between its end vertices
+            v = self.vertices[self.boundary_facets[self.boundary_bc == 3]]
+            seg = v[np.argmin(np.linalg.norm(v.mean(axis=1) - Xf[f].mean(axis=0), axis=1))]
+            t = (Xf[f] - seg[0]) @ (seg[1] - seg[0]) / np.dot(seg[1] - seg[0], seg[1] - seg[0])
+            li = np.ones_like(t)
+            for m in range(N + 1):
+                if m != i:
+                    li *= (t - nodes1d[m]) / (nodes1d[i] - nodes1d[m])
+            side0_up = np.dot(cent[fct[f, 0]] - Xf[f].mean(axis=0), nrm) > 0
+            sgn = 1.0 if side0_up else -1.0
+            out[j, f] = sgn * li[None, :] * self.tangent[:, None]
+        return out
