@@ -135,6 +135,257 @@ def solve_comparison(op, mesh, nodes, fields, rtol=1e-10):
     return out
 
 
+def _boundary_facets(op):
+    f = op.facets()
+    return np.nonzero(f[:, 0] == f[:, 1])[0]
+
+
+def run_precision(args):
+    """FP32 MFMA + FP64 residual replacement vs all-FP64 solve on the P4 tet mesh at cells^3 (the
+    precision study: 40^3 = 384,000 elements by default), manufactured cosine solution with the
+    seeded variable Lame fields, block-Jacobi preconditioned CG to 1e-10.  Errors of the FP32-MFMA
+    solution against the FP64 solution (whose apply matches the CPU FP64 oracle to 1e-12 in the
+    parity tests) for the solution and for the traction on the boundary faces, and both solutions'
+    errors against the manufactured solution."""
+    import torch
+    from tandem_amd import scenario as S
+
+    t0 = time.perf_counter()
+    op, mesh, nodes, (lam, mu) = build_workload(args.cells)
+    op.upload()
+    X = op.volume_points()
+    f = S.elasticity_body_force(X, lam, mu)                       # (E, Nq, 3)
+    Xf = op.facet_points()
+    g, _ = S.cosine_solution(Xf)                                  # (F, nq, 3)
+    force = torch.from_numpy(np.ascontiguousarray(f.transpose(0, 2, 1)).ravel()).cuda()
+    gdat = torch.from_numpy(np.ascontiguousarray(g.transpose(0, 2, 1)).ravel()).cuda()
+    b = torch.empty(op.n, dtype=torch.float64, device="cuda")
+    op.rhs_device(b, force, gdat)
+    del force
+    log("precision: setup %.1fs, %d elements, %d DoF" % (time.perf_counter() - t0, op.E, op.n))
+    t1 = time.perf_counter()
+    ncol = op.block_jacobi_build()
+    torch.cuda.synchronize()
+    t_pc = time.perf_counter() - t1
+    out = {"workload": "P4 tet elasticity, %d^3 Kuhn cubes x 6 = %d elements, %d DoF" % (args.cells, op.E, op.n),
+           "block_jacobi_setup_s": t_pc, "colours": ncol}
+    xs = {}
+    for mode in ("fp64", "fp32-ru"):
+        x = torch.zeros_like(b)
+        r = op.solve(b, x, rtol=1e-10, mode=mode, precond="block-jacobi", max_iter=100000)
+        log("precision solve", mode, r)
+        out[mode] = {k: r[k] for k in ("seconds", "iterations", "outer_iterations", "rel_residual", "converged")}
+        xs[mode] = x
+    out["speedup_fp32ru_vs_fp64"] = out["fp64"]["seconds"] / out["fp32-ru"]["seconds"]
+    x64, x32 = xs["fp64"], xs["fp32-ru"]
+    out["solution_rel_l2_fp32ru_vs_fp64"] = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
+    # traction on the boundary faces
+    bf = torch.from_numpy(_boundary_facets(op)).cuda()
+    t64 = op.traction(x64, gdat)[bf]
+    t32 = op.traction(x32, gdat)[bf]
+    out["boundary_traction_rel_l2_fp32ru_vs_fp64"] = float(torch.linalg.norm(t32 - t64) / torch.linalg.norm(t64))
+    # errors against the manufactured solution at the volume points (L2 with W |det J|)
+    EQ = op.host_array("E_Q").reshape(op.Nq, -1).T               # (Nbf, Nq)
+    wj = op.host_array("absDetJ").reshape(op.E, op.Nq) * op.host_array("volWeights")[None, :]
+    ue, _ = S.cosine_solution(X)                                  # (E, Nq, 3)
+    for mode, x in xs.items():
+        U = x.view(op.E, 3, -1).cpu().numpy()                      # (E, 3, Nbf)
+        uh = np.einsum("epk,kq->eqp", U, EQ)
+        out[mode]["l2_error_vs_manufactured"] = float(np.sqrt(np.sum(wj[..., None] * (uh - ue) ** 2) /
+                                                              np.sum(wj[..., None] * ue ** 2)))
+    return out
+
+
+def run_greens(args):
+    """Discrete Green's function of a 2D plane-strain fault problem (P6 triangles, ~200k elements,
+    512 fault facets x 7 slip basis functions = 3,584 right-hand sides): blocks of 64 slip
+    right-hand sides built on the GPU in one batched launch and solved together by lockstep CG
+    (one batched apply of 64 columns per iteration), FP32 MFMA + FP64 residual replacement,
+    block-Jacobi preconditioned, to 1e-10.  Times args.blocks blocks (a bounded sample of the 56)
+    and one single-column solve for comparison."""
+    import torch
+    import tandem_amd as T
+    from tandem_amd import scenario as S
+
+    t0 = time.perf_counter()
+    fr = S.FaultRectangle()
+    mesh = fr.mesh()
+    op = T.DGOperator(mesh, T.ELASTICITY, 6)
+    X = op.volume_points()
+    lf, mf = S.lame_fields(2, SEED + 3, length=50.0)
+    op.set_material(lf(X), mf(X))
+    op.upload()
+    ncol = op.block_jacobi_build()
+    torch.cuda.synchronize()
+    log("greens: setup %.1fs, %d elements, %d DoF, %d colours" % (time.perf_counter() - t0, op.E, op.n, ncol))
+    total_cols = int(np.sum(op.facets()[:, 4] == 3)) * (op.degree + 1)
+    nb = 64
+    ld = op.n
+    out = {"workload": "2D P6 plane strain, %d elements, %d DoF, %d slip right-hand sides in blocks of %d"
+                       % (op.E, op.n, total_cols, nb), "blocks_timed": args.blocks}
+    times, its = [], []
+    for blk in range(args.blocks):
+        cols = np.arange(blk * nb, min((blk + 1) * nb, total_cols))
+        fdat = torch.from_numpy(fr.slip_basis(op, cols).reshape(len(cols), -1)).cuda()
+        B = torch.empty((len(cols), ld), dtype=torch.float64, device="cuda")
+        Xs = torch.zeros_like(B)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        op.rhs_device(B, None, fdat)
+        r = op.solve_multi(B, Xs, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=100000)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        log("greens block", blk, "%.2fs" % dt, {k: r[k] for k in ("iterations", "outer_iterations", "rel_residual",
+                                                                   "converged")})
+        times.append(dt)
+        its.append(int(r["iterations"]))
+        del fdat, B, Xs
+    out["seconds_per_block"] = times
+    out["lockstep_iterations"] = its
+    out["rhs_per_second"] = nb * len(times) / sum(times)
+    out["projected_seconds_all_rhs"] = total_cols / out["rhs_per_second"]
+    # one right-hand side alone (same solver)
+    fdat = torch.from_numpy(fr.slip_basis(op, [0]).reshape(-1)).cuda()
+    b = torch.empty(op.n, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    op.rhs_device(b, None, fdat)
+    r = op.solve(b, x, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=100000)
+    torch.cuda.synchronize()
+    out["single_rhs_seconds"] = time.perf_counter() - t1
+    out["single_rhs_iterations"] = r["iterations"]
+    out["batching_speedup_per_rhs"] = out["single_rhs_seconds"] * out["rhs_per_second"]
+    return out
+
+
+class SurfaceSampler:
+    """B: the displacement at a seeded subset of the facet quadrature points of the top face z = 1
+    (the sample points are facet quadrature points, so B is the facet trace of the element basis)."""
+
+    def __init__(self, op, count, seed):
+        fct = op.facets()
+        Xf = op.facet_points()
+        top = np.nonzero((fct[:, 0] == fct[:, 1]) & np.all(np.abs(Xf[:, :, 2] - 1.0) < 1e-12, axis=1))[0]
+        rng = np.random.default_rng(seed)
+        pick = rng.choice(top.size * op.nq, size=count, replace=False)
+        self.f, self.q = top[pick // op.nq], pick % op.nq
+        self.el, self.loc = fct[self.f, 0], fct[self.f, 2]
+        tabs = [op.host_array("E_q%d" % l).reshape(op.nq, -1) for l in range(op.dim + 1)]   # (nq, Nbf)
+        self.phi = np.stack([tabs[l][q] for l, q in zip(self.loc, self.q)])                # (count, Nbf)
+        self.op = op
+
+    def apply(self, x):
+        U = x.view(self.op.E, 3, -1).cpu().numpy()[self.el]                                  # (count, 3, Nbf)
+        return np.einsum("snk,sk->sn", U, self.phi)
+
+    def transpose(self, r):
+        import torch
+        B = np.zeros((self.op.E, 3, self.phi.shape[1]))
+        np.add.at(B, self.el, r[:, :, None] * self.phi[:, None, :])
+        return torch.from_numpy(B.reshape(-1)).cuda()
+
+
+def run_adjoint(args):
+    """Adjoint gradient of the misfit J(lam, mu) = 1/2 |B u - d|^2 on the P4 tet mesh (c1 size by
+    default): B samples the displacement at 2,048 seeded points of the free surface z = 1 (Dirichlet
+    zero on the other faces, seeded smooth body force), d comes from Lame fields perturbed by a
+    seeded 10% smooth field.  Gradient per volume quadrature point from the adjoint kernel, checked
+    (i) against central differences of psi^T A(m) u built from the FP64 apply along 16 seeded
+    directions and (ii) against central differences of J itself (with FP64 solves) along 2 of them;
+    then the FP64 and FP32 gradient kernels are timed."""
+    import torch
+    import tandem_amd as T
+    from tandem_amd import scenario as S
+
+    t0 = time.perf_counter()
+    mesh = T.Mesh.box(3, args.cells, scheme=1, bc=[5, 5, 5, 5, 5, 1])
+    nodes = S.warp(mesh.affine_nodes(4))
+    op = T.DGOperator(mesh, T.ELASTICITY, 4, nodes)
+    X = op.volume_points()
+    lf, mf = S.lame_fields(3, SEED)
+    lam, mu = lf(X), mf(X)
+    pert = S.LogNormalField(3, 1.0, 1.0, 0.3, SEED + 7)
+    ptrue = 0.1 * np.tanh(np.log(pert(X)))                      # smooth seeded field, |.| < 0.1
+    force = np.zeros((op.E, 3, op.Nq))
+    force[:, 2, :] = -1.0 + 0.5 * np.sin(2 * np.pi * X[..., 0]) * np.cos(2 * np.pi * X[..., 1])
+    force[:, 0, :] = 0.3 * np.cos(3 * X[..., 2])
+    fdev = torch.from_numpy(force.ravel()).cuda()
+    b = torch.empty(op.n, dtype=torch.float64, device="cuda")
+    sampler = SurfaceSampler(op, 2048, SEED + 11)
+
+    def forward(lq, mq, rtol):
+        op.set_material(lq, mq)
+        op.rhs_device(b, fdev, None)
+        x = torch.zeros_like(b)
+        r = op.solve(b, x, rtol=rtol, mode="fp64", precond="block-jacobi", max_iter=200000)
+        if not r["converged"]:
+            raise RuntimeError("forward solve did not converge: %s" % r)
+        return x
+
+    d = sampler.apply(forward(lam * (1 + ptrue), mu * (1 + ptrue), 1e-13))
+    log("adjoint: setup + data %.1fs, %d elements" % (time.perf_counter() - t0, op.E))
+    u = forward(lam, mu, 1e-13)
+    res = sampler.apply(u) - d
+    J = 0.5 * float(np.sum(res ** 2))
+    psi = torch.zeros_like(u)
+    rhs_adj = sampler.transpose(res)
+    ra = op.solve(rhs_adj, psi, rtol=1e-13, mode="fp64", precond="block-jacobi", max_iter=200000)
+    gl, gm = op.material_gradient(u, psi)
+    g_lam, g_mu = -gl.cpu().numpy(), -gm.cpu().numpy()
+    out = {"workload": "P4 tet elasticity, %d^3 Kuhn cubes x 6 = %d elements, 2048 surface samples" % (args.cells, op.E),
+           "J": J, "adjoint_iterations": ra["iterations"]}
+    rng = np.random.default_rng(SEED + 13)
+    # (i) directional derivatives of psi^T A(m) u from the FP64 apply
+    ut, pt = u.clone(), psi.clone()
+    y = torch.empty_like(u)
+    errs = []
+    for k in range(16):
+        dl = rng.uniform(-1, 1, lam.shape) * lam
+        dm = rng.uniform(-1, 1, mu.shape) * mu
+        an = float(np.sum(g_lam * dl) + np.sum(g_mu * dm))
+        vals = []
+        for eps in (1e-6, -1e-6):
+            op.set_material(lam + eps * dl, mu + eps * dm)
+            op.apply(ut, y)
+            vals.append(float(torch.dot(pt, y)))
+        fd = -(vals[0] - vals[1]) / 2e-6
+        errs.append(abs(fd - an) / abs(an))
+    out["apply_fd_directions"] = 16
+    out["apply_fd_max_rel_err"] = max(errs)
+    # (ii) finite differences of J with FP64 solves
+    jerr = []
+    for k in range(2):
+        dl = rng.uniform(-1, 1, lam.shape) * lam
+        dm = rng.uniform(-1, 1, mu.shape) * mu
+        an = float(np.sum(g_lam * dl) + np.sum(g_mu * dm))
+        Js = []
+        for eps in (1e-4, -1e-4, 5e-5, -5e-5):
+            xs = forward(lam + eps * dl, mu + eps * dm, 1e-13)
+            Js.append(0.5 * float(np.sum((sampler.apply(xs) - d) ** 2)))
+        c1 = (Js[0] - Js[1]) / 2e-4
+        c2 = (Js[2] - Js[3]) / 1e-4
+        fd = (4.0 * c2 - c1) / 3.0
+        jerr.append(abs(fd - an) / abs(an))
+        log("adjoint J-FD direction", k, fd, an)
+    out["J_fd_directions"] = 2
+    out["J_fd_max_rel_err"] = max(jerr)
+    # timing of the gradient kernel (FP64 and FP32)
+    op.set_material(lam, mu)
+    for dt, name in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+        uu, pp = u.to(dt), psi.to(dt)
+        op.material_gradient(uu, pp)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(10):
+            op.material_gradient(uu, pp)
+        ev[1].record()
+        torch.cuda.synchronize()
+        out["gradient_kernel_ms_" + name] = ev[0].elapsed_time(ev[1]) / 10
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +394,11 @@ def main():
     ap.add_argument("--cells", type=int, default=24)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["apply", "precision", "greens", "adjoint"], default="apply",
+                    help="apply: the headline line (default); precision: FP32-MFMA vs FP64 solve at --cells "
+                         "(40 for the 384,000-element study); greens: batched Green's-function solves; "
+                         "adjoint: misfit gradient with finite-difference checks")
+    ap.add_argument("--blocks", type=int, default=2, help="greens: number of 64-column blocks timed")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy kernel (y = -x over BYTES of float64) --steps times; "
                          "its FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the apply kernel")
@@ -164,6 +420,15 @@ def main():
             torch.neg(x, out=y)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": args.steps}))
+        return
+    if args.workload == "precision":
+        print(json.dumps(run_precision(args)), flush=True)
+        return
+    if args.workload == "greens":
+        print(json.dumps(run_greens(args)), flush=True)
+        return
+    if args.workload == "adjoint":
+        print(json.dumps(run_adjoint(args)), flush=True)
         return
     t0 = time.perf_counter()
     op, mesh, nodes, fields = build_workload(args.cells)

@@ -172,11 +172,24 @@ int tdg_block_jacobi_build(tdg_op* op, int* colours);
 int tdg_block_jacobi_apply(tdg_op* op, double const* r, double* z);
 int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block);
 
+/* Adjoint gradient kernel (elasticity): g_lam, g_mu (device, elements x Nq) = d/d(lam_Q, mu_Q) of
+ * psi^T A(lam, mu) u for device vectors u and psi, through everything the coefficients enter: the
+ * L2 projection to nodal coefficients and the volume weights (Elasticity.cpp:104-175,
+ * prepare_volume_post_skeleton :227-253), the facet traces (:189-225) and the penalty
+ * (:255-292, differentiated at its extremal nodes).  With A psi = dJ/du it gives
+ * dJ/dm = -(g_lam, g_mu) for a misfit J(u(m)) whose right-hand side does not depend on m.
+ * Strains are recomputed from u and psi.  No counterpart in the reference (it has no adjoint);
+ * the FP64 form is checked against central differences of the reference's apply. */
+int tdg_material_gradient(tdg_op* op, double const* u, double const* psi, double* g_lam, double* g_mu);
+int tdg_material_gradient_f32(tdg_op* op, float const* u, float const* psi, float* g_lam, float* g_mu);
+
 /* Algorithmic bytes (for the given bytes per real) and flops of one apply. */
 int tdg_apply_counts(tdg_op const* op, int precision_bytes, double* bytes, double* flops);
 
 /* Named FP64 precompute arrays (host copy), for inspection: absDetJ, JInv, vcoords, volume,
- * JInv0, JInv1, normal, normalLength, fcoords, area, penalty, mat0, mat1, c0Q, c1Q, fc0, fc1.
+ * JInv0, JInv1, normal, normalLength, fcoords, area, penalty, mat0, mat1, c0Q, c1Q, fc0, fc1,
+ * E_Q (basis at the volume points, k + Nbf q), E_q0 ... E_q3 (basis at the facet points of local
+ * face f, k + Nbf q), volWeights, facetWeights.
  * Returns the element count in *n; data copied if out != NULL. */
 int tdg_host_array(tdg_op const* op, char const* name, double* out, int64_t* n);
 

@@ -178,6 +178,11 @@ struct Elasticity : LocalOp {
     void rhs(double const* force_Q, double const* dirichlet_q, double* b) const override;
     // traction on a facet (compute_traction / compute_traction_bnd), t(p,q) D x nq
     void traction(std::size_t fctNo, double const* x, double const* f_q, double* t) const;
+    // d/d(lam_Q, mu_Q) of psi^T A(lam, mu) u: the sensitivity of the bilinear form to the
+    // per-quadrature-point Lame fields given to setup (through the L2 projection, the volume
+    // weights, the facet traces and the penalty).  g_lam, g_mu: E x Nq.
+    void material_gradient(double const* u, double const* psi, double* g_lam, double* g_mu) const;
+    unsigned degree = 0;
 };
 
 // app/localoperator/Poisson.cpp (modal Dubiner solution space)

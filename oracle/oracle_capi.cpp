@@ -239,6 +239,14 @@ int oracle_op_eval_volume(void* h, double const* x, double* out, double* wj) {
 }
 
 // facet table: F x 5 (up0, up1, localNo0, localNo1, bc)
+int oracle_op_material_gradient(void* h, double const* u, double const* psi, double* g_lam, double* g_mu) {
+    return guard([&] {
+        auto o = static_cast<OpHandle*>(h);
+        if (o->kind != 0) throw std::invalid_argument("material gradient is defined for elasticity");
+        o->el->material_gradient(u, psi, g_lam, g_mu);
+    });
+}
+
 void oracle_op_facets(void* h, int64_t* out) {
     auto& c = static_cast<OpHandle*>(h)->c();
     for (std::size_t f = 0; f < c.nF(); ++f) {

@@ -68,6 +68,7 @@ void Elasticity::setup(Mesh const& mesh, unsigned N, std::vector<double> const* 
     c.init(mesh, N, nodes);
     unsigned D = c.D;
     Q = D;
+    degree = N;
     space = RefElement(D, N, false);
     matSpace = RefElement(D, N, false);
     Nbf = space.nbf;
@@ -541,6 +542,166 @@ void Elasticity::traction(std::size_t f, double const* x, double const* f_q_in, 
             double jump = uv[p * nq + q] - (skel ? uv[(D + p) * nq + q] : 0.0) - fv;
             t[p * nq + q] = tr[p * nq + q] + c00 * jump;
         }
+}
+
+// Sensitivity of a(u, psi) = psi^T A u to lam_Q, mu_Q.  The bilinear form of the apply is, per
+// element, sum_q W|J| [lam' div u div psi + mu' (grad u + grad u^T) : grad psi] (apply_volume) and,
+// per facet with side-0 area-scaled normal n, the symmetric interior penalty terms
+//   skeleton: sum_q w [ -{sigma(psi)} n . [[u]] - {sigma(u)} n . [[psi]] + delta |n| [[psi]] . [[u]] ]
+//   boundary: sum_q w [ -sigma(psi_0) n . u_0 - sigma(u_0) n . psi_0 + delta |n| psi_0 . u_0 ]
+// (flux_u_*, flux_sigma_*, apply_facet of elasticity.py:201-234 summed over both elements).  lam',
+// the traces lam_q and the penalty all derive from the nodal projection lam_nodal = P lam_Q
+// (Elasticity.cpp:104-164, 189-225, 255-292); the penalty through min_i 2 mu_i and
+// max_i (D lam_i + 2 mu_i) of each side, differentiated at the extremal node.
+void Elasticity::material_gradient(double const* u, double const* psi, double* g_lam, double* g_mu) const {
+    unsigned D = c.D, DD = D * D, bs = Nbf * Q;
+    std::size_t ne = c.nE(), nf = c.nF();
+    std::vector<double> gl(ne * Nbf, 0.0), gm(ne * Nbf, 0.0); // gradients w.r.t. nodal coefficients
+    // gradient (J[r][j] = d v_r / d x_j) and value of a vector at points from a physical-gradient table
+    auto grad_at = [&](double const* v, std::vector<double> const& Dx, unsigned np, unsigned p, double* J) {
+        for (unsigned r = 0; r < D; ++r)
+            for (unsigned j = 0; j < D; ++j) {
+                double t = 0.0;
+                for (unsigned k = 0; k < Nbf; ++k) t += Dx[k + Nbf * (j + D * p)] * v[k + r * Nbf];
+                J[r * D + j] = t;
+            }
+        (void)np;
+    };
+    // volume
+    for (std::size_t el = 0; el < ne; ++el) {
+        auto Dx = physical_grad(D, Nbf, Nq, &c.JInv[el * Nq * DD], Dxi_Q);
+        for (unsigned q = 0; q < Nq; ++q) {
+            double Ju[9], Jp[9];
+            grad_at(u + el * bs, Dx, Nq, q, Ju);
+            grad_at(psi + el * bs, Dx, Nq, q, Jp);
+            double du = 0.0, dp = 0.0, ss = 0.0;
+            for (unsigned r = 0; r < D; ++r) {
+                du += Ju[r * D + r];
+                dp += Jp[r * D + r];
+            }
+            for (unsigned r = 0; r < D; ++r)
+                for (unsigned j = 0; j < D; ++j) ss += (Ju[r * D + j] + Ju[j * D + r]) * Jp[r * D + j];
+            double wj = c.volRule.wts[q] * c.absDetJ[el * Nq + q];
+            for (unsigned i = 0; i < Nbf; ++i) {
+                gl[el * Nbf + i] += matE_Q_T[q + i * Nq] * wj * du * dp;
+                gm[el * Nbf + i] += matE_Q_T[q + i * Nq] * wj * ss;
+            }
+        }
+    }
+    // facets
+    double cN1 = trace_constant(D, degree - 1);
+    auto extremes = [&](std::size_t el, int* imin, int* imax, double* c0, double* c1) {
+        *c0 = std::numeric_limits<double>::max();
+        *c1 = std::numeric_limits<double>::lowest();
+        for (unsigned i = 0; i < Nbf; ++i) {
+            double a = 2.0 * mu[el * Nbf + i], b = D * lam[el * Nbf + i] + 2.0 * mu[el * Nbf + i];
+            if (a < *c0) { *c0 = a; *imin = int(i); }
+            if (b > *c1) { *c1 = b; *imax = int(i); }
+        }
+    };
+    for (std::size_t f = 0; f < nf; ++f) {
+        auto const& info = c.topo.fct[f];
+        bool skel = info.up[0] != info.up[1];
+        if (!skel && info.bc != BC::Dirichlet && info.bc != BC::Fault) continue; // natural: no terms
+        int ns = skel ? 2 : 1;
+        double h = skel ? 0.5 : 1.0;
+        double const* lq[2] = {&lam_q0[f * nq], &lam_q1[f * nq]};
+        double const* mq[2] = {&mu_q0[f * nq], &mu_q1[f * nq]};
+        std::vector<double> Dx[2];
+        std::vector<double> uv[2], pv[2];
+        for (int s2 = 0; s2 < ns; ++s2) {
+            Dx[s2] = physical_grad(D, Nbf, nq, s2 == 0 ? &c.JInv0[f * nq * DD] : &c.JInv1[f * nq * DD],
+                                   Dxi_q[info.localNo[s2]]);
+            uv[s2].assign(D * nq, 0.0);
+            pv[s2].assign(D * nq, 0.0);
+            auto const& Eq = E_q[info.localNo[s2]];
+            for (unsigned q = 0; q < nq; ++q)
+                for (unsigned r = 0; r < D; ++r)
+                    for (unsigned k = 0; k < Nbf; ++k) {
+                        uv[s2][r * nq + q] += Eq[k + Nbf * q] * u[info.up[s2] * bs + k + r * Nbf];
+                        pv[s2][r * nq + q] += Eq[k + Nbf * q] * psi[info.up[s2] * bs + k + r * Nbf];
+                    }
+        }
+        double gdelta = 0.0;
+        for (unsigned q = 0; q < nq; ++q) {
+            double w = c.fctRule.wts[q];
+            double const* n = &c.normal[(f * nq + q) * D];
+            double ju[3], jp[3];
+            for (unsigned r = 0; r < D; ++r) {
+                ju[r] = uv[0][r * nq + q] - (skel ? uv[1][r * nq + q] : 0.0);
+                jp[r] = pv[0][r * nq + q] - (skel ? pv[1][r * nq + q] : 0.0);
+            }
+            double jun = 0.0, jpn = 0.0, jj = 0.0;
+            for (unsigned r = 0; r < D; ++r) {
+                jun += ju[r] * n[r];
+                jpn += jp[r] * n[r];
+                jj += ju[r] * jp[r];
+            }
+            gdelta += w * c.normalLength[f * nq + q] * jj;
+            for (int s2 = 0; s2 < ns; ++s2) {
+                double Ju[9], Jp[9];
+                grad_at(u + info.up[s2] * bs, Dx[s2], nq, q, Ju);
+                grad_at(psi + info.up[s2] * bs, Dx[s2], nq, q, Jp);
+                double du = 0.0, dp = 0.0, su = 0.0, sp = 0.0;
+                for (unsigned r = 0; r < D; ++r) {
+                    du += Ju[r * D + r];
+                    dp += Jp[r * D + r];
+                }
+                for (unsigned r = 0; r < D; ++r)
+                    for (unsigned j = 0; j < D; ++j) {
+                        su += jp[r] * (Ju[r * D + j] + Ju[j * D + r]) * n[j];
+                        sp += ju[r] * (Jp[r * D + j] + Jp[j * D + r]) * n[j];
+                    }
+                double dl = w * h * (-dp * jun - du * jpn), dm = w * h * (-sp - su);
+                auto const& mT = matE_q_T[info.localNo[s2]];
+                std::size_t el = info.up[s2];
+                for (unsigned i = 0; i < Nbf; ++i) {
+                    gl[el * Nbf + i] += mT[q + i * nq] * dl;
+                    gm[el * Nbf + i] += mT[q + i * nq] * dm;
+                }
+                (void)lq;
+                (void)mq;
+            }
+        }
+        // penalty: delta = (p_0 + p_1) / 4 (skeleton) or p_0 (boundary), p_s = C c1^2 / c0
+        for (int s2 = 0; s2 < ns; ++s2) {
+            std::size_t el = info.up[s2];
+            int imin = 0, imax = 0;
+            double c0 = 0.0, c1 = 0.0;
+            extremes(el, &imin, &imax, &c0, &c1);
+            double C = (D + 1) * cN1 * (c.area[f] / c.volume[el]);
+            double gp = gdelta * (skel ? 0.25 : 1.0);
+            double dp_dc0 = -C * c1 * c1 / (c0 * c0), dp_dc1 = 2.0 * C * c1 / c0;
+            gm[el * Nbf + imin] += gp * dp_dc0 * 2.0;
+            gl[el * Nbf + imax] += gp * dp_dc1 * D;
+            gm[el * Nbf + imax] += gp * dp_dc1 * 2.0;
+        }
+    }
+    // through the projection lam_nodal = M^-1 matE^T diag(W|J|) lam_Q: g_Q = diag(W|J|) matE M^-1 g
+    for (std::size_t el = 0; el < ne; ++el) {
+        std::vector<double> M(Nbf * Nbf, 0.0), rhs(2 * Nbf);
+        for (unsigned k = 0; k < Nbf; ++k)
+            for (unsigned l = 0; l < Nbf; ++l) {
+                double t = 0.0;
+                for (unsigned q = 0; q < Nq; ++q)
+                    t += matE_Q_T[q + k * Nq] * c.volRule.wts[q] * c.absDetJ[el * Nq + q] * matE_Q_T[q + l * Nq];
+                M[k + l * Nbf] = t;
+            }
+        for (unsigned k = 0; k < Nbf; ++k) {
+            rhs[k] = gl[el * Nbf + k];
+            rhs[Nbf + k] = gm[el * Nbf + k];
+        }
+        lu_solve(M, Nbf, rhs.data(), 2); // M symmetric: M^-T g = M^-1 g
+        for (unsigned q = 0; q < Nq; ++q) {
+            double wj = c.volRule.wts[q] * c.absDetJ[el * Nq + q], a = 0.0, b = 0.0;
+            for (unsigned k = 0; k < Nbf; ++k) {
+                a += matE_Q_T[q + k * Nq] * rhs[k];
+                b += matE_Q_T[q + k * Nq] * rhs[Nbf + k];
+            }
+            g_lam[el * Nq + q] = wj * a;
+            g_mu[el * Nq + q] = wj * b;
+        }
+    }
 }
 
 // ================================================================================= Poisson

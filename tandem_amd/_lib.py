@@ -51,7 +51,8 @@ class MeshDesc(C.Structure):
 EXPORTS = [
     "tdg_create", "tdg_destroy", "tdg_last_error", "tdg_sizes", "tdg_volume_points", "tdg_facet_points",
     "tdg_facets", "tdg_set_material", "tdg_upload", "tdg_mult", "tdg_mult_f32", "tdg_mult_batched", "tdg_mult_batched_f32",
-    "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_solve_multi", "tdg_rhs", "tdg_rhs_device", "tdg_traction", "tdg_block_jacobi_build", "tdg_block_jacobi_apply", "tdg_block_jacobi_inverse", "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
+    "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_solve_multi", "tdg_rhs", "tdg_rhs_device", "tdg_traction", "tdg_block_jacobi_build", "tdg_block_jacobi_apply", "tdg_block_jacobi_inverse", "tdg_material_gradient",
+    "tdg_material_gradient_f32", "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
 ]
 
 
@@ -85,6 +86,8 @@ def lib():
     L.tdg_block_jacobi_build.argtypes = [vp, C.POINTER(C.c_int)]
     L.tdg_block_jacobi_apply.argtypes = [vp, vp, vp]
     L.tdg_block_jacobi_inverse.argtypes = [vp, C.c_int64, _dp]
+    L.tdg_material_gradient.argtypes = [vp, vp, vp, vp, vp]
+    L.tdg_material_gradient_f32.argtypes = [vp, vp, vp, vp, vp]
     L.tdg_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]
     L.tdg_solve_multi.argtypes = [vp, vp, vp, C.c_int, C.c_int64, C.POINTER(SolveOpts), C.POINTER(SolveResult),
                                   _ip32, _dp]

@@ -311,6 +311,20 @@ int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block) {
     });
 }
 
+int tdg_material_gradient(tdg_op* op, double const* u, double const* psi, double* g_lam, double* g_mu) {
+    return guarded([&] {
+        require(u && psi && g_lam && g_mu, "null argument");
+        ready(op).material_gradient(u, psi, g_lam, g_mu);
+    });
+}
+
+int tdg_material_gradient_f32(tdg_op* op, float const* u, float const* psi, float* g_lam, float* g_mu) {
+    return guarded([&] {
+        require(u && psi && g_lam && g_mu, "null argument");
+        ready(op).material_gradient(u, psi, g_lam, g_mu);
+    });
+}
+
 int tdg_apply_counts(tdg_op const* op, int pb, double* bytes, double* flops) {
     return guarded([&] {
         require(op && op->impl && bytes && flops, "null argument");

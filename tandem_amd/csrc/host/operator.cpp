@@ -379,6 +379,8 @@ void Operator::set_material(double const* c0_Q, double const* c1_Q) {
     m32_.reset();
     rhs64_.reset();
     bj_.reset();
+    g64_.reset();
+    g32_.reset();
     ready_ = true;
 }
 
@@ -651,6 +653,97 @@ void Operator::rhs_device(double const* force_Q, double const* facet_q, double* 
     TDG_HIP(launch_elasticity_rhs(D_, N_, a, force_Q, facet_q, b, ncols, ld_force, ld_facet, ld_b, stream_));
 }
 
+// Data of the material-gradient kernel: W|J|, the inverse material mass matrix of every element
+// (the projection's M, Elasticity.cpp:104-164), and the penalty's extremal nodes and factors
+// (Elasticity.cpp:255-292).
+template <class T> void Operator::upload_grad(DeviceGrad<T>& g) {
+    const unsigned D = D_, NF = D + 1, Nm = Nbf_;
+    std::vector<double> tabE(std::size_t(Nbf_) * Nq_), jw(E_ * Nq_), Minv(E_ * Nm * Nm), pc(E_ * 2), pcoef(E_ * NF);
+    std::vector<int32_t> pinfo(E_ * 4, 0);
+    for (unsigned k = 0; k < Nbf_; ++k)
+        for (unsigned q = 0; q < Nq_; ++q) tabE[k * Nq_ + q] = E_Q_[k + Nbf_ * q];
+    const double cN1 = trace_constant(D, N_ - 1);
+#pragma omp parallel for schedule(static)
+    for (std::size_t e = 0; e < E_; ++e) {
+        std::vector<double> M(Nm * Nm, 0.0), I(Nm * Nm, 0.0);
+        for (unsigned q = 0; q < Nq_; ++q) {
+            double wj = vol_.weights[q] * absDetJ_[e * Nq_ + q];
+            jw[e * Nq_ + q] = wj;
+            for (unsigned k = 0; k < Nm; ++k)
+                for (unsigned l = 0; l < Nm; ++l) M[k + l * Nm] += matE_Q_[k + Nm * q] * wj * matE_Q_[l + Nm * q];
+        }
+        for (unsigned i = 0; i < Nm; ++i) I[i + i * Nm] = 1.0;
+        dense_solve(M, Nm, I.data(), Nm);
+        std::copy(I.begin(), I.end(), Minv.begin() + e * Nm * Nm);
+        double c0 = std::numeric_limits<double>::max(), c1 = std::numeric_limits<double>::lowest();
+        int i0 = 0, i1 = 0;
+        for (unsigned i = 0; i < Nm; ++i) {
+            double a = 2.0 * mat1_[e * Nm + i], b = D * mat0_[e * Nm + i] + 2.0 * mat1_[e * Nm + i];
+            if (a < c0) { c0 = a; i0 = int(i); }
+            if (b > c1) { c1 = b; i1 = int(i); }
+        }
+        pc[e * 2 + 0] = c0;
+        pc[e * 2 + 1] = c1;
+        pinfo[e * 4 + 0] = i0;
+        pinfo[e * 4 + 1] = i1;
+        for (unsigned f = 0; f < NF; ++f) {
+            std::size_t fct = std::size_t(ef_[(e * NF + f) * 4]);
+            pcoef[e * NF + f] = (D + 1) * cN1 * area_[fct] / volume_[e];
+        }
+    }
+    g.tabE.upload(cast<T>(tabE));
+    g.jw.upload(cast<T>(jw));
+    g.Minv.upload(cast<T>(Minv));
+    g.pc.upload(cast<T>(pc));
+    g.pcoef.upload(cast<T>(pcoef));
+    g.pinfo.upload(pinfo);
+}
+
+template <class T> void Operator::grad_dispatch(T const* u, T const* psi, T* gl, T* gm) {
+    if (!ready_) throw std::logic_error("set_material() must be called first");
+    if (kind_ != Kind_Elasticity) throw std::invalid_argument("the material gradient is built for elasticity");
+    upload_device();
+    auto& slot = [&]() -> auto& {
+        if constexpr (std::is_same<T, double>::value) return g64_;
+        else return g32_;
+    }();
+    if (!slot) {
+        auto g = std::make_unique<DeviceGrad<T>>();
+        upload_grad(*g);
+        slot = std::move(g);
+    }
+    DeviceOperator<T> const& d = device<T>();
+    ApplyArgs<T> a;
+    a.tabV = d.tabV.ptr;
+    a.tabVt = d.tabVt.ptr;
+    a.tabF = d.tabF.ptr;
+    a.tabFt = d.tabFt.ptr;
+    a.w = d.w.ptr;
+    a.perm = d.perm.ptr;
+    a.iperm = d.iperm.ptr;
+    a.G = d.G.ptr;
+    a.c0 = d.c0.ptr;
+    a.c1 = d.c1.ptr;
+    a.fG = d.fG.ptr;
+    a.fc0 = d.fc0.ptr;
+    a.fc1 = d.fc1.ptr;
+    a.fn = d.fn.ptr;
+    a.fpen = d.fpen.ptr;
+    a.ef = reinterpret_cast<int4 const*>(d.ef.ptr);
+    a.num_elements = int64_t(E_);
+    GradArgs<T> g;
+    g.tabE = slot->tabE.ptr;
+    g.jw = slot->jw.ptr;
+    g.Minv = slot->Minv.ptr;
+    g.pc = slot->pc.ptr;
+    g.pinfo = reinterpret_cast<int4 const*>(slot->pinfo.ptr);
+    g.pcoef = slot->pcoef.ptr;
+    int kq = 0, kf = 0;
+    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
+        throw std::runtime_error("no kernel compiled for this dimension/degree");
+    TDG_HIP(launch_material_gradient<T>(D_, N_, a, g, u, psi, gl, gm, stream_));
+}
+
 BlockJacobi& Operator::block_jacobi() {
     if (!bj_) bj_ = std::make_unique<BlockJacobi>();
     if (!bj_->built()) bj_->build(*this);
@@ -681,6 +774,11 @@ std::vector<double> const* Operator::host_array(std::string const& n) const {
     if (n == "c1Q") return &c1Q_;
     if (n == "fc0") return &fc0_;
     if (n == "fc1") return &fc1_;
+    if (n == "E_Q") return &E_Q_;
+    if (n.size() == 4 && n.compare(0, 3, "E_q") == 0 && n[3] >= '0' && n[3] < char('0' + E_q_.size()))
+        return &E_q_[std::size_t(n[3] - '0')];
+    if (n == "volWeights") return &vol_.weights;
+    if (n == "facetWeights") return &fct_.weights;
     return nullptr;
 }
 
@@ -751,7 +849,7 @@ void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, 
     int kq = 0, kf = 0;
     if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
         throw std::runtime_error("no kernel compiled for this dimension/degree");
-    MfmaLdsArgs<T> b;
+    MfmaArgs<T> b;
     b.V = d.V.ptr;
     b.T0 = d.T0.ptr;
     b.perm = d.perm2.ptr;
@@ -763,7 +861,7 @@ void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, 
     b.num_elements = int64_t(E_);
     b.num_groups = int64_t((E_ + 15) / 16);
     b.xcd_remap = variant_ == Variant_MfmaLinear ? 0 : 1;
-    TDG_HIP(launch_elasticity_lds<T>(D_, N_, b, x, y, ncols, ld, stream_));
+    TDG_HIP(launch_elasticity_mfma<T>(D_, N_, b, x, y, ncols, ld, stream_));
 }
 
 template <class T> void Operator::dispatch(T const* x, T* y, int ncols, int64_t ld) {
@@ -781,5 +879,12 @@ void Operator::apply(double const* x, double* y) { dispatch(x, y, 1, 0); }
 void Operator::apply(float const* x, float* y) { dispatch(x, y, 1, 0); }
 void Operator::apply_batched(double const* X, double* Y, int ncols, int64_t ld) { dispatch(X, Y, ncols, ld); }
 void Operator::apply_batched(float const* X, float* Y, int ncols, int64_t ld) { dispatch(X, Y, ncols, ld); }
+
+void Operator::material_gradient(double const* u, double const* psi, double* g_lam, double* g_mu) {
+    grad_dispatch(u, psi, g_lam, g_mu);
+}
+void Operator::material_gradient(float const* u, float const* psi, float* g_lam, float* g_mu) {
+    grad_dispatch(u, psi, g_lam, g_mu);
+}
 
 } // namespace tdg

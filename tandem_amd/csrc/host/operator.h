@@ -50,6 +50,12 @@ template <class T> struct DeviceMfma {
     DeviceArray<int32_t> perm2, FI; // perm_f followed by iperm_f; face info
 };
 
+// Extra data of the material-gradient kernel (one precision).
+template <class T> struct DeviceGrad {
+    DeviceArray<T> tabE, jw, Minv, pc, pcoef;
+    DeviceArray<int32_t> pinfo; // int4 per element
+};
+
 // Extra FP64 data of the right-hand side and traction kernels (elasticity).
 struct DeviceRhs {
     DeviceArray<double> tabE, jw;
@@ -118,6 +124,11 @@ public:
     // Element-block Jacobi preconditioner (built on first use; rebuilt after set_material).
     BlockJacobi& block_jacobi();
 
+    // g_lam, g_mu (E x Nq, device) = d/d(lam_Q, mu_Q) of psi^T A u for device vectors u, psi
+    // (elasticity): the adjoint gradient kernel.
+    void material_gradient(double const* u, double const* psi, double* g_lam, double* g_mu);
+    void material_gradient(float const* u, float const* psi, float* g_lam, float* g_mu);
+
     // named host arrays for inspection
     std::vector<double> const* host_array(std::string const& name) const;
 
@@ -170,6 +181,10 @@ private:
     std::unique_ptr<DeviceMfma<float>> m32_;
     std::unique_ptr<DeviceRhs> rhs64_;
     std::unique_ptr<BlockJacobi> bj_;
+    std::unique_ptr<DeviceGrad<double>> g64_;
+    std::unique_ptr<DeviceGrad<float>> g32_;
+    template <class T> void upload_grad(DeviceGrad<T>& g);
+    template <class T> void grad_dispatch(T const* u, T const* psi, T* gl, T* gm);
     RhsArgs rhs_args();
     int variant_ = Variant_Auto;
 };

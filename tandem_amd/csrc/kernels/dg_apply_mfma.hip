@@ -54,7 +54,7 @@ constexpr int kWaves = 4;
 // FP32: two waves per SIMD (the LDS footprint allows two blocks per CU); FP64: one.
 template <int D, int N, class T>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
-void elasticity_lds_kernel(MfmaLdsArgs<T> a, T const* __restrict__ xs,
+void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
                                                                      T* __restrict__ ys, int64_t ld) {
     using L = MfmaLayout<D, N>;
     using acc = typename Mma<T>::acc;
@@ -358,17 +358,17 @@ void elasticity_lds_kernel(MfmaLdsArgs<T> a, T const* __restrict__ xs,
 }
 
 template <int D, int N, class T>
-hipError_t run(MfmaLdsArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t s) {
+hipError_t run(MfmaArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t s) {
     const int64_t blocks = (a.num_groups + kWaves - 1) / kWaves;
     dim3 grid(unsigned((blocks + 7) / 8 * 8), unsigned(ncols));
-    hipLaunchKernelGGL((elasticity_lds_kernel<D, N, T>), grid, dim3(64 * kWaves), 0, s, a, x, y, ld);
+    hipLaunchKernelGGL((elasticity_mfma_kernel<D, N, T>), grid, dim3(64 * kWaves), 0, s, a, x, y, ld);
     return hipGetLastError();
 }
 
 } // namespace
 
 template <class T>
-hipError_t launch_elasticity_lds(int dim, int degree, MfmaLdsArgs<T> const& a, T const* x, T* y, int ncols,
+hipError_t launch_elasticity_mfma(int dim, int degree, MfmaArgs<T> const& a, T const* x, T* y, int ncols,
                                  int64_t ld, hipStream_t s) {
     if (dim == 3) {
         switch (degree) {
@@ -390,9 +390,9 @@ bool mfma_supported(int dim, int degree) {
     return (dim == 3 && degree >= 2 && degree <= 4) || (dim == 2 && (degree == 2 || degree == 4 || degree == 6));
 }
 
-template hipError_t launch_elasticity_lds<double>(int, int, MfmaLdsArgs<double> const&, double const*, double*, int,
+template hipError_t launch_elasticity_mfma<double>(int, int, MfmaArgs<double> const&, double const*, double*, int,
                                                   int64_t, hipStream_t);
-template hipError_t launch_elasticity_lds<float>(int, int, MfmaLdsArgs<float> const&, float const*, float*, int,
+template hipError_t launch_elasticity_mfma<float>(int, int, MfmaArgs<float> const&, float const*, float*, int,
                                                  int64_t, hipStream_t);
 
 } // namespace tdg

@@ -37,6 +37,21 @@ template <class T>
 hipError_t launch_poisson(int dim, int degree, ApplyArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld,
                           hipStream_t stream);
 
+// Extra device data of the material-gradient (adjoint) kernel, one precision.
+template <class T> struct GradArgs {
+    T const* tabE;      // [k][q] basis values at the volume points (= the nodal material basis)
+    T const* jw;        // [E][Nq] W |det J|
+    T const* Minv;      // [E][Nbf][Nbf] inverse material mass matrix (column-major)
+    T const* pc;        // [E][2] c0 = min 2 mu_i, c1 = max (D lam_i + 2 mu_i) of the nodal coefficients
+    int4 const* pinfo;  // [E] {argmin of 2 mu_i, argmax of D lam_i + 2 mu_i, -, -}
+    T const* pcoef;     // [E][D+1] (D+1) c_{N-1} |F| / |K| of each face
+};
+
+// g_lam, g_mu (E x Nq) = d/d(lam_Q, mu_Q) of psi^T A u (elasticity).
+template <class T>
+hipError_t launch_material_gradient(int dim, int degree, ApplyArgs<T> const& a, GradArgs<T> const& g, T const* u,
+                                    T const* psi, T* g_lam, T* g_mu, hipStream_t stream);
+
 // Volume / facet rule sizes the kernels are compiled for (2N+1 rules of tandem's autorule).
 int kernel_rule_sizes(int dim, int degree, int* Nq, int* nq);
 

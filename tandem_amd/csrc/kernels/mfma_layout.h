@@ -50,7 +50,7 @@ bool mfma_supported(int dim, int degree);
 
 // Arguments of the MFMA kernel: canonical tables (copied to LDS by every block) and the
 // element-grouped point data.
-template <class T> struct MfmaLdsArgs {
+template <class T> struct MfmaArgs {
     T const* V;          // [e][q][k]  d phi_k / d xi_e at volume point q
     T const* T0;         // [c][q][k]  face-0 value (c = 0) and d/dxi_(c-1) at facet point q
     int32_t const* perm; // [2][f][k]  facet relabelling perm_f, then its inverse
@@ -65,7 +65,7 @@ template <class T> struct MfmaLdsArgs {
 };
 
 template <class T>
-hipError_t launch_elasticity_lds(int dim, int degree, MfmaLdsArgs<T> const& a, T const* x, T* y, int ncols,
+hipError_t launch_elasticity_mfma(int dim, int degree, MfmaArgs<T> const& a, T const* x, T* y, int ncols,
                                  int64_t ld, hipStream_t stream);
 
 } // namespace tdg

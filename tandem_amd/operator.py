@@ -255,6 +255,19 @@ class DGOperator:
         check(lib().tdg_block_jacobi_inverse(self._h, element, out))
         return out.reshape(self.bs, self.bs, order="F")
 
+    def material_gradient(self, u, psi, g_lam=None, g_mu=None):
+        """d/d(lam_Q, mu_Q) of psi^T A u for GPU vectors u, psi (float64 or float32); returns two
+        (E, Nq) tensors."""
+        torch = _torch()
+        assert u.dtype == psi.dtype and u.is_cuda and psi.is_cuda
+        if g_lam is None:
+            g_lam = torch.empty(self.E * self.Nq, dtype=u.dtype, device=u.device)
+            g_mu = torch.empty_like(g_lam)
+        f = lib().tdg_material_gradient if u.dtype == torch.float64 else lib().tdg_material_gradient_f32
+        check(f(self._h, C.c_void_p(u.data_ptr()), C.c_void_p(psi.data_ptr()), C.c_void_p(g_lam.data_ptr()),
+                C.c_void_p(g_mu.data_ptr())))
+        return g_lam.view(self.E, self.Nq), g_mu.view(self.E, self.Nq)
+
     def apply_counts(self, precision_bytes=8):
         b, f = C.c_double(), C.c_double()
         check(lib().tdg_apply_counts(self._h, precision_bytes, C.byref(b), C.byref(f)))

@@ -213,34 +213,40 @@ class FaultRectangle:
 
     def slip_basis(self, op, columns=None):
         """Facet data (ncols, F, 2, nq) of the slip basis functions: facet-nodal Lagrange polynomials
-        of degree N on each fault facet (nodes at the facet's warp-and-blend = Gauss-Lobatto points),
-        times the along-fault unit tangent, oriented so that the slip is the hanging wall's
-        displacement minus the footwall's ([[u]] = u(side 0) - u(side 1), hence the sign by side).
+        of degree N on each fault facet (nodes at the Gauss-Lobatto points of the facet), times the
+        along-fault unit tangent, oriented so that the slip is the hanging wall's displacement
+        minus the footwall's ([[u]] = u(side 0) - u(side 1), hence the sign by side).
         Column c = (fault facet c // (N+1), node c % (N+1)); `columns` selects a subset."""
         N = op.degree
-        fct = op.facets()                        # (F, 5): elem0, elem1, local0, local1, bc
-        Xf = op.facet_points()                   # (F, nq, 2)
-        faults = np.nonzero(fct[:, 4] == 3)[0]
-        # Gauss-Lobatto points on [0, 1]: the end points and the roots of P_N'
-        lg = np.polynomial.legendre.Legendre.basis(N).deriv().roots()
-        nodes1d = np.sort(np.concatenate([[0.0, 1.0], 0.5 * (np.real(lg) + 1.0)]))
+        if not hasattr(self, "_slip"):
+            fct = op.facets()                    # (F, 5): elem0, elem1, local0, local1, bc
+            Xf = op.facet_points()               # (F, nq, 2)
+            faults = np.nonzero(fct[:, 4] == 3)[0]
+            # Gauss-Lobatto points on [0, 1]: the end points and the roots of P_N'
+            lg = np.polynomial.legendre.Legendre.basis(N).deriv().roots()
+            nodes1d = np.sort(np.concatenate([[0.0, 1.0], 0.5 * (np.real(lg) + 1.0)]))
+            # each fault facet's end points (its two mesh vertices), matched by midpoint
+            fv = self.vertices[self.boundary_facets[self.boundary_bc == 3]]      # (nf, 2, 2)
+            mids = fv.mean(axis=1)
+            fmid = Xf[faults].mean(axis=1)
+            match = np.argmin(np.linalg.norm(fmid[:, None, :] - mids[None, :, :], axis=2), axis=1)
+            seg = fv[match]                                                        # (nf, 2, 2)
+            dv = seg[:, 1] - seg[:, 0]
+            t = np.einsum("fqd,fd->fq", Xf[faults] - seg[:, None, 0], dv) / np.einsum("fd,fd->f", dv, dv)[:, None]
+            L = np.ones((len(faults), N + 1, t.shape[1]))                          # l_i(t_q)
+            for i in range(N + 1):
+                for m in range(N + 1):
+                    if m != i:
+                        L[:, i] *= (t - nodes1d[m]) / (nodes1d[i] - nodes1d[m])
+            cent = self.vertices[self.elements].mean(axis=1)
+            nrm = np.array([-self.tangent[1], self.tangent[0]])                    # towards the hanging wall
+            up = np.einsum("fd,d->f", cent[fct[faults, 0]] - fmid, nrm) > 0
+            self._slip = (faults, L, np.where(up, 1.0, -1.0))
+        faults, L, sgn = self._slip
         ncols_all = len(faults) * (N + 1)
         cols = np.arange(ncols_all) if columns is None else np.asarray(columns)
         out = np.zeros((len(cols), op.F, 2, op.nq))
-        cent = self.vertices[self.elements].mean(axis=1)
-        nrm = np.array([-self.tangent[1], self.tangent[0]])  # points to the hanging wall (up)
-        for j, c in enumerate(cols):
-            f = faults[c // (N + 1)]
-            i = c % (N + 1)
-            # facet parameter of the quadrature points along the segment between its end vertices
-            v = self.vertices[self.boundary_facets[self.boundary_bc == 3]]
-            seg = v[np.argmin(np.linalg.norm(v.mean(axis=1) - Xf[f].mean(axis=0), axis=1))]
-            t = (Xf[f] - seg[0]) @ (seg[1] - seg[0]) / np.dot(seg[1] - seg[0], seg[1] - seg[0])
-            li = np.ones_like(t)
-            for m in range(N + 1):
-                if m != i:
-                    li *= (t - nodes1d[m]) / (nodes1d[i] - nodes1d[m])
-            side0_up = np.dot(cent[fct[f, 0]] - Xf[f].mean(axis=0), nrm) > 0
-            sgn = 1.0 if side0_up else -1.0
-            out[j, f] = sgn * li[None, :] * self.tangent[:, None]
+        fi, ni = cols // (N + 1), cols % (N + 1)
+        out[np.arange(len(cols)), faults[fi]] = (sgn[fi][:, None, None] * L[fi, ni][:, None, :] *
+                                                 self.tangent[None, :, None])
         return out

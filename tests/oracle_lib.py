@@ -63,6 +63,7 @@ def lib():
     L.oracle_op_assemble.argtypes = [C.c_void_p, dp]
     L.oracle_op_rhs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, dp]
     L.oracle_op_traction.argtypes = [C.c_void_p, C.c_int64, dp, C.c_void_p, dp]
+    L.oracle_op_material_gradient.argtypes = [C.c_void_p, dp, dp, dp, dp]
     L.oracle_op_facets.argtypes = [C.c_void_p, ip64]
     L.oracle_op_eval_volume.argtypes = [C.c_void_p, dp, dp, C.c_void_p]
     L.oracle_op_array.argtypes = [C.c_void_p, C.c_char_p, dp]
@@ -262,6 +263,15 @@ class Operator:
         if lib().oracle_op_traction(self.h, fct, np.ascontiguousarray(x, dtype=np.float64), _ptr(fq), t) != 0:
             _err()
         return t.reshape(self.D, self.nq)
+
+    def material_gradient(self, u, psi):
+        """d/d(lam_Q, mu_Q) of psi^T A u, each (E, Nq)."""
+        gl = np.zeros(self.E * self.Nq)
+        gm = np.zeros(self.E * self.Nq)
+        if lib().oracle_op_material_gradient(self.h, np.ascontiguousarray(u, dtype=np.float64),
+                                             np.ascontiguousarray(psi, dtype=np.float64), gl, gm) != 0:
+            _err()
+        return gl.reshape(self.E, self.Nq), gm.reshape(self.E, self.Nq)
 
     def eval_volume(self, x):
         """Discrete solution at the volume points (E, Q, Nq) and the weights W|J| (E, Nq)."""
