@@ -10,9 +10,11 @@ y = A x of a seeded uniform[-1,1] vector in FP64, inputs resident in HBM.
 Prints one JSON line: GDoF/s of the FP64 apply (`value`), the roofline of the apply kernel measured
 with HIP events on its stream, the CPU baseline (the CPU FP64 oracle timed on the host cores on a
 smaller mesh of the same element type), the FP32 apply, and the time to a 1e-10 relative residual
-of FP64 CG versus FP32 CG inside FP64 iterative refinement on the same mesh and body force.
+of FP64 CG versus FP32 CG with FP64 residual replacement on the same mesh and body force, without
+and with the element-block Jacobi preconditioner.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-solve] [--no-cpu-baseline]
+    python bench.py --workload precision|greens|adjoint   (the other studies; one JSON line each)
 """
 import argparse
 import json
@@ -109,7 +111,9 @@ def traffic_from_profile():
 
 
 def solve_comparison(op, mesh, nodes, fields, rtol=1e-10):
-    """FP64 CG vs FP32 CG + FP64 refinement on the manufactured cosine solution."""
+    """Time to a 1e-10 FP64 true relative residual: FP64 CG vs FP32 CG with FP64 residual
+    replacement, each without and with the element-block Jacobi preconditioner, on the
+    manufactured cosine solution (right-hand side built on the GPU)."""
     import torch
     from tandem_amd import scenario as S
 
@@ -118,20 +122,29 @@ def solve_comparison(op, mesh, nodes, fields, rtol=1e-10):
     f = S.elasticity_body_force(X, lam, mu)               # (E, Nq, 3)
     Xf = op.facet_points()
     g, _ = S.cosine_solution(Xf)                            # (F, nq, 3)
-    b = op.rhs(np.ascontiguousarray(f.transpose(0, 2, 1)), np.ascontiguousarray(g.transpose(0, 2, 1)))
-    bd = torch.from_numpy(b).cuda()
+    bd = torch.empty(op.n, dtype=torch.float64, device="cuda")
+    op.rhs_device(bd, torch.from_numpy(np.ascontiguousarray(f.transpose(0, 2, 1)).ravel()).cuda(),
+                  torch.from_numpy(np.ascontiguousarray(g.transpose(0, 2, 1)).ravel()).cuda())
     out = {}
-    xs = {}
-    for mode in ("fp64", "fp32-ru"):
-        x = torch.zeros_like(bd)
-        r = op.solve(bd, x, rtol=rtol, mode=mode, max_iter=40000)
-        log("solve", mode, r)
-        out[mode] = {"seconds": r["seconds"], "iterations": r["iterations"], "fp64_replacements": r["outer_iterations"],
-                     "rel_residual": r["rel_residual"], "converged": bool(r["converged"])}
-        xs[mode] = x.cpu().numpy()
-    out["speedup_fp32ru_vs_fp64"] = out["fp64"]["seconds"] / out["fp32-ru"]["seconds"]
-    out["solution_rel_diff_fp32ru_vs_fp64"] = float(np.linalg.norm(xs["fp32-ru"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
-    # discretisation error of the FP64 solution against the manufactured solution at the volume points
+    for pc in (None, "block-jacobi"):
+        tag = "none" if pc is None else pc
+        if pc is not None:
+            t0 = time.perf_counter()
+            op.block_jacobi_build()
+            torch.cuda.synchronize()
+            out["block_jacobi_setup_s"] = time.perf_counter() - t0
+        xs = {}
+        for mode in ("fp64", "fp32-ru"):
+            x = torch.zeros_like(bd)
+            r = op.solve(bd, x, rtol=rtol, mode=mode, max_iter=40000, precond=pc)
+            log("solve", mode, tag, r)
+            out["%s/%s" % (mode, tag)] = {"seconds": r["seconds"], "iterations": r["iterations"],
+                                          "fp64_replacements": r["outer_iterations"],
+                                          "rel_residual": r["rel_residual"], "converged": bool(r["converged"])}
+            xs[mode] = x
+        out["speedup_fp32ru_vs_fp64/" + tag] = out["fp64/" + tag]["seconds"] / out["fp32-ru/" + tag]["seconds"]
+        out["solution_rel_diff_fp32ru_vs_fp64/" + tag] = float(torch.linalg.norm(xs["fp32-ru"] - xs["fp64"]) /
+                                                               torch.linalg.norm(xs["fp64"]))
     return out
 
 

@@ -90,12 +90,12 @@ int tdg_mult_batched_f32(tdg_op* op, float const* X, float* Y, int ncols, int64_
 int tdg_set_stream(tdg_op* op, void* stream);
 
 /* Kernel choice: TDG_VARIANT_AUTO (default) runs the matrix-core (MFMA) kernel where one is
- * compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6), with workgroups renumbered so
- * that each XCD takes one contiguous range of elements, and the thread-per-point kernel otherwise;
- * TDG_VARIANT_POINTS always runs the thread-per-point kernel; TDG_VARIANT_MFMA_LINEAR runs the MFMA
- * kernel with workgroups in launch order.  All compute the same operator (the reference's
- * element-centric apply); the two kernels differ in summation order only. */
-enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1, TDG_VARIANT_MFMA_LINEAR = 2 };
+ * compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6) and the thread-per-point kernel
+ * otherwise; TDG_VARIANT_POINTS always runs the thread-per-point kernel; TDG_VARIANT_MFMA_XCD runs
+ * the MFMA kernel with its workgroups renumbered so that each XCD takes one contiguous range of
+ * elements.  All compute the same operator (the reference's element-centric apply); the two
+ * kernels differ in summation order only. */
+enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1, TDG_VARIANT_MFMA_XCD = 2 };
 int tdg_set_variant(tdg_op* op, int variant);
 
 /* Right-hand side b (host, local size) from a body force at the volume points (elements x Q x Nq,

@@ -193,7 +193,7 @@ int tdg_set_stream(tdg_op* op, void* stream) {
 int tdg_set_variant(tdg_op* op, int variant) {
     return guarded([&] {
         require(op && op->impl, "null operator");
-        require(variant == TDG_VARIANT_AUTO || variant == TDG_VARIANT_POINTS || variant == TDG_VARIANT_MFMA_LINEAR,
+        require(variant == TDG_VARIANT_AUTO || variant == TDG_VARIANT_POINTS || variant == TDG_VARIANT_MFMA_XCD,
                 "unknown kernel variant");
         op->impl->set_variant(variant);
     });

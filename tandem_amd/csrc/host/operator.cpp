@@ -408,7 +408,7 @@ void Operator::upload_device() {
 }
 
 void Operator::set_variant(int v) {
-    if (v != Variant_Auto && v != Variant_Points && v != Variant_MfmaLinear)
+    if (v != Variant_Auto && v != Variant_Points && v != Variant_MfmaXcd)
         throw std::invalid_argument("unknown kernel variant");
     variant_ = v;
 }
@@ -420,7 +420,7 @@ bool Operator::uses_mfma() const {
 // Tables and element-grouped point data of the MFMA kernel (kernels/dg_apply_lds.hip): canonical
 // reference tables V[e][q][k] and T0[c][q][k] (face 0, c = 0 value, 1 + e gradient), the facet
 // relabellings, and per group of 16 elements (element innermost) the volume point data VD, the
-// one-side facet records FS, penalties FP and face info FI.
+// two-sided facet records FD, penalties FP and face info FI.
 template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
     const unsigned D = D_, DD = D * D, NF = D + 1;
     const MfmaDims L{int(D), int(Nbf_), int(Nq_), int(nq_)};
@@ -439,7 +439,7 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
     // element-grouped point data
     const std::size_t ng = (E_ + 15) / 16;
     std::vector<T> VD(ng * Nq_ * L.NCH * 16, T(0)), FP(ng * NF * 16, T(0));
-    std::vector<T> FS(ng * NF * nq_ * L.NFS * 16, T(0));
+    std::vector<T> FD(ng * NF * nq_ * L.NFC * 16, T(0));
     std::vector<int32_t> FI(ng * NF * 16 * 4, 0);
 #pragma omp parallel for schedule(static)
     for (std::size_t e = 0; e < E_; ++e) {
@@ -459,17 +459,23 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
             fi[1] = ef[1];
             FP[(grp * NF + f) * 16 + j] = T(penalty_[fct]);
             for (unsigned q = 0; q < nq_; ++q) {
+                T* o = &FD[(((grp * NF + f) * nq_ + q) * L.NFC) * 16 + j];
                 double const* g0 = &fGp_[((fct * 2 + side) * nq_ + q) * DD];
+                double const* g1 = &fGp_[((fct * 2 + (1 - side)) * nq_ + q) * DD];
+                for (unsigned c = 0; c < DD; ++c) {
+                    o[c * 16] = T(g0[c]);
+                    o[(DD + 2 + c) * 16] = T(g1[c]);
+                }
+                o[DD * 16] = T(fc0_[(fct * 2 + side) * nq_ + q]);
+                o[(DD + 1) * 16] = T(fc1_[(fct * 2 + side) * nq_ + q]);
+                o[(2 * DD + 2) * 16] = T(fc0_[(fct * 2 + (1 - side)) * nq_ + q]);
+                o[(2 * DD + 3) * 16] = T(fc1_[(fct * 2 + (1 - side)) * nq_ + q]);
                 double sg = side ? -1.0 : 1.0;
-                T* os = &FS[(((grp * NF + f) * nq_ + q) * L.NFS) * 16 + j];
-                for (unsigned c = 0; c < DD; ++c) os[c * 16] = T(g0[c]);
-                os[DD * 16] = T(fc0_[(fct * 2 + side) * nq_ + q]);
-                os[(DD + 1) * 16] = T(fc1_[(fct * 2 + side) * nq_ + q]);
-                for (unsigned d = 0; d < D; ++d) os[(DD + 2 + d) * 16] = T(sg * normal_[(fct * nq_ + q) * D + d]);
+                for (unsigned d = 0; d < D; ++d) o[(2 * DD + 4 + d) * 16] = T(sg * normal_[(fct * nq_ + q) * D + d]);
             }
         }
     }
-    dm.FS.upload(FS);
+    dm.FD.upload(FD);
     dm.V.upload(cast<T>(V));
     dm.T0.upload(cast<T>(TT));
     dm.perm2.upload(P2);
@@ -855,12 +861,12 @@ void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, 
     b.perm = d.perm2.ptr;
     b.w = d.w.ptr;
     b.VD = d.VD.ptr;
-    b.FS = d.FS.ptr;
+    b.FD = d.FD.ptr;
     b.FP = d.FP.ptr;
     b.FI = reinterpret_cast<int4 const*>(d.FI.ptr);
     b.num_elements = int64_t(E_);
     b.num_groups = int64_t((E_ + 15) / 16);
-    b.xcd_remap = variant_ == Variant_MfmaLinear ? 0 : 1;
+    b.xcd_remap = variant_ == Variant_MfmaXcd ? 1 : 0;
     TDG_HIP(launch_elasticity_mfma<T>(D_, N_, b, x, y, ncols, ld, stream_));
 }
 

@@ -46,7 +46,7 @@ template <class T> struct DeviceOperator {
 // Operand tables and element-grouped point data of the MFMA elasticity kernel (one precision).
 template <class T> struct DeviceMfma {
     DeviceArray<T> V, T0, w;       // canonical reference tables, facet weights
-    DeviceArray<T> VD, FS, FP;     // volume point data, one-side facet records, penalties
+    DeviceArray<T> VD, FD, FP;     // volume point data, two-sided facet records, penalties
     DeviceArray<int32_t> perm2, FI; // perm_f followed by iperm_f; face info
 };
 
@@ -62,10 +62,10 @@ struct DeviceRhs {
     DeviceArray<int32_t> fbc, fct; // fct: int4 per facet
 };
 
-// Kernel selection: Variant_Auto uses the MFMA kernel where one is compiled (elasticity; workgroups
-// renumbered per XCD), the thread-per-point kernel otherwise; Variant_Points forces the
-// thread-per-point kernel; Variant_MfmaLinear the MFMA kernel with workgroups in launch order.
-enum Variant : int { Variant_Auto = 0, Variant_Points = 1, Variant_MfmaLinear = 2 };
+// Kernel selection: Variant_Auto uses the MFMA kernel where one is compiled (elasticity), the
+// thread-per-point kernel otherwise; Variant_Points forces the thread-per-point kernel;
+// Variant_MfmaXcd the MFMA kernel with workgroups renumbered per XCD.
+enum Variant : int { Variant_Auto = 0, Variant_Points = 1, Variant_MfmaXcd = 2 };
 
 class BlockJacobi;
 

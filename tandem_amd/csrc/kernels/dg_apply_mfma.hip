@@ -60,7 +60,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
     using acc = typename Mma<T>::acc;
     constexpr int Nbf = L::Nbf, Nq = L::Nq, nq = L::nq, Q = D, BS = Nbf * Q, DD = D * D;
     constexpr int KS = L::KS, QB = L::QB, MT = L::MT, FT = L::FT, NF = D + 1;
-    constexpr int NCH = L::NCH, NFS = L::NFS;
+    constexpr int NCH = L::NCH, NFC = L::NFC;
     constexpr int nV = D * Nq * Nbf, nT = NF * nq * Nbf;
 
     __shared__ T sV[nV];
@@ -217,22 +217,15 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
             int il[MT];
 #pragma unroll
             for (int m = 0; m < MT; ++m) il[m] = sP[NF * Nbf + f * Nbf + min(16 * m + j, Nbf - 1)];
-            // facet records: this side's, and the other side's from the neighbour's own record of
-            // the shared facet (this element's own record on boundary faces, where it is unused)
-            T const* const fso = a.FS + ((gj * NF + f) * nq) * NFS * 16 + j;
-            const int64_t oe = kind == 1 ? nb : gj * 16 + j;
-            const int ofc = kind == 1 ? nbface : f;
-            T const* const fsn = a.FS + (((oe >> 4) * NF + ofc) * nq) * NFS * 16 + (oe & 15);
+            T const* const fdf = a.FD + ((gj * NF + f) * nq) * NFC * 16 + j;
 #pragma unroll
             for (int t = 0; t < FT; ++t) {
                 const int q = 4 * t + g;
                 const int qc = min(q, nq - 1);
                 const bool act = kind != 0 && q < nq;
-                T fo[NFS], fn[DD + 2];
+                T fd[NFC];
 #pragma unroll
-                for (int c = 0; c < NFS; ++c) fo[c] = fso[(qc * NFS + c) * 16];
-#pragma unroll
-                for (int c = 0; c < DD + 2; ++c) fn[c] = fsn[(qc * NFS + c) * 16];
+                for (int c = 0; c < NFC; ++c) fd[c] = fdf[(qc * NFC + c) * 16];
                 const T wq = a.w[qc];
                 acc Co[Q], Cn[Q];
 #pragma unroll
@@ -251,10 +244,10 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
                     }
                 }
                 // this lane: facet point q; register c = value (0) / d/dxi_e (1 + e)
-                T const* G0 = fo;
-                T const* G1 = fn;
-                const T l0 = fo[DD], m0 = fo[DD + 1], l1 = fn[DD], m1 = fn[DD + 1];
-                T const* n = fo + DD + 2;
+                T const* G0 = fd;
+                T const* G1 = fd + DD + 2;
+                const T l0 = fd[DD], m0 = fd[DD + 1], l1 = fd[2 * DD + 2], m1 = fd[2 * DD + 3];
+                T const* n = fd + 2 * DD + 4;
                 T nn = T(0);
 #pragma unroll
                 for (int d = 0; d < D; ++d) nn += n[d] * n[d];

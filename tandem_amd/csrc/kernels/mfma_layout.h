@@ -12,12 +12,12 @@ namespace tdg {
 //   KS = ceil(Nbf / 4)   K-steps over the basis,         MT = ceil(Nbf / 16) basis tiles,
 //   QB = ceil(Nq / 16)   16-point volume blocks,          FT = ceil(nq / 4)   4-point facet tiles,
 //   NCH = D*D + 2        volume channels (JInv, lamWJ, muWJ),
-//   NFS = D*D + 2 + D    facet channels of one side (G', lam, mu, oriented normal).
+//   NFC = 2 D*D + 4 + D  facet channels (own G', lam, mu, neighbour G', lam, mu, oriented normal).
 struct MfmaDims {
-    int D, Nbf, Nq, nq, KS, QB, MT, FT, NCH, NFS;
+    int D, Nbf, Nq, nq, KS, QB, MT, FT, NCH, NFC;
     __host__ __device__ constexpr MfmaDims(int D_, int Nbf_, int Nq_, int nq_)
         : D(D_), Nbf(Nbf_), Nq(Nq_), nq(nq_), KS((Nbf_ + 3) / 4), QB((Nq_ + 15) / 16), MT((Nbf_ + 15) / 16),
-          FT((nq_ + 3) / 4), NCH(D_ * D_ + 2), NFS(D_ * D_ + 2 + D_) {}
+          FT((nq_ + 3) / 4), NCH(D_ * D_ + 2), NFC(2 * D_ * D_ + 4 + D_) {}
 };
 
 namespace detail {
@@ -34,7 +34,7 @@ template <int D_, int N_> struct MfmaLayout {
     static constexpr int Nq = detail::vol_points(D_, N_);
     static constexpr int nq = detail::fct_points(D_, N_);
     static constexpr MfmaDims dims{D_, Nbf, Nq, nq};
-    static constexpr int KS = dims.KS, QB = dims.QB, MT = dims.MT, FT = dims.FT, NCH = dims.NCH, NFS = dims.NFS;
+    static constexpr int KS = dims.KS, QB = dims.QB, MT = dims.MT, FT = dims.FT, NCH = dims.NCH, NFC = dims.NFC;
 };
 
 // Output row of a 16x16x4 MFMA tile held in register i (0..3) of lane group g = lane / 16:
@@ -56,8 +56,8 @@ template <class T> struct MfmaArgs {
     int32_t const* perm; // [2][f][k]  facet relabelling perm_f, then its inverse
     T const* w;          // facet weights
     T const* VD;         // [group][q][NCH][16]
-    T const* FS;         // [group][f][q][NFS][16]  this side's G', lam, mu and the oriented normal;
-                         // the other side's G', lam, mu are read from the neighbour's own record
+    T const* FD;         // [group][f][q][NFC][16]  both sides' G', lam, mu and the oriented normal
+                         // (stored per element side: every load is a contiguous 16-element row)
     T const* FP;         // [group][f][16]
     int4 const* FI;      // [group][f][16]
     int64_t num_elements, num_groups;

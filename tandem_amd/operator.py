@@ -130,9 +130,9 @@ class DGOperator:
         self._material = True
 
     def set_variant(self, variant):
-        """"auto" (matrix-core kernel where compiled, workgroups renumbered per XCD), "points"
-        (thread-per-point kernel) or "mfma-linear" (matrix-core kernel, workgroups in launch order)."""
-        check(lib().tdg_set_variant(self._h, {"auto": 0, "points": 1, "mfma-linear": 2}[variant]))
+        """"auto" (matrix-core kernel where compiled), "points" (thread-per-point kernel) or
+        "mfma-xcd" (matrix-core kernel, workgroups renumbered so each XCD takes one element range)."""
+        check(lib().tdg_set_variant(self._h, {"auto": 0, "points": 1, "mfma-xcd": 2}[variant]))
 
     def set_stream(self, stream):
         """stream: a torch.cuda.Stream (or None for the default stream)."""

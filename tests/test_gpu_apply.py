@@ -30,12 +30,12 @@ def _ids(c):
     return "D%d-n%s-s%d-P%d-%s" % (c[0], c[1], c[2], c[3], "elast" if c[4] == 0 else "poisson")
 
 
-@pytest.mark.parametrize("variant", ["auto", "points", "mfma-linear"])
+@pytest.mark.parametrize("variant", ["auto", "points", "mfma-xcd"])
 @pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
 def test_apply_matches_oracle(cuda, case, variant):
     """variant "auto" runs the matrix-core kernel where one is compiled (elasticity 3D P2-P4, 2D
-    P2/P4/P6), "points" the thread-per-point kernel, "mfma-linear" the matrix-core kernel with its
-    workgroups in launch order."""
+    P2/P4/P6), "points" the thread-per-point kernel, "mfma-xcd" the matrix-core kernel with its
+    workgroups renumbered per XCD."""
     import torch
 
     D, n, scheme, N, kind, bc = case
