@@ -140,6 +140,8 @@ typedef struct {
     int check_every;
     double reliable_delta;
     int precond;
+    int graphs; /* 1: replay the iterations between residual checks as a captured HIP graph (needs a
+                 * stream set with tdg_set_stream and an even check_every; otherwise plain launches) */
 } tdg_solve_opts;
 typedef struct {
     int iterations;

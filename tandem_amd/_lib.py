@@ -25,7 +25,7 @@ class TdgError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("max_iter", C.c_int), ("mode", C.c_int), ("inner_rtol", C.c_double),
                 ("max_outer", C.c_int), ("check_every", C.c_int), ("reliable_delta", C.c_double),
-                ("precond", C.c_int)]
+                ("precond", C.c_int), ("graphs", C.c_int)]
 
 
 class SolveResult(C.Structure):

@@ -94,11 +94,25 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
     const bool valid = j < nel;
     const int64_t gj = live ? grp : 0; // point data row of this lane (in bounds for every lane)
     T const* const Uj = U + j * BS;    // element j: U(k, r) at Uj[k + r Nbf]
+    // Output rows k of the lift: the first MM tiles of 16 on the matrix cores; when the last tile
+    // holds at most 4 real rows (3D P3: 4 of 16, P4: 3 of 16) those rows are summed on the vector
+    // ALU instead (each lane over its own K entries, then across the four lane groups), which
+    // removes a quarter of the lift MFMAs that would mostly multiply padding.  (FP64 P4 has no
+    // registers left for the tail accumulators and keeps the matrix-core path.)
+    constexpr int TAIL = Nbf - 16 * (MT - 1);
+    constexpr bool VTAIL = D == 3 && MT > 1 && TAIL <= 4 && (sizeof(T) == 4 || N < 4); // FP64 P4: no registers left
+    constexpr int MM = VTAIL ? MT - 1 : MT;
+    constexpr int NT = VTAIL ? TAIL : 1;
     acc Y[MT][Q];
+    T Yt[NT][Q];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int u = 0; u < Q; ++u) Y[m][u] = acc{0, 0, 0, 0};
+#pragma unroll
+    for (int kt = 0; kt < NT; ++kt)
+#pragma unroll
+        for (int u = 0; u < Q; ++u) Yt[kt][u] = T(0);
 
     // ------------------------------------------------------------------ volume
     // one 16-point block; LAST marks the final block, whose fully padded K steps are skipped
@@ -172,13 +186,24 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
                 }
             const int qr = min(16 * b + mfma_row<T>(g, i), Nq - 1) * Nbf;
 #pragma unroll
-            for (int e = 0; e < D; ++e)
+            for (int e = 0; e < D; ++e) {
 #pragma unroll
-                for (int m = 0; m < MT; ++m) {
+                for (int m = 0; m < MM; ++m) {
                     const T av = sV[e * Nq * Nbf + qr + min(16 * m + j, Nbf - 1)];
 #pragma unroll
                     for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, C[e][u][i], Y[m][u]);
                 }
+                if constexpr (VTAIL) {
+                    // this lane's point is K entry mfma_row(g, i) of the step; its share of the
+                    // tail rows (C = 0 on padded points)
+#pragma unroll
+                    for (int kt = 0; kt < NT; ++kt) {
+                        const T a = sV[e * Nq * Nbf + qr + 16 * MM + kt];
+#pragma unroll
+                        for (int u = 0; u < Q; ++u) Yt[kt][u] += a * C[e][u][i];
+                    }
+                }
+            }
         }
     };
 
@@ -217,6 +242,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
             int il[MT];
 #pragma unroll
             for (int m = 0; m < MT; ++m) il[m] = sP[NF * Nbf + f * Nbf + min(16 * m + j, Nbf - 1)];
+            int const* const iperm_f = sP + NF * Nbf + f * Nbf; // face-0 labels (tail rows read at use)
             T const* const fdf = a.FD + ((gj * NF + f) * nq) * NFC * 16 + j;
 #pragma unroll
             for (int t = 0; t < FT; ++t) {
@@ -317,22 +343,43 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
                         W[1 + e][u] = act ? wq * v : T(0);
                     }
 #pragma unroll
-                for (int c = 0; c <= D; ++c)
+                for (int c = 0; c <= D; ++c) {
 #pragma unroll
-                    for (int m = 0; m < MT; ++m) {
+                    for (int m = 0; m < MM; ++m) {
                         const T av = sT[(c * nq + qc) * Nbf + il[m]];
 #pragma unroll
                         for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, W[c][u], Y[m][u]);
                     }
+                    if constexpr (VTAIL) {
+#pragma unroll
+                        for (int kt = 0; kt < NT; ++kt) {
+                            const T a = sT[(c * nq + qc) * Nbf + iperm_f[min(16 * MM + kt, Nbf - 1)]];
+#pragma unroll
+                            for (int u = 0; u < Q; ++u) Yt[kt][u] += a * W[c][u];
+                        }
+                    }
+                }
             }
         }
     }
 
     // ------------------------------------------------------------------ store through LDS
+    if constexpr (VTAIL) {
+        // sum the tail rows over the four lane groups (lanes j, j + 16, j + 32, j + 48)
+#pragma unroll
+        for (int kt = 0; kt < NT; ++kt)
+#pragma unroll
+            for (int u = 0; u < Q; ++u) {
+                T v = Yt[kt][u];
+                v += __shfl_xor(v, 16, 64);
+                v += __shfl_xor(v, 32, 64);
+                Yt[kt][u] = v;
+            }
+    }
     __syncthreads(); // every read of U is complete
     if (live) {
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
+        for (int m = 0; m < MM; ++m)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int k = 16 * m + mfma_row<T>(g, i);
@@ -341,6 +388,14 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
                     for (int u = 0; u < Q; ++u) U[j * BS + k + u * Nbf] = Y[m][u][i];
                 }
             }
+        if constexpr (VTAIL) {
+            if (g == 0) {
+#pragma unroll
+                for (int kt = 0; kt < NT; ++kt)
+#pragma unroll
+                    for (int u = 0; u < Q; ++u) U[j * BS + 16 * MM + kt + u * Nbf] = Yt[kt][u];
+            }
+        }
     }
     __syncthreads();
     if (live) {

@@ -20,6 +20,8 @@ struct SolveOptions {
     int check_every = 10;     // host reads the residual every check_every iterations
     double reliable_delta = 0.1; // residual drop that triggers an FP64 replacement (mode 2)
     int precond = Precond_None;  // element-block Jacobi (built on first use, FP64 and FP32 inverses)
+    int graphs = 1;              // replay the iterations between host checks as a captured HIP graph
+                                 // (needs a non-default stream and an even check_every)
 };
 
 struct SolveResult {

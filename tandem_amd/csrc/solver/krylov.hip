@@ -123,6 +123,45 @@ template <class T> struct Buf {
     void alloc(std::size_t n) { KHIP(hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T))); }
 };
 
+// Replays a fixed sequence of launches: captured once into a HIP graph on a non-default stream
+// (the solver iterations between two host checks are the same launches with the same arguments),
+// or launched directly when graphs are off or the stream is the default stream, which cannot be
+// captured.  The sequence must not allocate or memset.
+class Replay {
+public:
+    Replay(hipStream_t s, bool enable) : s_(s), on_(enable && s != nullptr) {}
+    ~Replay() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+    }
+    template <class F> void run(F&& launches) {
+        if (!on_) {
+            launches();
+            return;
+        }
+        if (!exec_) {
+            KHIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+            try {
+                launches();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            KHIP(hipStreamEndCapture(s_, &graph_));
+            KHIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        }
+        KHIP(hipGraphLaunch(exec_, s_));
+    }
+
+private:
+    hipStream_t s_;
+    bool on_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
 double read_scalar(double const* d, hipStream_t s) {
     double h = 0.0;
     KHIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -135,7 +174,7 @@ double read_scalar(double const* d, hipStream_t s) {
 // M = null is plain CG (rho = r.r).  w: work vector for M r.
 template <class T, class TX>
 int cg(Operator& op, BlockJacobi const* M, T const* rhs, TX* z, T* r, T* p, T* q, T* w, double* s, double* partial,
-       int64_t n, double target_rr, int max_it, int check_every, hipStream_t st, double* rr_out) {
+       int64_t n, double target_rr, int max_it, int check_every, bool graphs, hipStream_t st, double* rr_out) {
     const dim3 g(kGrid), b(kBlock);
     hipLaunchKernelGGL((fill<TX>), g, b, 0, st, z, TX(0), n);
     hipLaunchKernelGGL((copy_to<T>), g, b, 0, st, r, rhs, n);
@@ -155,21 +194,35 @@ int cg(Operator& op, BlockJacobi const* M, T const* rhs, TX* z, T* r, T* p, T* q
         *rr_out = rr;
         return 0;
     }
-    int it = 0, cur = 0;
-    while (it < max_it) {
+    auto iteration = [&](int c) {
         op.apply(p, q);
         hipLaunchKernelGGL((dot_partial<T>), g, b, 0, st, p, q, n, partial);
         hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + 2);
-        hipLaunchKernelGGL((cg_xr<T, TX>), g, b, 0, st, z, r, p, q, n, s, cur, 2, partial);
-        hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (M ? 4 : 1 - cur));
+        hipLaunchKernelGGL((cg_xr<T, TX>), g, b, 0, st, z, r, p, q, n, s, c, 2, partial);
+        hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (M ? 4 : 1 - c));
         if (M) {
             M->apply(r, w, st);
             hipLaunchKernelGGL((dot_partial<T>), g, b, 0, st, r, w, n, partial);
-            hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (1 - cur));
+            hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (1 - c));
         }
-        hipLaunchKernelGGL((cg_p<T>), g, b, 0, st, p, M ? w : r, n, s, 1 - cur, cur);
-        cur = 1 - cur;
-        ++it;
+        hipLaunchKernelGGL((cg_p<T>), g, b, 0, st, p, M ? w : r, n, s, 1 - c, c);
+    };
+    // check_every iterations between host checks; an even count keeps the rho slots in phase, so
+    // the same captured sequence is replayed for every chunk
+    Replay chunk(st, graphs && check_every % 2 == 0);
+    int it = 0, cur = 0;
+    while (it < max_it) {
+        if (it + check_every <= max_it) {
+            chunk.run([&] {
+                for (int k = 0; k < check_every; ++k) iteration((cur + k) & 1);
+            });
+            it += check_every;
+            cur = (cur + check_every) & 1;
+        } else {
+            iteration(cur);
+            cur = 1 - cur;
+            ++it;
+        }
         if (it % check_every == 0 || it == max_it) {
             KHIP(hipGetLastError());
             rr = read_scalar(s + (M ? 4 : cur), st);
@@ -218,7 +271,7 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
         if (M) w.alloc(n);
         double rr = 0.0;
         res.iterations = cg<double, double>(op, M, b, x, r.p, p.p, q.p, w.p, s.p, partial.p, n, target, o.max_iter,
-                                            o.check_every, st, &rr);
+                                            o.check_every, o.graphs != 0, st, &rr);
         // report the true residual ||b - A x|| / ||b||
         op.apply(x, q.p);
         hipLaunchKernelGGL(residual, g, bl, 0, st, b, q.p, r.p, (float*)nullptr, n, partial.p);
@@ -250,7 +303,8 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
             double inner_target = o.inner_rtol * o.inner_rtol * rr;
             double rr_in = 0.0;
             int its = cg<float, float>(op, M, r32.p, d32.p, w32.p, p32.p, q32.p, z32.p, s.p, partial.p, n,
-                                       inner_target, o.max_iter - inner_total, o.check_every, st, &rr_in);
+                                       inner_target, o.max_iter - inner_total, o.check_every, o.graphs != 0, st,
+                                       &rr_in);
             inner_total += its;
             hipLaunchKernelGGL(add_update, g, bl, 0, st, x, d32.p, n);
             ++outer;
@@ -300,16 +354,38 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
         KHIP(hipGetLastError());
         double rr_true = read_scalar(s.p + (M ? 4 : 0), st);
         double max_r = std::sqrt(rr_true);
-        int it = 0, cur = 0, updates = 0;
-        while (it < o.max_iter) {
-            const int irr = M ? 4 : 1 - cur; // where r.r lands
+        // iteration = part A (apply, step, r.r) + part B (preconditioner, direction); the host
+        // check sits between A and B, so a chunk between checks is A B A B ... A
+        auto partA = [&](int c) {
             op.apply(p32.p, q32.p);
             hipLaunchKernelGGL((dot_partial<float>), g, bl, 0, st, p32.p, q32.p, n, partial.p);
             hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + 2);
-            hipLaunchKernelGGL((cg_xr<float, float>), g, bl, 0, st, xa32.p, r32.p, p32.p, q32.p, n, s.p, cur, 2,
+            hipLaunchKernelGGL((cg_xr<float, float>), g, bl, 0, st, xa32.p, r32.p, p32.p, q32.p, n, s.p, c, 2,
                                partial.p);
-            hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + irr);
-            ++it;
+            hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + (M ? 4 : 1 - c));
+        };
+        auto partB = [&](int c) {
+            if (M) precondition(1 - c);
+            hipLaunchKernelGGL((cg_p<float>), g, bl, 0, st, p32.p, M ? z32.p : r32.p, n, s.p, 1 - c, c);
+        };
+        Replay chunk(st, o.graphs != 0 && o.check_every % 2 == 0);
+        int it = 0, cur = 0, updates = 0;
+        while (it < o.max_iter) {
+            if (it % o.check_every == 0 && it + o.check_every <= o.max_iter) {
+                // iterations it+1 .. it+check_every up to the last one's part A
+                chunk.run([&] {
+                    for (int k = 0; k < o.check_every; ++k) {
+                        partA((cur + k) & 1);
+                        if (k + 1 < o.check_every) partB((cur + k) & 1);
+                    }
+                });
+                cur = (cur + o.check_every - 1) & 1;
+                it += o.check_every;
+            } else {
+                partA(cur);
+                ++it;
+            }
+            const int irr = M ? 4 : 1 - cur; // where r.r of the last part A landed
             bool replace = false;
             if (it % o.check_every == 0 || it == o.max_iter) {
                 KHIP(hipGetLastError());
@@ -329,8 +405,7 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
                 ++updates;
                 if (rr_true <= target) break;
             }
-            if (M) precondition(1 - cur);
-            hipLaunchKernelGGL((cg_p<float>), g, bl, 0, st, p32.p, M ? z32.p : r32.p, n, s.p, 1 - cur, cur);
+            partB(cur);
             cur = 1 - cur;
         }
         res.rel_residual = std::sqrt(rr_true / bb);

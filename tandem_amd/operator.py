@@ -166,14 +166,15 @@ class DGOperator:
         check(lib().tdg_upload(self._h))
 
     def solve(self, b, x, rtol=1e-10, mode="fp64", inner_rtol=1e-4, max_iter=200000, max_outer=60,
-              check_every=10, reliable_delta=0.1, precond=None):
+              check_every=10, reliable_delta=0.1, precond=None, graphs=True):
         """Solve A x = b (float64 GPU tensors).  mode "fp64": FP64 CG; "fp32-ir": FP32 CG solves
         restarted inside FP64 iterative refinement; "fp32-ru": FP32 CG with FP64 residual
         replacement (refinement that keeps the Krylov direction).  precond: None or
-        "block-jacobi".  Returns the statistics."""
+        "block-jacobi".  graphs: replay the iterations between residual checks as a captured HIP
+        graph when a stream is set.  Returns the statistics."""
         assert b.is_cuda and x.is_cuda and b.dtype == x.dtype and b.numel() == self.n == x.numel()
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ir": 1, "fp32-ru": 2}[mode], inner_rtol, max_outer,
-                              check_every, reliable_delta, {None: 0, "block-jacobi": 1}[precond])
+                              check_every, reliable_delta, {None: 0, "block-jacobi": 1}[precond], int(graphs))
         res = _lib.SolveResult()
         check(lib().tdg_solve(self._h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), C.byref(opts),
                               C.byref(res)))
@@ -188,7 +189,7 @@ class DGOperator:
         assert B.dtype == X.dtype and B.is_contiguous() and X.is_contiguous()
         nc, ld = B.shape
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ru": 2}[mode], 1e-4, 1, check_every, reliable_delta,
-                              {None: 0, "block-jacobi": 1}[precond])
+                              {None: 0, "block-jacobi": 1}[precond], 0)
         res = _lib.SolveResult()
         its = np.zeros(nc, dtype=np.int32)
         rel = np.zeros(nc)
