@@ -185,6 +185,28 @@ int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block);
 int tdg_material_gradient(tdg_op* op, double const* u, double const* psi, double* g_lam, double* g_mu);
 int tdg_material_gradient_f32(tdg_op* op, float const* u, float const* psi, float* g_lam, float* g_mu);
 
+/* Elastic-wave pieces (elasticity, FP64, device vectors of local size).
+ * tdg_inverse_mass: y = M^-1 x with the weight-adjusted inverse of the curvilinear element mass
+ *   (unit density): Mhat^-1 E diag(-W/|J|) E^T Mhat^-1 per element and component; replaces the
+ *   apply_inverse_mass kernel call of Elasticity::wave_rhs (app/localoperator/Elasticity.cpp:873-879,
+ *   generated from app/kernels/elasticity.py:236-238).  The sign follows the reference's
+ *   negative_rhoInv_W_Jinv_Q, so y = -M^-1 x.
+ * tdg_wave_rhs: y = M^-1 (A x - b_facet(facet)) with the facet data of tdg_rhs (may be NULL);
+ *   replaces Elasticity::wave_rhs (app/localoperator/Elasticity.cpp:865-880) over all elements. */
+int tdg_inverse_mass(tdg_op* op, double const* x, double* y);
+int tdg_wave_rhs(tdg_op* op, double const* x, double const* facet, double* y);
+
+/* p-multigrid transfer between nodal spaces of one element (NodalInterpolation::assemble,
+ * app/localoperator/NodalInterpolation.cpp:8-38).  tdg_interpolation_matrix writes the host matrix
+ * I (n_to x n_from, row-major: I[k n_from + l] = phi^from_l(x^to_k)) for 1 <= from < to <= 8; the
+ * reference's per-quantity block-diagonal matrix is I repeated over the quantities.
+ * tdg_interpolate applies it on the device to num_elements blocks of num_quantities components:
+ * y = I x (prolongation) or, with transpose != 0, y = I^T x (restriction); I is a device copy,
+ * stream a hipStream_t (NULL: default stream). */
+int tdg_interpolation_matrix(int dim, int to_degree, int from_degree, double* out);
+int tdg_interpolate(double const* I, int n_to, int n_from, int num_quantities, int64_t num_elements, int transpose,
+                    double const* x, double* y, void* stream);
+
 /* Algorithmic bytes (for the given bytes per real) and flops of one apply. */
 int tdg_apply_counts(tdg_op const* op, int precision_bytes, double* bytes, double* flops);
 

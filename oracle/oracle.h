@@ -182,8 +182,18 @@ struct Elasticity : LocalOp {
     // per-quadrature-point Lame fields given to setup (through the L2 projection, the volume
     // weights, the facet traces and the penalty).  g_lam, g_mu: E x Nq.
     void material_gradient(double const* u, double const* psi, double* g_lam, double* g_mu) const;
+    // apply_inverse_mass with density 1 (elasticity.py:236-238; weights -W/|J|, Elasticity.cpp:227-253):
+    // y_e = Mhat^-1 E_Q diag(-W/|J|) E_Q^T Mhat^-1 x_e
+    void inverse_mass(double const* x, double* y) const;
+    // wave_rhs (Elasticity.cpp:865-880): apply_<true> with the facet data f_q (F x Q x nq, may be
+    // null), then apply_inverse_mass
+    void wave_rhs(double const* x, double const* f_q, double* y) const;
     unsigned degree = 0;
 };
+
+// NodalInterpolation::assemble (app/localoperator/NodalInterpolation.cpp:8-38): I (n_to x n_from,
+// column-major) = V_to[:, :n_from] VInv_from, the from-degree nodal basis at the to-degree nodes.
+std::vector<double> nodal_interpolation(unsigned D, unsigned to_degree, unsigned from_degree);
 
 // app/localoperator/Poisson.cpp (modal Dubiner solution space)
 struct Poisson : LocalOp {

@@ -239,6 +239,32 @@ int oracle_op_eval_volume(void* h, double const* x, double* out, double* wj) {
 }
 
 // facet table: F x 5 (up0, up1, localNo0, localNo1, bc)
+int oracle_op_wave_rhs(void* h, double const* x, double const* f_q, double* y) {
+    return guard([&] {
+        auto o = static_cast<OpHandle*>(h);
+        if (o->kind != 0) throw std::invalid_argument("wave_rhs is defined for elasticity");
+        o->el->wave_rhs(x, f_q, y);
+    });
+}
+
+int oracle_op_inverse_mass(void* h, double const* x, double* y) {
+    return guard([&] {
+        auto o = static_cast<OpHandle*>(h);
+        if (o->kind != 0) throw std::invalid_argument("inverse_mass is defined for elasticity");
+        o->el->inverse_mass(x, y);
+    });
+}
+
+// I (n_to x n_from, row-major for the caller): I[k][l] = phi^from_l(x^to_k)
+int oracle_nodal_interpolation(int D, int to, int from, double* out) {
+    return guard([&] {
+        auto E = oracle::nodal_interpolation(unsigned(D), unsigned(to), unsigned(from));
+        oracle::RefElement rt(unsigned(D), unsigned(to), false), rf(unsigned(D), unsigned(from), false);
+        for (unsigned k = 0; k < rt.nbf; ++k)
+            for (unsigned l = 0; l < rf.nbf; ++l) out[k * rf.nbf + l] = E[l + rf.nbf * k];
+    });
+}
+
 int oracle_op_material_gradient(void* h, double const* u, double const* psi, double* g_lam, double* g_mu) {
     return guard([&] {
         auto o = static_cast<OpHandle*>(h);

@@ -704,6 +704,51 @@ void Elasticity::material_gradient(double const* u, double const* psi, double* g
     }
 }
 
+void Elasticity::inverse_mass(double const* x, double* y) const {
+    unsigned bs = Nbf * Q;
+    auto Minv = space.inverse_mass_matrix(); // Nbf x Nbf, column-major
+    std::vector<double> t(Nbf), uq(Nq);
+    for (std::size_t el = 0; el < c.nE(); ++el)
+        for (unsigned p = 0; p < Q; ++p) {
+            double const* xe = x + el * bs + p * Nbf;
+            for (unsigned k = 0; k < Nbf; ++k) {
+                double v = 0.0;
+                for (unsigned l = 0; l < Nbf; ++l) v += Minv[k + l * Nbf] * xe[l];
+                t[k] = v;
+            }
+            for (unsigned q = 0; q < Nq; ++q) {
+                double v = 0.0;
+                for (unsigned k = 0; k < Nbf; ++k) v += E_Q[k + Nbf * q] * t[k];
+                uq[q] = -v * c.volRule.wts[q] / c.absDetJ[el * Nq + q];
+            }
+            for (unsigned k = 0; k < Nbf; ++k) {
+                double v = 0.0;
+                for (unsigned q = 0; q < Nq; ++q) v += E_Q[k + Nbf * q] * uq[q];
+                t[k] = v;
+            }
+            for (unsigned k = 0; k < Nbf; ++k) {
+                double v = 0.0;
+                for (unsigned l = 0; l < Nbf; ++l) v += Minv[k + l * Nbf] * t[l];
+                y[el * bs + p * Nbf + k] = v;
+            }
+        }
+}
+
+void Elasticity::wave_rhs(double const* x, double const* f_q, double* y) const {
+    std::size_t n = block_size() * c.nE();
+    std::vector<double> ax(n), b(n, 0.0);
+    apply(x, ax.data(), 1);
+    if (f_q) rhs(nullptr, f_q, b.data()); // the add_bc terms of apply_<true> are -rhs_facet
+    for (std::size_t i = 0; i < n; ++i) ax[i] -= b[i];
+    inverse_mass(ax.data(), y);
+}
+
+std::vector<double> nodal_interpolation(unsigned D, unsigned to_degree, unsigned from_degree) {
+    RefElement to(D, to_degree, false), from(D, from_degree, false);
+    std::vector<double> pts(to.nodes.begin(), to.nodes.end());
+    return from.eval_basis(pts); // phi^from_l(x^to_k) stored (l, k): transpose to (k, l)
+}
+
 // ================================================================================= Poisson
 // Poisson ctor + prepare_* (Poisson.cpp:27-61, 133-223); solution space is modal Dubiner.
 void Poisson::setup(Mesh const& mesh, unsigned N, std::vector<double> const* nodes, double const* K_Q) {

@@ -326,6 +326,41 @@ int tdg_material_gradient_f32(tdg_op* op, float const* u, float const* psi, floa
     });
 }
 
+int tdg_inverse_mass(tdg_op* op, double const* x, double* y) {
+    return guarded([&] {
+        require(x && y, "null argument");
+        ready(op).inverse_mass_device(x, y);
+    });
+}
+
+int tdg_wave_rhs(tdg_op* op, double const* x, double const* facet, double* y) {
+    return guarded([&] {
+        require(x && y, "null argument");
+        ready(op).wave_rhs_device(x, facet, y);
+    });
+}
+
+int tdg_interpolation_matrix(int dim, int to_degree, int from_degree, double* out) {
+    return guarded([&] {
+        require(out && (dim == 2 || dim == 3), "bad arguments");
+        require(from_degree >= 1 && to_degree > from_degree && to_degree <= 8, "need 1 <= from < to <= 8");
+        auto I = tdg::Operator::interpolation_matrix(unsigned(dim), unsigned(to_degree), unsigned(from_degree));
+        std::memcpy(out, I.data(), I.size() * sizeof(double));
+    });
+}
+
+int tdg_interpolate(double const* I, int n_to, int n_from, int num_quantities, int64_t num_elements, int transpose,
+                    double const* x, double* y, void* stream) {
+    return guarded([&] {
+        require(I && x && y, "null argument");
+        require(n_to > 0 && n_from > 0 && num_quantities > 0 && num_elements >= 0, "bad sizes");
+        if (num_elements == 0) return;
+        hipError_t e = tdg::launch_interpolate(I, n_to, n_from, num_quantities, num_elements, transpose ? 1 : 0, x,
+                                               y, static_cast<hipStream_t>(stream));
+        if (e != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e));
+    });
+}
+
 int tdg_apply_counts(tdg_op const* op, int pb, double* bytes, double* flops) {
     return guarded([&] {
         require(op && op->impl && bytes && flops, "null argument");

@@ -750,6 +750,67 @@ template <class T> void Operator::grad_dispatch(T const* u, T const* psi, T* gl,
     TDG_HIP(launch_material_gradient<T>(D_, N_, a, g, u, psi, gl, gm, stream_));
 }
 
+// Weight-adjusted inverse mass of the curvilinear element with unit density (Elasticity.cpp:227-253,
+// applied as in elasticity.py:236-238): Mhat^-1 E diag(-W/|J|) E^T Mhat^-1, Mhat the nodal mass
+// matrix of the reference simplex (exact with the 2N+1 volume rule).
+WaveArgs Operator::wave_args() {
+    if (kind_ != Kind_Elasticity) throw std::invalid_argument("the wave operator is built for elasticity");
+    if (!ready_) throw std::logic_error("set_material() must be called first");
+    upload_device();
+    if (!wave_) {
+        auto wv = std::make_unique<DeviceWave>();
+        const unsigned n = Nbf_;
+        std::vector<double> M(std::size_t(n) * n, 0.0), Minv(std::size_t(n) * n, 0.0);
+        for (unsigned q = 0; q < Nq_; ++q)
+            for (unsigned l = 0; l < n; ++l)
+                for (unsigned k = 0; k < n; ++k) M[k + l * n] += vol_.weights[q] * E_Q_[k + n * q] * E_Q_[l + n * q];
+        for (unsigned k = 0; k < n; ++k) Minv[k + k * n] = 1.0;
+        dense_solve(M, int(n), Minv.data(), int(n));
+        std::vector<double> tabE(std::size_t(n) * Nq_), w(E_ * Nq_);
+        for (unsigned k = 0; k < n; ++k)
+            for (unsigned q = 0; q < Nq_; ++q) tabE[k * Nq_ + q] = E_Q_[k + n * q];
+        for (std::size_t e = 0; e < E_; ++e)
+            for (unsigned q = 0; q < Nq_; ++q) w[e * Nq_ + q] = -vol_.weights[q] / absDetJ_[e * Nq_ + q];
+        wv->Minv.upload(Minv);
+        wv->E.upload(tabE);
+        wv->w.upload(w);
+        wv->ax.alloc(local_size());
+        wv->b.alloc(local_size());
+        wave_ = std::move(wv);
+    }
+    WaveArgs a;
+    a.Minv = wave_->Minv.ptr;
+    a.E = wave_->E.ptr;
+    a.w = wave_->w.ptr;
+    a.nbf = int(Nbf_);
+    a.nq = int(Nq_);
+    a.nquant = int(Q_);
+    a.num_elements = int64_t(E_);
+    return a;
+}
+
+void Operator::inverse_mass_device(double const* x, double* y) {
+    WaveArgs a = wave_args();
+    TDG_HIP(launch_inverse_mass(a, x, y, stream_));
+}
+
+// apply_<true> of Elasticity.cpp:865-880 followed by the inverse mass: y = M^-1 (A x - b_facet(f)).
+void Operator::wave_rhs_device(double const* x, double const* facet_q, double* y) {
+    WaveArgs a = wave_args();
+    apply(x, wave_->ax.ptr);
+    if (facet_q) {
+        rhs_device(nullptr, facet_q, wave_->b.ptr, 1, 0, 0, 0);
+        TDG_HIP(launch_axpy(wave_->ax.ptr, -1.0, wave_->b.ptr, int64_t(local_size()), stream_));
+    }
+    TDG_HIP(launch_inverse_mass(a, wave_->ax.ptr, y, stream_));
+}
+
+// I[k][l] = phi^from_l(x^to_k) on the nodal bases (NodalInterpolation.cpp:8-38).
+std::vector<double> Operator::interpolation_matrix(unsigned D, unsigned to, unsigned from) {
+    Basis bt(D, to, true), bf(D, from, true);
+    return bf.values(bt.nodes()); // column-major (n_from x n_to) == row-major (n_to x n_from)
+}
+
 BlockJacobi& Operator::block_jacobi() {
     if (!bj_) bj_ = std::make_unique<BlockJacobi>();
     if (!bj_->built()) bj_->build(*this);

@@ -62,6 +62,12 @@ struct DeviceRhs {
     DeviceArray<int32_t> fbc, fct; // fct: int4 per facet
 };
 
+// Data of the elastic-wave kernels: reference inverse mass, E table, -W/|J| per volume point and
+// two work vectors.
+struct DeviceWave {
+    DeviceArray<double> Minv, E, w, ax, b;
+};
+
 // Kernel selection: Variant_Auto uses the MFMA kernel where one is compiled (elasticity), the
 // thread-per-point kernel otherwise; Variant_Points forces the thread-per-point kernel;
 // Variant_MfmaXcd the MFMA kernel with workgroups renumbered per XCD.
@@ -129,6 +135,13 @@ public:
     void material_gradient(double const* u, double const* psi, double* g_lam, double* g_mu);
     void material_gradient(float const* u, float const* psi, float* g_lam, float* g_mu);
 
+    // Elastic-wave operator pieces on device vectors (elasticity, FP64): y = M^-1 x with the
+    // weight-adjusted inverse mass, and y = M^-1 (A x - b_facet(f)) (facet data may be null).
+    void inverse_mass_device(double const* x, double* y);
+    void wave_rhs_device(double const* x, double const* facet_q, double* y);
+    // Nodal interpolation matrix from degree `from` to degree `to`, row-major (to x from).
+    static std::vector<double> interpolation_matrix(unsigned D, unsigned to, unsigned from);
+
     // named host arrays for inspection
     std::vector<double> const* host_array(std::string const& name) const;
 
@@ -183,6 +196,8 @@ private:
     std::unique_ptr<BlockJacobi> bj_;
     std::unique_ptr<DeviceGrad<double>> g64_;
     std::unique_ptr<DeviceGrad<float>> g32_;
+    std::unique_ptr<DeviceWave> wave_;
+    WaveArgs wave_args();
     template <class T> void upload_grad(DeviceGrad<T>& g);
     template <class T> void grad_dispatch(T const* u, T const* psi, T* gl, T* gm);
     RhsArgs rhs_args();

@@ -52,6 +52,21 @@ template <class T>
 hipError_t launch_material_gradient(int dim, int degree, ApplyArgs<T> const& a, GradArgs<T> const& g, T const* u,
                                     T const* psi, T* g_lam, T* g_mu, hipStream_t stream);
 
+// Weight-adjusted inverse mass (FP64): reference inverse mass, basis at the volume points and the
+// per-point weights -W/|J|.
+struct WaveArgs {
+    double const* Minv; // [Nbf][Nbf] column-major reference inverse mass
+    double const* E;    // [k][q] basis at the volume points
+    double const* w;    // [E][Nq] -W / |det J|
+    int nbf, nq, nquant;
+    int64_t num_elements;
+};
+hipError_t launch_inverse_mass(WaveArgs const& a, double const* x, double* y, hipStream_t stream);
+// y = I x per element and quantity (I: n_to x n_from row-major), or y = I^T x if transpose.
+hipError_t launch_interpolate(double const* I, int n_to, int n_from, int nquant, int64_t nelem, int transpose,
+                              double const* x, double* y, hipStream_t stream);
+hipError_t launch_axpy(double* y, double alpha, double const* x, int64_t n, hipStream_t stream);
+
 // Volume / facet rule sizes the kernels are compiled for (2N+1 rules of tandem's autorule).
 int kernel_rule_sizes(int dim, int degree, int* Nq, int* nq);
 

@@ -67,6 +67,30 @@ def reference_nodes(dim, degree):
     return out.reshape(-1, dim)
 
 
+def interpolation_matrix(dim, to_degree, from_degree):
+    """Nodal interpolation matrix (n_to x n_from) from degree from_degree to to_degree."""
+    nt, nf = comb(to_degree + dim, dim), comb(from_degree + dim, dim)
+    out = np.zeros(nt * nf)
+    check(lib().tdg_interpolation_matrix(dim, to_degree, from_degree, out))
+    return out.reshape(nt, nf)
+
+
+def interpolate(I, x, num_quantities, transpose=False, y=None):
+    """Per element and quantity y = I x (or I^T x) on the GPU; I is a (n_to, n_from) float64 GPU
+    tensor and x holds elements x quantities x (n_from or n_to) values."""
+    torch = _torch()
+    nt, nf = I.shape
+    n_in, n_out = (nt, nf) if transpose else (nf, nt)
+    assert I.is_cuda and I.dtype == torch.float64 and I.is_contiguous()
+    assert x.is_cuda and x.dtype == torch.float64 and x.numel() % (n_in * num_quantities) == 0
+    E = x.numel() // (n_in * num_quantities)
+    y = torch.empty(E * num_quantities * n_out, dtype=x.dtype, device=x.device) if y is None else y
+    stream = torch.cuda.current_stream(x.device).cuda_stream
+    check(lib().tdg_interpolate(C.c_void_p(I.data_ptr()), nt, nf, num_quantities, E, 1 if transpose else 0,
+                                C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), C.c_void_p(stream)))
+    return y
+
+
 def _torch():
     import torch
     return torch
@@ -268,6 +292,25 @@ class DGOperator:
         check(f(self._h, C.c_void_p(u.data_ptr()), C.c_void_p(psi.data_ptr()), C.c_void_p(g_lam.data_ptr()),
                 C.c_void_p(g_mu.data_ptr())))
         return g_lam.view(self.E, self.Nq), g_mu.view(self.E, self.Nq)
+
+    def inverse_mass(self, x, y=None):
+        """y = M^-1 x (weight-adjusted inverse mass with the reference's negative sign) for a float64
+        GPU vector x."""
+        torch = _torch()
+        assert x.dtype == torch.float64 and x.is_cuda
+        y = torch.empty_like(x) if y is None else y
+        check(lib().tdg_inverse_mass(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
+        return y
+
+    def wave_rhs(self, x, facet=None, y=None):
+        """y = M^-1 (A x - b_facet(facet)) for a float64 GPU vector x and optional GPU facet data
+        (facets x quantities x facet points)."""
+        torch = _torch()
+        assert x.dtype == torch.float64 and x.is_cuda
+        y = torch.empty_like(x) if y is None else y
+        fp = None if facet is None else C.c_void_p(facet.data_ptr())
+        check(lib().tdg_wave_rhs(self._h, C.c_void_p(x.data_ptr()), fp, C.c_void_p(y.data_ptr())))
+        return y
 
     def apply_counts(self, precision_bytes=8):
         b, f = C.c_double(), C.c_double()

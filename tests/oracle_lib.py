@@ -64,6 +64,9 @@ def lib():
     L.oracle_op_rhs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, dp]
     L.oracle_op_traction.argtypes = [C.c_void_p, C.c_int64, dp, C.c_void_p, dp]
     L.oracle_op_material_gradient.argtypes = [C.c_void_p, dp, dp, dp, dp]
+    L.oracle_op_wave_rhs.argtypes = [C.c_void_p, dp, C.c_void_p, dp]
+    L.oracle_op_inverse_mass.argtypes = [C.c_void_p, dp, dp]
+    L.oracle_nodal_interpolation.argtypes = [C.c_int, C.c_int, C.c_int, dp]
     L.oracle_op_facets.argtypes = [C.c_void_p, ip64]
     L.oracle_op_eval_volume.argtypes = [C.c_void_p, dp, dp, C.c_void_p]
     L.oracle_op_array.argtypes = [C.c_void_p, C.c_char_p, dp]
@@ -273,6 +276,19 @@ class Operator:
             _err()
         return gl.reshape(self.E, self.Nq), gm.reshape(self.E, self.Nq)
 
+    def inverse_mass(self, x):
+        y = np.zeros(self.n)
+        if lib().oracle_op_inverse_mass(self.h, np.ascontiguousarray(x, dtype=np.float64), y) != 0:
+            _err()
+        return y
+
+    def wave_rhs(self, x, f_q=None):
+        y = np.zeros(self.n)
+        fq = None if f_q is None else np.ascontiguousarray(f_q, dtype=np.float64)
+        if lib().oracle_op_wave_rhs(self.h, np.ascontiguousarray(x, dtype=np.float64), _ptr(fq), y) != 0:
+            _err()
+        return y
+
     def eval_volume(self, x):
         """Discrete solution at the volume points (E, Q, Nq) and the weights W|J| (E, Nq)."""
         Q = self.D if self.kind == 0 else 1
@@ -299,3 +315,12 @@ class Operator:
         if getattr(self, "h", None) and _lib is not None:
             _lib.oracle_op_free(self.h)
             self.h = None
+
+
+def nodal_interpolation(D, to, frm):
+    """I (n_to, n_from): the degree-frm nodal basis at the degree-to warp-and-blend nodes."""
+    from math import comb
+    out = np.zeros(comb(to + D, D) * comb(frm + D, D))
+    if lib().oracle_nodal_interpolation(D, to, frm, out) != 0:
+        _err()
+    return out.reshape(comb(to + D, D), comb(frm + D, D))

@@ -120,3 +120,14 @@ def test_rhs_with_fault_and_boundary_fault_matches_oracle(kind, D, N, scheme):
     g = rng.uniform(-1, 1, (op.F, Q, op.nq))
     rb, ro = op.rhs(f, g), oo.rhs(f, g)
     assert np.abs(rb - ro).max() <= 1e-12 * np.abs(ro).max()
+
+
+@pytest.mark.parametrize("D,to,frm", [(3, 4, 2), (3, 2, 1), (3, 4, 3), (2, 6, 3), (2, 3, 2), (2, 2, 1)])
+def test_interpolation_matrix_matches_oracle(D, to, frm):
+    """Host nodal interpolation matrix (NodalInterpolation.cpp:8-38) equals the oracle's, rows sum
+    to one (constants are reproduced) and nested nodes give unit rows."""
+    I = T.interpolation_matrix(D, to, frm)
+    ref = O.nodal_interpolation(D, to, frm)
+    assert I.shape == ref.shape
+    assert np.abs(I - ref).max() <= 1e-13
+    assert np.abs(I.sum(axis=1) - 1).max() <= 1e-13

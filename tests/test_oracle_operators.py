@@ -346,3 +346,59 @@ def test_material_gradient_matches_finite_differences(D, N, scheme, bcs):
         rich = (4.0 * fd(dl, dm, eps / 2) - fd(dl, dm, eps)) / 3.0
         an = np.sum(gl * dl) + np.sum(gm * dm)
         assert abs(rich - an) <= 1e-8 * abs(an), (trial, rich, an)
+
+
+@pytest.mark.parametrize("D,N,scheme", [(3, 3, 1), (2, 4, 0)])
+def test_inverse_mass_is_exact_on_affine_elements(D, N, scheme):
+    """apply_inverse_mass (elasticity.py:236-238) with weights -W/|J|: on affine elements |J| is
+    constant and Mhat^-1 E diag(W/|J|) E^T Mhat^-1 = M_K^-1 exactly, so it inverts the element
+    mass matrix (M_K u built independently as rhsVolume of u at the volume points)."""
+    m = O.Mesh.box(D, [2] * D, scheme, [BC_D] * (2 * D))
+    vc, _ = m.coords(N)
+    op = O.Operator(0, m, N, np.full(vc.shape[:2], 2.0), np.full(vc.shape[:2], 1.0))
+    x = np.random.default_rng(5).uniform(-1, 1, op.n)
+    uq, _ = op.eval_volume(x)                     # (E, D, Nq)
+    Mx = op.rhs(uq, None)                         # M_K x per element
+    np.testing.assert_allclose(op.inverse_mass(Mx), -x, rtol=0, atol=1e-11)
+    # curved elements: only approximately (weight-adjusted inverse)
+    nodes = warp_nodes(m.affine_nodes(N), 0.05)
+    vc2, _ = m.coords(N, nodes)
+    op2 = O.Operator(0, m, N, np.full(vc2.shape[:2], 2.0), np.full(vc2.shape[:2], 1.0), nodes=nodes)
+    uq2, _ = op2.eval_volume(x)
+    err = np.linalg.norm(op2.inverse_mass(op2.rhs(uq2, None)) + x) / np.linalg.norm(x)
+    assert 1e-8 < err < 0.1
+
+
+def test_wave_rhs_of_a_rigid_motion_vanishes():
+    """wave_rhs (apply_<true> + apply_inverse_mass): a rigid translation with matching Dirichlet
+    data has zero stress and zero jumps, so its acceleration vanishes."""
+    D, N = 3, 2
+    m = O.Mesh.box(D, [2] * D, 1, [BC_D] * 6)
+    nodes = warp_nodes(m.affine_nodes(N), 0.02)
+    vc, fc = m.coords(N, nodes)
+    op = O.Operator(0, m, N, np.full(vc.shape[:2], 2.0), np.full(vc.shape[:2], 1.0), nodes=nodes)
+    t = np.array([0.3, -0.1, 0.2])
+    x = np.repeat(t[None, :, None], op.bs // 3, axis=2).repeat(1, axis=0)
+    x = np.broadcast_to(x, (op.E, 3, op.bs // 3)).reshape(-1).copy()
+    f = np.broadcast_to(t[None, :, None], (op.F, 3, op.nq)).copy()
+    assert np.abs(op.wave_rhs(x, f)).max() < 1e-10
+    assert np.abs(op.wave_rhs(x, None)).max() > 1e-3
+
+
+@pytest.mark.parametrize("D,to,frm", [(2, 4, 2), (3, 3, 1), (3, 4, 3), (2, 6, 5)])
+def test_nodal_interpolation_reproduces_polynomials(D, to, frm):
+    """NodalInterpolation (NodalInterpolation.cpp:8-38): a degree-frm polynomial's nodal values are
+    carried exactly to the degree-to nodes; rows sum to one."""
+    I = O.nodal_interpolation(D, to, frm)
+    xt, xf = O.wab_nodes(D, to), O.wab_nodes(D, frm)
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        c = rng.normal(size=(frm + 1,) * D)
+        def poly(X):
+            v = np.zeros(X.shape[0])
+            for idx in np.ndindex(*c.shape):
+                if sum(idx) <= frm:
+                    v += c[idx] * np.prod(X ** np.array(idx), axis=1)
+            return v
+        np.testing.assert_allclose(I @ poly(xf), poly(xt), atol=1e-12)
+    np.testing.assert_allclose(I.sum(axis=1), 1.0, atol=1e-13)
