@@ -98,6 +98,13 @@ int tdg_set_stream(tdg_op* op, void* stream);
 enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1, TDG_VARIANT_MFMA_XCD = 2 };
 int tdg_set_variant(tdg_op* op, int variant);
 
+/* Autotuner: times every variant that applies to the operator (the MFMA kernel with linear and
+ * XCD-grouped workgroup order, the thread-per-point kernel) on ncols columns in FP64
+ * (precision_bytes 8) or FP32 (4), keeps the fastest as the operator's variant and reports it in
+ * *chosen and the milliseconds per apply in ms[variant] (3 entries, -1 where a variant does not
+ * apply).  No counterpart in the reference (its kernels are generated for one CPU target). */
+int tdg_autotune(tdg_op* op, int ncols, int precision_bytes, int* chosen, double* ms);
+
 /* Right-hand side b (host, local size) from a body force at the volume points (elements x Q x Nq,
  * may be NULL) and facet data at the facet points (facets x Q x nq, may be NULL): the Dirichlet
  * values on Dirichlet facets and the slip on Fault facets (halved on boundary Fault facets, as
@@ -162,6 +169,16 @@ int tdg_solve(tdg_op* op, double const* b, double* x, tdg_solve_opts const* opts
  * result->iterations = lockstep iterations, result->outer_iterations = replacements. */
 int tdg_solve_multi(tdg_op* op, double const* B, double* X, int ncols, int64_t ld, tdg_solve_opts const* opts,
                     tdg_solve_result* result, int* iterations, double* rel_residual);
+
+/* Independent meshes held in one operator (created from the disjoint union of their meshes):
+ * segment c of the device vectors b, x is [offsets[c], offsets[c+1]) (host array of nseg+1 DoF
+ * offsets, whole elements, offsets[0] = 0, offsets[nseg] = local size; no facet may join two
+ * segments).  Each segment is its own CG recursion, all advanced in lockstep by one apply of the
+ * whole operator per iteration (one launch over the element groups of every mesh).  The reference
+ * solves one mesh per KSPSolve (app/common/PetscLinearSolver.h:42); no counterpart for several.
+ * Modes, preconditioner and outputs as for tdg_solve_multi, per segment. */
+int tdg_solve_segments(tdg_op* op, double const* b, double* x, int nseg, int64_t const* offsets,
+                       tdg_solve_opts const* opts, tdg_solve_result* result, int* iterations, double* rel_residual);
 
 /* Element-block Jacobi preconditioner (the inverse of each element's diagonal block of A, probed
  * with the matrix-free apply on an element colouring; assemble_volume/skeleton/boundary restricted

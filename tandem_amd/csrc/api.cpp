@@ -289,6 +289,40 @@ int tdg_solve_multi(tdg_op* op, double const* B, double* X, int ncols, int64_t l
     });
 }
 
+int tdg_autotune(tdg_op* op, int ncols, int precision_bytes, int* chosen, double* ms) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(ncols >= 1 && (precision_bytes == 8 || precision_bytes == 4), "bad autotune arguments");
+        auto t = o.autotune(ncols, precision_bytes);
+        if (chosen) *chosen = o.variant();
+        if (ms)
+            for (int v = 0; v < 3; ++v) ms[v] = t[std::size_t(v)];
+    });
+}
+
+int tdg_solve_segments(tdg_op* op, double const* b, double* x, int nseg, int64_t const* offsets,
+                       tdg_solve_opts const* opts, tdg_solve_result* result, int* iterations, double* rel_residual) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(b && x && opts && result && offsets && b != x && nseg >= 1, "bad segmented solve arguments");
+        auto so = solve_options(opts);
+        require(so.mode != TDG_SOLVE_FP32_CG_FP64_IR, "segmented solves support modes 0 and 2");
+        auto r = tdg::solve_segments(o, b, x, nseg, offsets, so);
+        result->iterations = r.total_iterations;
+        result->outer_iterations = r.replacements;
+        result->inner_iterations = r.total_iterations;
+        double worst = 0.0;
+        for (int c = 0; c < nseg; ++c) {
+            worst = std::max(worst, r.rel_residual[c]);
+            if (iterations) iterations[c] = r.iterations[c];
+            if (rel_residual) rel_residual[c] = r.rel_residual[c];
+        }
+        result->rel_residual = worst;
+        result->seconds = r.seconds;
+        result->converged = r.converged;
+    });
+}
+
 int tdg_block_jacobi_build(tdg_op* op, int* colours) {
     return guarded([&] {
         auto& bj = ready(op).block_jacobi();

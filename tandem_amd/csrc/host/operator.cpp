@@ -413,6 +413,60 @@ void Operator::set_variant(int v) {
     variant_ = v;
 }
 
+std::vector<double> Operator::autotune(int ncols, int pb) {
+    if (!ready_) throw std::logic_error("set_material() must be called first");
+    if (ncols < 1 || (pb != 8 && pb != 4)) throw std::invalid_argument("bad autotune arguments");
+    const int saved = variant_;
+    std::vector<double> ms(3, -1.0);
+    const bool mfma = kind_ == Kind_Elasticity && mfma_supported(D_, N_);
+    const std::size_t n = local_size() * std::size_t(ncols);
+    void *X = nullptr, *Y = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+        if (X) (void)hipFree(X);
+        if (Y) (void)hipFree(Y);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+        TDG_HIP(hipMalloc(&X, n * pb));
+        TDG_HIP(hipMalloc(&Y, n * pb));
+        TDG_HIP(hipMemsetAsync(X, 0, n * pb, stream_));
+        TDG_HIP(hipEventCreate(&e0));
+        TDG_HIP(hipEventCreate(&e1));
+        const int64_t ld = int64_t(local_size());
+        for (int v : {int(Variant_Auto), int(Variant_MfmaXcd), int(Variant_Points)}) {
+            if (v != Variant_Points && !mfma) continue;
+            variant_ = v;
+            upload_device();
+            auto run = [&] {
+                if (pb == 8) dispatch(static_cast<double const*>(X), static_cast<double*>(Y), ncols, ld);
+                else dispatch(static_cast<float const*>(X), static_cast<float*>(Y), ncols, ld);
+            };
+            run();
+            run();
+            constexpr int reps = 5;
+            TDG_HIP(hipEventRecord(e0, stream_));
+            for (int r = 0; r < reps; ++r) run();
+            TDG_HIP(hipEventRecord(e1, stream_));
+            TDG_HIP(hipEventSynchronize(e1));
+            float t = 0.0f;
+            TDG_HIP(hipEventElapsedTime(&t, e0, e1));
+            ms[std::size_t(v)] = double(t) / reps;
+        }
+    } catch (...) {
+        variant_ = saved;
+        cleanup();
+        throw;
+    }
+    cleanup();
+    int best = -1;
+    for (int v = 0; v < 3; ++v)
+        if (ms[std::size_t(v)] >= 0.0 && (best < 0 || ms[std::size_t(v)] < ms[std::size_t(best)])) best = v;
+    variant_ = best;
+    return ms;
+}
+
 bool Operator::uses_mfma() const {
     return variant_ != Variant_Points && kind_ == Kind_Elasticity && mfma_supported(D_, N_);
 }

@@ -108,6 +108,10 @@ public:
     hipStream_t stream() const { return stream_; }
     void set_variant(int v);
     int variant() const { return variant_; }
+    // Times every kernel variant that applies to this operator on ncols columns in the given
+    // precision (8 or 4 bytes), keeps the fastest and returns the milliseconds per apply indexed by
+    // variant (negative where a variant does not apply).
+    std::vector<double> autotune(int ncols, int precision_bytes);
     bool uses_mfma() const;
 
     // y = A x on device buffers (x, y of local_size() entries).

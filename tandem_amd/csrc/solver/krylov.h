@@ -49,4 +49,10 @@ struct MultiSolveResult {
 // apply; modes Solve_FP64_CG and Solve_FP32_CG_FP64_RELIABLE.  X overwritten (initial guess 0).
 MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols, int64_t ld, SolveOptions const& o);
 
+// Independent meshes held in one operator: segment c of b and x is [offsets[c], offsets[c+1]) (whole
+// elements, no facet between segments); each segment is its own CG recursion (own scalars,
+// convergence and FP64 replacements), all advanced in lockstep by one apply of the operator.
+MultiSolveResult solve_segments(Operator& op, double const* b, double* x, int nseg, int64_t const* offsets,
+                                SolveOptions const& o);
+
 } // namespace tdg

@@ -8,7 +8,9 @@
 // its step lengths are zero), and in the FP32 mode its own FP64 residual replacements (the
 // reliable-update scheme of the single-vector solver, applied to the columns whose residual
 // dropped by reliable_delta at a check).  Reductions are two-stage with a fixed grid (bitwise
-// reproducible).  Vectors are column-major: column c at offset c ld.
+// reproducible).  Vectors are column-major (column c at offset c ld, one batched apply per
+// iteration) or, for independent meshes held in one operator, segments of one vector (segment c at
+// off[c] with len[c] entries, one apply of the whole operator per iteration).
 #include "krylov.h"
 
 #include "../host/operator.h"
@@ -36,6 +38,15 @@ constexpr int kB = 256;
 constexpr int kG = 256;  // partial sums per column
 constexpr int kS = 8;    // scalars per column: 0,1 rho; 2 p.Ap; 3 b.b; 4 r.r; 5 active; 6 replace
 
+// Where column c lives: c ld with n entries, or the device tables off/len (segments).
+struct Cols {
+    int64_t ld = 0, n = 0;
+    int64_t const* off = nullptr;
+    int64_t const* len = nullptr;
+    __device__ int64_t base(int64_t c) const { return off ? off[c] : c * ld; }
+    __device__ int64_t size(int64_t c) const { return off ? len[c] : n; }
+};
+
 __device__ inline double bsum(double v) {
     __shared__ double s[kB / 64];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -49,11 +60,11 @@ __device__ inline double bsum(double v) {
 }
 
 template <class A, class B>
-__global__ __launch_bounds__(kB) void mdot(A const* __restrict__ a, B const* __restrict__ b, int64_t n, int64_t ld,
+__global__ __launch_bounds__(kB) void mdot(A const* __restrict__ a, B const* __restrict__ b, Cols L,
                                            double* __restrict__ partial) {
-    const int64_t c = blockIdx.y;
-    A const* ac = a + c * ld;
-    B const* bc = b + c * ld;
+    const int64_t c = blockIdx.y, n = L.size(c);
+    A const* ac = a + L.base(c);
+    B const* bc = b + L.base(c);
     double v = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
         v += double(ac[i]) * double(bc[i]);
@@ -74,16 +85,16 @@ __global__ __launch_bounds__(kB) void mfinal(double const* __restrict__ partial,
 // active columns: x += alpha p, r -= alpha q, alpha = rho / pAp; partial sums of r.r
 template <class T, class TX>
 __global__ __launch_bounds__(kB) void mxr(TX* __restrict__ x, T* __restrict__ r, T const* __restrict__ p,
-                                          T const* __restrict__ q, int64_t n, int64_t ld, double const* __restrict__ S,
+                                          T const* __restrict__ q, Cols L, double const* __restrict__ S,
                                           int irho, double* __restrict__ partial) {
-    const int64_t c = blockIdx.y;
+    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
     double const* s = S + c * kS;
     const bool act = s[5] != 0.0;
     const T a = act ? T(s[irho] / s[2]) : T(0);
-    TX* xc = x + c * ld;
-    T* rc = r + c * ld;
-    T const* pc = p + c * ld;
-    T const* qc = q + c * ld;
+    TX* xc = x + o;
+    T* rc = r + o;
+    T const* pc = p + o;
+    T const* qc = q + o;
     double v = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) {
         T ri = rc[i];
@@ -100,53 +111,53 @@ __global__ __launch_bounds__(kB) void mxr(TX* __restrict__ x, T* __restrict__ r,
 
 // active columns: p = z + (rho_new / rho_old) p
 template <class T>
-__global__ __launch_bounds__(kB) void mp(T* __restrict__ p, T const* __restrict__ z, int64_t n, int64_t ld,
+__global__ __launch_bounds__(kB) void mp(T* __restrict__ p, T const* __restrict__ z, Cols L,
                                          double const* __restrict__ S, int inew, int iold) {
-    const int64_t c = blockIdx.y;
+    const int64_t c = blockIdx.y, n = L.size(c);
     double const* s = S + c * kS;
     if (s[5] == 0.0) return;
     const T beta = T(s[inew] / s[iold]);
-    T* pc = p + c * ld;
-    T const* zc = z + c * ld;
+    T* pc = p + L.base(c);
+    T const* zc = z + L.base(c);
     for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
         pc[i] = zc[i] + beta * pc[i];
 }
 
 template <class T>
-__global__ __launch_bounds__(kB) void mcopy(T* __restrict__ d, T const* __restrict__ s, int64_t n, int64_t ld) {
-    const int64_t c = blockIdx.y;
+__global__ __launch_bounds__(kB) void mcopy(T* __restrict__ d, T const* __restrict__ s, Cols L) {
+    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
     for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
-        d[c * ld + i] = s[c * ld + i];
+        d[o + i] = s[o + i];
 }
 template <class T>
-__global__ __launch_bounds__(kB) void mfill(T* __restrict__ d, T v, int64_t n, int64_t ld) {
-    const int64_t c = blockIdx.y;
-    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) d[c * ld + i] = v;
+__global__ __launch_bounds__(kB) void mfill(T* __restrict__ d, T v, Cols L) {
+    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
+    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) d[o + i] = v;
 }
 
 // columns flagged for replacement (S[c][6]): x += xa (FP64), xa = 0
-__global__ __launch_bounds__(kB) void mfold(double* __restrict__ x, float* __restrict__ xa, int64_t n, int64_t ld,
+__global__ __launch_bounds__(kB) void mfold(double* __restrict__ x, float* __restrict__ xa, Cols L,
                                             double const* __restrict__ S) {
-    const int64_t c = blockIdx.y;
+    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
     if (S[c * kS + 6] == 0.0) return;
     for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) {
-        x[c * ld + i] += double(xa[c * ld + i]);
-        xa[c * ld + i] = 0.0f;
+        x[o + i] += double(xa[o + i]);
+        xa[o + i] = 0.0f;
     }
 }
 
 // flagged columns (or all if all != 0): r = b - Ax in FP64, r32 = float(r); partial sums of r.r
 __global__ __launch_bounds__(kB) void mresidual(double const* __restrict__ b, double const* __restrict__ Ax,
-                                                double* __restrict__ r, float* __restrict__ r32, int64_t n, int64_t ld,
+                                                double* __restrict__ r, float* __restrict__ r32, Cols L,
                                                 double const* __restrict__ S, int all, double* __restrict__ partial) {
-    const int64_t c = blockIdx.y;
+    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
     const bool on = all || S[c * kS + 6] != 0.0;
     double v = 0.0;
     if (on)
         for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) {
-            const double ri = b[c * ld + i] - Ax[c * ld + i];
-            r[c * ld + i] = ri;
-            if (r32) r32[c * ld + i] = float(ri);
+            const double ri = b[o + i] - Ax[o + i];
+            r[o + i] = ri;
+            if (r32) r32[o + i] = float(ri);
             v += ri * ri;
         }
     v = bsum(v);
@@ -163,9 +174,10 @@ template <class T> struct DBuf {
 
 } // namespace
 
-MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols, int64_t ld, SolveOptions const& o) {
-    if (!op.ready()) throw std::logic_error("set_material() must be called before solve");
-    if (ncols < 1 || ld < int64_t(op.local_size())) throw std::invalid_argument("bad column layout");
+// The lockstep recursions over ncols columns laid out by L (tot entries per vector).  Segments:
+// one apply of the whole operator per iteration; columns: one batched apply.
+MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols, Cols L, std::size_t tot,
+                             bool segments, SolveOptions const& o) {
     if (o.mode != Solve_FP64_CG && o.mode != Solve_FP32_CG_FP64_RELIABLE)
         throw std::invalid_argument("multi-column solves support FP64 CG and FP32 CG with FP64 replacement");
     op.upload_device();
@@ -173,9 +185,16 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
     if (o.precond == Precond_BlockJacobi) M = &op.block_jacobi();
     else if (o.precond != Precond_None) throw std::invalid_argument("unknown preconditioner");
     hipStream_t st = op.stream();
-    const int64_t n = int64_t(op.local_size());
-    const std::size_t tot = std::size_t(ncols) * ld;
+    const int64_t ld = L.ld;
     const dim3 g(kG, ncols), bl(kB);
+    auto applyA = [&](auto const* P, auto* Q) {
+        if (segments) op.apply(P, Q);
+        else op.apply_batched(P, Q, ncols, ld);
+    };
+    auto applyM = [&](auto const* R, auto* Z) {
+        if (segments) M->apply(R, Z, st, 1, 0);
+        else M->apply(R, Z, st, ncols, ld);
+    };
     DBuf<double> S, partial;
     S.alloc(std::size_t(ncols) * kS);
     partial.alloc(std::size_t(ncols) * kG);
@@ -195,7 +214,7 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
     // b.b per column, all columns active
     for (int c = 0; c < ncols; ++c) hs[c * kS + 5] = 1.0;
     push();
-    hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, B, B, n, ld, partial.p);
+    hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, B, B, L, partial.p);
     hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 3, -1);
     MHIP(hipGetLastError());
     pull();
@@ -204,7 +223,7 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
         target[c] = o.rtol * o.rtol * hs[c * kS + 3];
         if (hs[c * kS + 3] == 0.0) hs[c * kS + 5] = 0.0;
     }
-    hipLaunchKernelGGL((mfill<double>), g, bl, 0, st, X, 0.0, n, ld);
+    hipLaunchKernelGGL((mfill<double>), g, bl, 0, st, X, 0.0, L);
 
     if (o.mode == Solve_FP64_CG) {
         DBuf<double> r, p, q, z;
@@ -213,30 +232,30 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
         q.alloc(tot);
         if (M) z.alloc(tot);
         push();
-        hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, r.p, B, n, ld);
-        hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, n, ld, partial.p);
+        hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, r.p, B, L);
+        hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, L, partial.p);
         hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
         auto prec = [&](int slot) {
             if (M) {
-                M->apply(r.p, z.p, st, ncols, ld);
-                hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, z.p, n, ld, partial.p);
+                applyM(r.p, z.p);
+                hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, z.p, L, partial.p);
             } else {
-                hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, n, ld, partial.p);
+                hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, L, partial.p);
             }
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, slot, -1);
         };
         prec(0);
-        hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, p.p, M ? z.p : r.p, n, ld);
+        hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, p.p, M ? z.p : r.p, L);
         MHIP(hipGetLastError());
         int it = 0, cur = 0;
         while (it < o.max_iter) {
-            op.apply_batched(p.p, q.p, ncols, ld);
-            hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, p.p, q.p, n, ld, partial.p);
+            applyA(p.p, q.p);
+            hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, p.p, q.p, L, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 2, -1);
-            hipLaunchKernelGGL((mxr<double, double>), g, bl, 0, st, X, r.p, p.p, q.p, n, ld, S.p, cur, partial.p);
+            hipLaunchKernelGGL((mxr<double, double>), g, bl, 0, st, X, r.p, p.p, q.p, L, S.p, cur, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
             prec(1 - cur);
-            hipLaunchKernelGGL((mp<double>), g, bl, 0, st, p.p, M ? z.p : r.p, n, ld, S.p, 1 - cur, cur);
+            hipLaunchKernelGGL((mp<double>), g, bl, 0, st, p.p, M ? z.p : r.p, L, S.p, 1 - cur, cur);
             cur = 1 - cur;
             ++it;
             if (it % o.check_every == 0 || it == o.max_iter) {
@@ -257,8 +276,8 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
         }
         res.total_iterations = it;
         // true residuals
-        op.apply_batched(X, q.p, ncols, ld);
-        hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, q.p, r.p, (float*)nullptr, n, ld, S.p, 1, partial.p);
+        applyA(X, q.p);
+        hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, q.p, r.p, (float*)nullptr, L, S.p, 1, partial.p);
         hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
         MHIP(hipGetLastError());
         pull();
@@ -273,22 +292,22 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
         q.alloc(tot);
         if (M) z.alloc(tot);
         push();
-        hipLaunchKernelGGL((mfill<float>), g, bl, 0, st, xa.p, 0.0f, n, ld);
-        hipLaunchKernelGGL((mfill<double>), g, bl, 0, st, Ax.p, 0.0, n, ld);
-        hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, Ax.p, r.p, r32.p, n, ld, S.p, 1, partial.p);
+        hipLaunchKernelGGL((mfill<float>), g, bl, 0, st, xa.p, 0.0f, L);
+        hipLaunchKernelGGL((mfill<double>), g, bl, 0, st, Ax.p, 0.0, L);
+        hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, Ax.p, r.p, r32.p, L, S.p, 1, partial.p);
         hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
         // rho into slot `slot` of the columns flagged in slot msk (all if msk < 0)
         auto prec = [&](int slot, int msk) {
             if (M) {
-                M->apply(r32.p, z.p, st, ncols, ld);
-                hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, z.p, n, ld, partial.p);
+                applyM(r32.p, z.p);
+                hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, z.p, L, partial.p);
             } else {
-                hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, r32.p, n, ld, partial.p);
+                hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, r32.p, L, partial.p);
             }
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, slot, msk);
         };
         prec(0, -1);
-        hipLaunchKernelGGL((mcopy<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, n, ld);
+        hipLaunchKernelGGL((mcopy<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, L);
         MHIP(hipGetLastError());
         pull();
         std::vector<double> maxr(ncols);
@@ -296,10 +315,10 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
         int it = 0, cur = 0;
         res.replacements = 0;
         while (it < o.max_iter) {
-            op.apply_batched(p.p, q.p, ncols, ld);
-            hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, p.p, q.p, n, ld, partial.p);
+            applyA(p.p, q.p);
+            hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, p.p, q.p, L, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 2, -1);
-            hipLaunchKernelGGL((mxr<float, float>), g, bl, 0, st, xa.p, r32.p, p.p, q.p, n, ld, S.p, cur, partial.p);
+            hipLaunchKernelGGL((mxr<float, float>), g, bl, 0, st, xa.p, r32.p, p.p, q.p, L, S.p, cur, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
             ++it;
             bool any_replace = false;
@@ -322,9 +341,9 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
             }
             if (any_replace) {
                 // FP64 replacement of the flagged columns: x += xa, r = b - A x
-                hipLaunchKernelGGL(mfold, g, bl, 0, st, X, xa.p, n, ld, S.p);
-                op.apply_batched(X, Ax.p, ncols, ld);
-                hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, Ax.p, r.p, r32.p, n, ld, S.p, 0, partial.p);
+                hipLaunchKernelGGL(mfold, g, bl, 0, st, X, xa.p, L, S.p);
+                applyA(X, Ax.p);
+                hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, Ax.p, r.p, r32.p, L, S.p, 0, partial.p);
                 hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, 6);
                 MHIP(hipGetLastError());
                 pull();
@@ -341,16 +360,16 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
                 if (hs[c * kS + 5] != 0.0) ++active;
             if ((any_replace || it % o.check_every == 0) && !active) break;
             prec(1 - cur, -1);
-            hipLaunchKernelGGL((mp<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, n, ld, S.p, 1 - cur, cur);
+            hipLaunchKernelGGL((mp<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, L, S.p, 1 - cur, cur);
             cur = 1 - cur;
         }
         res.total_iterations = it;
         // fold any remaining update and report the true residuals
         for (int c = 0; c < ncols; ++c) hs[c * kS + 6] = 1.0;
         push();
-        hipLaunchKernelGGL(mfold, g, bl, 0, st, X, xa.p, n, ld, S.p);
-        op.apply_batched(X, Ax.p, ncols, ld);
-        hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, Ax.p, r.p, (float*)nullptr, n, ld, S.p, 1, partial.p);
+        hipLaunchKernelGGL(mfold, g, bl, 0, st, X, xa.p, L, S.p);
+        applyA(X, Ax.p);
+        hipLaunchKernelGGL(mresidual, g, bl, 0, st, B, Ax.p, r.p, (float*)nullptr, L, S.p, 1, partial.p);
         hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
         MHIP(hipGetLastError());
         pull();
@@ -364,6 +383,44 @@ MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols
     MHIP(hipStreamSynchronize(st));
     res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return res;
+}
+
+MultiSolveResult solve_multi(Operator& op, double const* B, double* X, int ncols, int64_t ld, SolveOptions const& o) {
+    if (!op.ready()) throw std::logic_error("set_material() must be called before solve");
+    if (ncols < 1 || ld < int64_t(op.local_size())) throw std::invalid_argument("bad column layout");
+    Cols L;
+    L.ld = ld;
+    L.n = int64_t(op.local_size());
+    return lockstep_cg(op, B, X, ncols, L, std::size_t(ncols) * ld, false, o);
+}
+
+MultiSolveResult solve_segments(Operator& op, double const* b, double* x, int nseg, int64_t const* offsets,
+                                SolveOptions const& o) {
+    if (!op.ready()) throw std::logic_error("set_material() must be called before solve");
+    const int64_t n = int64_t(op.local_size()), bs = int64_t(op.block_size());
+    if (nseg < 1 || !offsets || offsets[0] != 0 || offsets[nseg] != n)
+        throw std::invalid_argument("segment offsets must start at 0 and end at the vector size");
+    for (int c = 0; c < nseg; ++c)
+        if (offsets[c + 1] <= offsets[c] || offsets[c] % bs != 0)
+            throw std::invalid_argument("segments must be non-empty ranges of whole elements");
+    // the segments must be uncoupled: no interior facet joins elements of two segments
+    std::vector<int32_t> seg(op.num_elements());
+    for (int c = 0; c < nseg; ++c)
+        for (int64_t e = offsets[c] / bs; e < offsets[c + 1] / bs; ++e) seg[std::size_t(e)] = c;
+    for (auto const& f : op.topology().facets)
+        if (seg[f.elem[0]] != seg[f.elem[1]])
+            throw std::invalid_argument("a facet couples two segments: they are not independent meshes");
+    std::vector<int64_t> off(offsets, offsets + nseg), len(nseg);
+    for (int c = 0; c < nseg; ++c) len[c] = offsets[c + 1] - offsets[c];
+    DBuf<int64_t> doff, dlen;
+    doff.alloc(nseg);
+    dlen.alloc(nseg);
+    MHIP(hipMemcpy(doff.p, off.data(), nseg * sizeof(int64_t), hipMemcpyHostToDevice));
+    MHIP(hipMemcpy(dlen.p, len.data(), nseg * sizeof(int64_t), hipMemcpyHostToDevice));
+    Cols L;
+    L.off = doff.p;
+    L.len = dlen.p;
+    return lockstep_cg(op, b, x, nseg, L, std::size_t(n), true, o);
 }
 
 } // namespace tdg

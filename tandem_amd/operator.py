@@ -53,6 +53,19 @@ class Mesh:
         check(lib().tdg_box_mesh(dim, n, scheme, bcs, sizes, ptr(v), ptr(e), ptr(bf), ptr(bb)))
         return Mesh(v, e, bf, bb)
 
+    @staticmethod
+    def union(meshes):
+        """Disjoint union of meshes (vertex ids offset, elements kept in order): one operator over it
+        holds every mesh; returns the mesh and the element offsets (len(meshes) + 1)."""
+        assert len({m.dim for m in meshes}) == 1
+        voff = np.cumsum([0] + [m.vertices.shape[0] for m in meshes])
+        eoff = np.cumsum([0] + [m.num_elements for m in meshes])
+        v = np.concatenate([m.vertices for m in meshes])
+        e = np.concatenate([m.elements + voff[i] for i, m in enumerate(meshes)])
+        bf = np.concatenate([m.boundary_facets + voff[i] for i, m in enumerate(meshes)])
+        bb = np.concatenate([m.boundary_bc for m in meshes])
+        return Mesh(v, e, bf, bb), eoff
+
     def affine_nodes(self, degree):
         """Warp-and-blend geometry nodes of the affine map, (elements, nbf, dim)."""
         ref = reference_nodes(self.dim, degree)
@@ -222,6 +235,34 @@ class DGOperator:
         out = {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
         out["column_iterations"] = its
         out["column_rel_residual"] = rel
+        return out
+
+    def autotune(self, ncols=1, precision="fp64"):
+        """Time the applicable kernel variants on ncols columns, keep the fastest; returns the chosen
+        variant name and the milliseconds per apply of each."""
+        chosen = C.c_int()
+        ms = np.zeros(3)
+        check(lib().tdg_autotune(self._h, ncols, 8 if precision == "fp64" else 4, C.byref(chosen), ms))
+        names = ["mfma", "mfma-xcd", "points"]
+        return names[chosen.value], {names[v]: float(ms[v]) for v in range(3) if ms[v] >= 0}
+
+    def solve_segments(self, b, x, offsets, rtol=1e-10, mode="fp64", max_iter=200000, check_every=10,
+                       reliable_delta=0.1, precond=None):
+        """Independent meshes in one operator: solve each segment [offsets[c], offsets[c+1]) of the
+        float64 GPU vectors b, x as its own system, all in lockstep (one apply per iteration)."""
+        assert b.is_cuda and x.is_cuda and b.shape == x.shape and b.numel() == self.n
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        ns = len(off) - 1
+        opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ru": 2}[mode], 1e-4, 1, check_every, reliable_delta,
+                              {None: 0, "block-jacobi": 1}[precond], 0)
+        res = _lib.SolveResult()
+        its = np.zeros(ns, dtype=np.int32)
+        rel = np.zeros(ns)
+        check(lib().tdg_solve_segments(self._h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), ns, off,
+                                       C.byref(opts), C.byref(res), its, rel))
+        out = {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
+        out["segment_iterations"] = its
+        out["segment_rel_residual"] = rel
         return out
 
     def rhs(self, force_Q=None, facet_q=None):

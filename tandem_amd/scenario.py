@@ -250,3 +250,22 @@ class FaultRectangle:
         out[np.arange(len(cols)), faults[fi]] = (sgn[fi][:, None, None] * L[fi, ni][:, None, :] *
                                                  self.tangent[None, :, None])
         return out
+
+    def uniform_slip(self, op):
+        """Facet data (F, 2, nq) of a unit along-fault slip on every fault facet (the sum of all slip
+        basis functions), with the sign convention of slip_basis."""
+        self.slip_basis(op, [0])
+        faults, _, sgn = self._slip
+        out = np.zeros((op.F, 2, op.nq))
+        out[faults] = sgn[:, None, None] * self.tangent[None, :, None]
+        return out
+
+    @staticmethod
+    def with_elements(n_elements, seed):
+        """A fault rectangle of the default geometry meshed with about n_elements triangles (spacing
+        from 2 W H / h^2 triangles; fault facets of about 0.55 h along the same 256-long fault)."""
+        h = float(np.sqrt(2.0 * 400.0 * 200.0 / n_elements))
+        fh = 0.55 * h
+        return FaultRectangle(h=h, nfault=max(4, int(round(256.0 / fh))), fault_h=256.0 / max(4, int(round(256.0 / fh))),
+                              seed=seed)
+

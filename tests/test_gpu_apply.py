@@ -96,3 +96,23 @@ def test_apply_is_deterministic(cuda):
     c.op.apply(x, y1)
     c.op.apply(x, y2)
     assert torch.equal(y1, y2)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_autotuned_variant_matches_oracle(cuda, precision):
+    """The autotuner times the applicable variants, keeps one, and the apply it leaves in place
+    still meets the parity bar."""
+    import torch
+
+    c = Case(3, 2, 1, 4, 0, seed=6)
+    name, ms = c.op.autotune(ncols=2, precision=precision)
+    print("autotune", precision, name, ms, flush=True)
+    assert name in ms and set(ms) == {"mfma", "mfma-xcd", "points"}
+    assert all(v > 0 for v in ms.values())
+    x = np.random.default_rng(7).uniform(-1, 1, c.n)
+    ref = c.oracle.apply(x)
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    xd = torch.from_numpy(x).to(cuda).to(dt)
+    y = torch.zeros_like(xd)
+    c.op.apply(xd, y)
+    assert rel_l2(y.double().cpu().numpy(), ref) <= (1e-12 if precision == "fp64" else 1e-5)

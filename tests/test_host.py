@@ -131,3 +131,40 @@ def test_interpolation_matrix_matches_oracle(D, to, frm):
     assert I.shape == ref.shape
     assert np.abs(I - ref).max() <= 1e-13
     assert np.abs(I.sum(axis=1) - 1).max() <= 1e-13
+
+
+def test_union_mesh_keeps_each_mesh_facets_in_order():
+    """An operator over the disjoint union of meshes numbers elements and facets mesh by mesh, so a
+    mesh's facet data and vector segment are contiguous slices."""
+    from tandem_amd import scenario as S
+    frs = [S.FaultRectangle.with_elements(n, 5 + i) for i, n in enumerate((300, 800, 500))]
+    meshes = [fr.mesh() for fr in frs]
+    union, eoff = T.Mesh.union(meshes)
+    op = T.DGOperator(union, T.ELASTICITY, 2)
+    parts = [T.DGOperator(m, T.ELASTICITY, 2).facets() for m in meshes]
+    ref = []
+    for i, f in enumerate(parts):
+        g = f.copy()
+        g[:, 0:2] += eoff[i]
+        ref.append(g)
+    assert np.array_equal(op.facets(), np.concatenate(ref))
+    assert list(eoff) == list(np.cumsum([0] + [m.num_elements for m in meshes]))
+
+
+def test_segmented_solve_rejects_coupled_segments():
+    """Segments must be whole elements of meshes with no facet between them (host-side checks run
+    before any device work)."""
+    m = T.Mesh.box(2, 4, 0, T.BC_DIRICHLET)
+    op = T.DGOperator(m, T.ELASTICITY, 2)
+    op.set_material(np.full((op.E, op.Nq), 2.0), np.full((op.E, op.Nq), 1.0))
+    half = (op.E // 2) * op.bs
+    with pytest.raises(L.TdgError, match="couples two segments"):
+        L.check(L.lib().tdg_solve_segments(op._h, C.c_void_p(8), C.c_void_p(16), 2,
+                                           np.array([0, half, op.n], dtype=np.int64),
+                                           C.byref(L.SolveOpts(1e-10, 10, 0, 1e-4, 1, 10, 0.1, 0, 0)),
+                                           C.byref(L.SolveResult()), np.zeros(2, dtype=np.int32), np.zeros(2)))
+    with pytest.raises(L.TdgError, match="whole elements"):
+        L.check(L.lib().tdg_solve_segments(op._h, C.c_void_p(8), C.c_void_p(16), 2,
+                                           np.array([0, half + 1, op.n], dtype=np.int64),
+                                           C.byref(L.SolveOpts(1e-10, 10, 0, 1e-4, 1, 10, 0.1, 0, 0)),
+                                           C.byref(L.SolveResult()), np.zeros(2, dtype=np.int32), np.zeros(2)))
