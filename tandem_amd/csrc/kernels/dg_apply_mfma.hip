@@ -63,7 +63,13 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
     constexpr int NCH = L::NCH, NFC = L::NFC;
     constexpr int nV = D * Nq * Nbf, nT = NF * nq * Nbf;
 
-    __shared__ T sV[nV];
+    // FP32 stages the 16 neighbour vectors of each face in LDS, in the volume table's space once the
+    // volume phase is done (contiguous loads; the FP32 kernel has no registers to hold them); FP64
+    // keeps them in registers, gathered per lane
+    constexpr bool STAGE = sizeof(T) == 4;
+    constexpr int nVN = STAGE && kWaves * 16 * BS > nV ? kWaves * 16 * BS : nV;
+    __shared__ T sVN[nVN];
+    T* const sV = sVN;
     __shared__ T sT[nT];
     __shared__ int sP[2 * NF * Nbf];
     __shared__ T sU[kWaves][16 * BS];
@@ -211,7 +217,10 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
 #pragma unroll 1
         for (int b = 0; b < QB - 1; ++b) volume_block(b, std::false_type{});
         volume_block(QB - 1, std::true_type{});
-
+    }
+    if constexpr (STAGE) __syncthreads(); // every read of the volume table is complete
+    T* const NbU = sVN + (STAGE ? wv * 16 * BS : 0); // this wave's 16 neighbour vectors (FP32)
+    if (live) {
         // -------------------------------------------------------------- facets
         // trace A operand row rho = j: facet point 4 t + mfma_group(rho), channel mfma_reg(rho)
         const int tq = mfma_group<T>(j);
@@ -223,27 +232,71 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
             const int nbface = (info.x >> 4) & 3;
             const int64_t nb = kind == 1 ? int64_t(info.y) : 0;
             const T pen = a.FP[(gj * NF + f) * 16 + j];
-            // own B operand offsets in the face-f frame; neighbour vector in its face frame
-            int po[KS];
-            T Nb[KS][Q];
+            // own and neighbour B operand offsets in the face frames (FP32: packed in one register,
+            // neighbour vectors staged in LDS; FP64: neighbour values gathered into registers)
+            int pp[KS];
+            T Nb[STAGE ? 1 : KS][Q];
+            if constexpr (STAGE) {
+                // element jj's neighbour index comes from lane jj; lanes without one read element 0
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier(); // reads of the previous face's neighbours are done
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int nb32 = int(nb);
+#pragma unroll
+                for (int d0 = 0; d0 < 16 * BS; d0 += 64) {
+                    const int d = d0 + lane;
+                    const int jj = min(d / BS, 15);
+                    const uint32_t off = uint32_t(__shfl(nb32, jj, 64)) * uint32_t(BS) + uint32_t(d - jj * BS);
+                    if (d < 16 * BS) NbU[d] = x[off];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 const int k = 4 * s + g;
                 const int kc = min(k, Nbf - 1);
-                po[s] = sP[f * Nbf + kc];
                 const int pk = sP[nbface * Nbf + kc];
+                if constexpr (STAGE) {
+                    pp[s] = sP[f * Nbf + kc] | (pk << 16);
+                } else {
+                    pp[s] = sP[f * Nbf + kc];
 #pragma unroll
-                for (int r = 0; r < Q; ++r) {
-                    const T v = x[nb * BS + pk + r * Nbf];
-                    Nb[s][r] = (4 * s + 3 < Nbf || k < Nbf) ? v : T(0);
+                    for (int r = 0; r < Q; ++r) {
+                        const T v = x[nb * BS + pk + r * Nbf];
+                        Nb[s][r] = (4 * s + 3 < Nbf || k < Nbf) ? v : T(0);
+                    }
                 }
             }
+            T const* const Nj = NbU + j * BS;
             // lift A operand columns: face-0 label of row k = 16 m + j
             int il[MT];
 #pragma unroll
             for (int m = 0; m < MT; ++m) il[m] = sP[NF * Nbf + f * Nbf + min(16 * m + j, Nbf - 1)];
             int const* const iperm_f = sP + NF * Nbf + f * Nbf; // face-0 labels (tail rows read at use)
             T const* const fdf = a.FD + ((gj * NF + f) * nq) * NFC * 16 + j;
+            // lift of tile t's point weights: Y(k) += sum_(q,c) T0(c, iperm_f(k), q) W(q, c)
+            auto lift = [&](int t, T const (&W)[D + 1][Q]) {
+                const int qc = min(4 * t + g, nq - 1);
+#pragma unroll
+                for (int c = 0; c <= D; ++c) {
+#pragma unroll
+                    for (int m = 0; m < MM; ++m) {
+                        const T av = sT[(c * nq + qc) * Nbf + il[m]];
+#pragma unroll
+                        for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, W[c][u], Y[m][u]);
+                    }
+                    if constexpr (VTAIL) {
+#pragma unroll
+                        for (int kt = 0; kt < NT; ++kt) {
+                            const T a = sT[(c * nq + qc) * Nbf + iperm_f[min(16 * MM + kt, Nbf - 1)]];
+#pragma unroll
+                            for (int u = 0; u < Q; ++u) Yt[kt][u] += a * W[c][u];
+                        }
+                    }
+                }
+            };
 #pragma unroll
             for (int t = 0; t < FT; ++t) {
                 const int q = 4 * t + g;
@@ -263,10 +316,17 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
                     const T av = sT[ab + min(k, Nbf - 1)];
 #pragma unroll
                     for (int r = 0; r < Q; ++r) {
-                        const T v = Uj[po[s] + r * Nbf];
-                        const T bo = (4 * s + 3 < Nbf || k < Nbf) ? v : T(0);
-                        Co[r] = Mma<T>::mma(av, bo, Co[r]);
-                        Cn[r] = Mma<T>::mma(av, Nb[s][r], Cn[r]);
+                        const bool kin = 4 * s + 3 < Nbf || k < Nbf;
+                        if constexpr (STAGE) {
+                            const T v = Uj[(pp[s] & 0xffff) + r * Nbf];
+                            const T vn = Nj[(pp[s] >> 16) + r * Nbf];
+                            Co[r] = Mma<T>::mma(av, kin ? v : T(0), Co[r]);
+                            Cn[r] = Mma<T>::mma(av, kin ? vn : T(0), Cn[r]);
+                        } else {
+                            const T v = Uj[pp[s] + r * Nbf];
+                            Co[r] = Mma<T>::mma(av, kin ? v : T(0), Co[r]);
+                            Cn[r] = Mma<T>::mma(av, Nb[s][r], Cn[r]);
+                        }
                     }
                 }
                 // this lane: facet point q; register c = value (0) / d/dxi_e (1 + e)
@@ -342,23 +402,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
                         }
                         W[1 + e][u] = act ? wq * v : T(0);
                     }
-#pragma unroll
-                for (int c = 0; c <= D; ++c) {
-#pragma unroll
-                    for (int m = 0; m < MM; ++m) {
-                        const T av = sT[(c * nq + qc) * Nbf + il[m]];
-#pragma unroll
-                        for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, W[c][u], Y[m][u]);
-                    }
-                    if constexpr (VTAIL) {
-#pragma unroll
-                        for (int kt = 0; kt < NT; ++kt) {
-                            const T a = sT[(c * nq + qc) * Nbf + iperm_f[min(16 * MM + kt, Nbf - 1)]];
-#pragma unroll
-                            for (int u = 0; u < Q; ++u) Yt[kt][u] += a * W[c][u];
-                        }
-                    }
-                }
+                lift(t, W);
             }
         }
     }

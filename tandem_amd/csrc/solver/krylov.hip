@@ -7,6 +7,7 @@
 // `check_every` iterations only.  Reductions are two-stage with a fixed grid, so results are
 // bitwise reproducible run to run.
 #include "krylov.h"
+#include "replay.h"
 
 #include "../host/operator.h"
 #include "block_jacobi.h"
@@ -121,45 +122,6 @@ template <class T> struct Buf {
         if (p) (void)hipFree(p);
     }
     void alloc(std::size_t n) { KHIP(hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T))); }
-};
-
-// Replays a fixed sequence of launches: captured once into a HIP graph on a non-default stream
-// (the solver iterations between two host checks are the same launches with the same arguments),
-// or launched directly when graphs are off or the stream is the default stream, which cannot be
-// captured.  The sequence must not allocate or memset.
-class Replay {
-public:
-    Replay(hipStream_t s, bool enable) : s_(s), on_(enable && s != nullptr) {}
-    ~Replay() {
-        if (exec_) (void)hipGraphExecDestroy(exec_);
-        if (graph_) (void)hipGraphDestroy(graph_);
-    }
-    template <class F> void run(F&& launches) {
-        if (!on_) {
-            launches();
-            return;
-        }
-        if (!exec_) {
-            KHIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
-            try {
-                launches();
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(s_, &g);
-                if (g) (void)hipGraphDestroy(g);
-                throw;
-            }
-            KHIP(hipStreamEndCapture(s_, &graph_));
-            KHIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
-        }
-        KHIP(hipGraphLaunch(exec_, s_));
-    }
-
-private:
-    hipStream_t s_;
-    bool on_;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t exec_ = nullptr;
 };
 
 double read_scalar(double const* d, hipStream_t s) {

@@ -15,6 +15,7 @@
 
 #include "../host/operator.h"
 #include "block_jacobi.h"
+#include "replay.h"
 
 #include <hip/hip_runtime.h>
 
@@ -247,17 +248,31 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         prec(0);
         hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, p.p, M ? z.p : r.p, L);
         MHIP(hipGetLastError());
-        int it = 0, cur = 0;
-        while (it < o.max_iter) {
+        auto iteration = [&](int c) {
             applyA(p.p, q.p);
             hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, p.p, q.p, L, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 2, -1);
-            hipLaunchKernelGGL((mxr<double, double>), g, bl, 0, st, X, r.p, p.p, q.p, L, S.p, cur, partial.p);
+            hipLaunchKernelGGL((mxr<double, double>), g, bl, 0, st, X, r.p, p.p, q.p, L, S.p, c, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
-            prec(1 - cur);
-            hipLaunchKernelGGL((mp<double>), g, bl, 0, st, p.p, M ? z.p : r.p, L, S.p, 1 - cur, cur);
-            cur = 1 - cur;
-            ++it;
+            prec(1 - c);
+            hipLaunchKernelGGL((mp<double>), g, bl, 0, st, p.p, M ? z.p : r.p, L, S.p, 1 - c, c);
+        };
+        // the check_every iterations between host checks replayed from one captured graph (an even
+        // count keeps the rho slots in phase)
+        Replay chunk(st, o.graphs != 0 && o.check_every % 2 == 0);
+        int it = 0, cur = 0;
+        while (it < o.max_iter) {
+            if (it % o.check_every == 0 && it + o.check_every <= o.max_iter) {
+                chunk.run([&] {
+                    for (int k = 0; k < o.check_every; ++k) iteration((cur + k) & 1);
+                });
+                it += o.check_every;
+                cur = (cur + o.check_every) & 1;
+            } else {
+                iteration(cur);
+                cur = 1 - cur;
+                ++it;
+            }
             if (it % o.check_every == 0 || it == o.max_iter) {
                 MHIP(hipGetLastError());
                 pull();
@@ -314,13 +329,36 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         for (int c = 0; c < ncols; ++c) maxr[c] = std::sqrt(hs[c * kS + 4]);
         int it = 0, cur = 0;
         res.replacements = 0;
-        while (it < o.max_iter) {
+        // first half of an iteration: q = A p, step, residual update; second half: preconditioner
+        // and new direction (after the host check of the iteration, which may replace residuals)
+        auto half1 = [&](int c) {
             applyA(p.p, q.p);
             hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, p.p, q.p, L, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 2, -1);
-            hipLaunchKernelGGL((mxr<float, float>), g, bl, 0, st, xa.p, r32.p, p.p, q.p, L, S.p, cur, partial.p);
+            hipLaunchKernelGGL((mxr<float, float>), g, bl, 0, st, xa.p, r32.p, p.p, q.p, L, S.p, c, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
-            ++it;
+        };
+        auto half2 = [&](int c) {
+            prec(1 - c, -1);
+            hipLaunchKernelGGL((mp<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, L, S.p, 1 - c, c);
+        };
+        // between two checks: (half1 half2) x (check_every - 1), then half1, replayed from one
+        // captured graph; the check's half2 follows it directly (an even count keeps the phase)
+        Replay chunk(st, o.graphs != 0 && o.check_every % 2 == 0);
+        while (it < o.max_iter) {
+            if (it % o.check_every == 0 && it + o.check_every <= o.max_iter) {
+                chunk.run([&] {
+                    for (int k = 0; k < o.check_every; ++k) {
+                        half1((cur + k) & 1);
+                        if (k + 1 < o.check_every) half2((cur + k) & 1);
+                    }
+                });
+                cur = (cur + o.check_every - 1) & 1;
+                it += o.check_every;
+            } else {
+                half1(cur);
+                ++it;
+            }
             bool any_replace = false;
             int active = 0;
             if (it % o.check_every == 0 || it == o.max_iter) {
@@ -359,8 +397,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
             for (int c = 0; c < ncols; ++c)
                 if (hs[c * kS + 5] != 0.0) ++active;
             if ((any_replace || it % o.check_every == 0) && !active) break;
-            prec(1 - cur, -1);
-            hipLaunchKernelGGL((mp<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, L, S.p, 1 - cur, cur);
+            half2(cur);
             cur = 1 - cur;
         }
         res.total_iterations = it;

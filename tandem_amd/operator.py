@@ -218,7 +218,7 @@ class DGOperator:
         return {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
 
     def solve_multi(self, B, X, rtol=1e-10, mode="fp64", max_iter=200000, check_every=10, reliable_delta=0.1,
-                    precond=None):
+                    precond=None, graphs=True):
         """Solve A X[c] = B[c] for the rows c of (ncols, ld) float64 GPU tensors together (one batched
         apply per iteration).  mode "fp64" or "fp32-ru".  Returns the statistics with per-column
         iterations and relative residuals."""
@@ -226,7 +226,7 @@ class DGOperator:
         assert B.dtype == X.dtype and B.is_contiguous() and X.is_contiguous()
         nc, ld = B.shape
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ru": 2}[mode], 1e-4, 1, check_every, reliable_delta,
-                              {None: 0, "block-jacobi": 1}[precond], 0)
+                              {None: 0, "block-jacobi": 1}[precond], int(graphs))
         res = _lib.SolveResult()
         its = np.zeros(nc, dtype=np.int32)
         rel = np.zeros(nc)
@@ -247,14 +247,14 @@ class DGOperator:
         return names[chosen.value], {names[v]: float(ms[v]) for v in range(3) if ms[v] >= 0}
 
     def solve_segments(self, b, x, offsets, rtol=1e-10, mode="fp64", max_iter=200000, check_every=10,
-                       reliable_delta=0.1, precond=None):
+                       reliable_delta=0.1, precond=None, graphs=True):
         """Independent meshes in one operator: solve each segment [offsets[c], offsets[c+1]) of the
         float64 GPU vectors b, x as its own system, all in lockstep (one apply per iteration)."""
         assert b.is_cuda and x.is_cuda and b.shape == x.shape and b.numel() == self.n
         off = np.ascontiguousarray(offsets, dtype=np.int64)
         ns = len(off) - 1
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ru": 2}[mode], 1e-4, 1, check_every, reliable_delta,
-                              {None: 0, "block-jacobi": 1}[precond], 0)
+                              {None: 0, "block-jacobi": 1}[precond], int(graphs))
         res = _lib.SolveResult()
         its = np.zeros(ns, dtype=np.int32)
         rel = np.zeros(ns)

@@ -101,3 +101,51 @@ def test_graph_replayed_solve_equals_launched(cuda, mode, precond):
     assert out[True][1] == out[False][1]
     assert np.array_equal(out[True][0], out[False][0])
     c.op.set_stream(None)
+
+
+@pytest.mark.parametrize("mode,precond", [("fp64", None), ("fp32-ru", "block-jacobi")])
+def test_graph_replayed_multi_solve_equals_launched(cuda, mode, precond):
+    """Lockstep multi-column and segmented solves with their iteration chunks replayed from
+    captured HIP graphs give bit-identical iterates to plain launches, over several replays."""
+    import torch
+    import tandem_amd as T
+
+    c = Case(2, 3, 0, 4, 0, seed=7)
+    stream = torch.cuda.Stream()
+    c.op.set_stream(stream)
+    rng = np.random.default_rng(8)
+    B = torch.from_numpy(rng.uniform(-1, 1, (3, c.n + 5))).to(cuda)
+    out = {}
+    for graphs in (False, True):
+        X = torch.zeros_like(B)
+        with torch.cuda.stream(stream):
+            r = c.op.solve_multi(B, X, rtol=1e-10, mode=mode, precond=precond, graphs=graphs)
+        stream.synchronize()
+        print("multi", mode, precond, graphs, r["iterations"], r["column_iterations"], flush=True)
+        assert r["converged"]
+        out[graphs] = (X.cpu().numpy(), r["iterations"])
+    assert out[True][1] == out[False][1]
+    assert np.array_equal(out[True][0], out[False][0])
+    c.op.set_stream(None)
+    # segmented: three boxes in one operator
+    meshes = []
+    for i, n in enumerate((2, 4, 3)):
+        m = T.Mesh.box(2, n, 0, T.BC_DIRICHLET)
+        meshes.append(T.Mesh(m.vertices + np.array([2.0 * i, 0.0]), m.elements, m.boundary_facets, m.boundary_bc))
+    union, eoff = T.Mesh.union(meshes)
+    op = T.DGOperator(union, T.ELASTICITY, 4)
+    op.set_material(np.full((op.E, op.Nq), 2.0), np.full((op.E, op.Nq), 1.0))
+    op.set_stream(stream)
+    b = torch.from_numpy(rng.uniform(-1, 1, op.n)).to(cuda)
+    seg = {}
+    for graphs in (False, True):
+        x = torch.zeros_like(b)
+        with torch.cuda.stream(stream):
+            r = op.solve_segments(b, x, eoff * op.bs, rtol=1e-10, mode=mode, precond=precond, graphs=graphs)
+        stream.synchronize()
+        print("segments", mode, precond, graphs, r["iterations"], r["segment_iterations"], flush=True)
+        assert r["converged"]
+        seg[graphs] = (x.cpu().numpy(), r["iterations"])
+    assert seg[True][1] == seg[False][1]
+    assert np.array_equal(seg[True][0], seg[False][0])
+    op.set_stream(None)
