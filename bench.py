@@ -113,7 +113,7 @@ def cpu_baseline(cells=10, reps=10):
 
 
 def traffic_from_profile():
-    p = os.path.join(ROOT, "profiles", "elasticity_apply_p4_c1_traffic.json")
+    p = os.path.join(ROOT, "profiles", "elasticity_mfma_p4_c1_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -223,11 +223,13 @@ def run_precision(args):
 
 def run_greens(args):
     """Discrete Green's function of a 2D plane-strain fault problem (P6 triangles, ~200k elements,
-    512 fault facets x 7 slip basis functions = 3,584 right-hand sides): blocks of 64 slip
-    right-hand sides built on the GPU in one batched launch and solved together by lockstep CG
-    (one batched apply of 64 columns per iteration), FP32 MFMA + FP64 residual replacement,
-    block-Jacobi preconditioned, to 1e-10.  Times args.blocks blocks (a bounded sample of the 56)
-    and one single-column solve for comparison."""
+    512 fault facets x 7 slip basis functions = 3,584 right-hand sides): a block of 64 slip
+    right-hand sides built on the GPU in one batched launch and advanced together by lockstep CG
+    (one batched apply of 64 columns per iteration), FP32 MFMA + FP64 residual replacement, block
+    Jacobi, against one right-hand side alone with the same solver.  Both run a fixed budget of
+    args.iters iterations (block-Jacobi CG needs 10^4-10^5 iterations to reach 1e-10 on this mesh,
+    more than a bounded run affords), so the comparison is throughput: right-hand-side iterations
+    per second, and the residual each reached."""
     import torch
     import tandem_amd as T
     from tandem_amd import scenario as S
@@ -246,69 +248,63 @@ def run_greens(args):
     total_cols = int(np.sum(op.facets()[:, 4] == 3)) * (op.degree + 1)
     nb = 64
     ld = op.n
-    out = {"workload": "2D P6 plane strain, %d elements, %d DoF, %d slip right-hand sides in blocks of %d"
-                       % (op.E, op.n, total_cols, nb), "blocks_timed": args.blocks}
-    times, its = [], []
-    for blk in range(args.blocks):
-        cols = np.arange(blk * nb, min((blk + 1) * nb, total_cols))
-        fdat = torch.from_numpy(fr.slip_basis(op, cols).reshape(len(cols), -1)).cuda()
-        B = torch.empty((len(cols), ld), dtype=torch.float64, device="cuda")
-        Xs = torch.zeros_like(B)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        op.rhs_device(B, None, fdat)
-        r = op.solve_multi(B, Xs, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=100000)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t1
-        log("greens block", blk, "%.2fs" % dt, {k: r[k] for k in ("iterations", "outer_iterations", "rel_residual",
-                                                                   "converged")})
-        times.append(dt)
-        its.append(int(r["iterations"]))
-        del fdat, B, Xs
-    out["seconds_per_block"] = times
-    out["lockstep_iterations"] = its
-    out["rhs_per_second"] = nb * len(times) / sum(times)
-    out["projected_seconds_all_rhs"] = total_cols / out["rhs_per_second"]
-    # one right-hand side alone (same solver)
-    fdat = torch.from_numpy(fr.slip_basis(op, [0]).reshape(-1)).cuda()
-    b = torch.empty(op.n, dtype=torch.float64, device="cuda")
+    variant, ms = op.autotune(ncols=nb, precision="fp32")
+    log("greens: autotune (64 columns, FP32)", variant, ms)
+    out = {"workload": "2D P6 plane strain, %d elements, %d DoF, %d slip right-hand sides, blocks of %d"
+                       % (op.E, op.n, total_cols, nb),
+           "autotune_fp32_64col_ms": ms, "autotune_choice": variant, "iterations_budget": args.iters}
+    # batched apply vs column-by-column apply (FP32)
+    Xf = torch.randn((nb, ld), dtype=torch.float32, device="cuda")
+    Yf = torch.empty_like(Xf)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    op.apply_batched(Xf, Yf)
+    ev[0].record()
+    for _ in range(5):
+        op.apply_batched(Xf, Yf)
+    ev[1].record()
+    for _ in range(5):
+        for c in range(nb):
+            op.apply(Xf[c], Yf[c])
+    ev[2].record()
+    torch.cuda.synchronize()
+    t_b, t_s = ev[0].elapsed_time(ev[1]) / 5, ev[1].elapsed_time(ev[2]) / 5
+    out["apply_fp32_64col_batched_ms"] = t_b
+    out["apply_fp32_64col_one_by_one_ms"] = t_s
+    out["apply_batching_speedup"] = t_s / t_b
+    cols = np.arange(nb)
+    fdat = torch.from_numpy(fr.slip_basis(op, cols).reshape(nb, -1)).cuda()
+    B = torch.empty((nb, ld), dtype=torch.float64, device="cuda")
+    Xs = torch.zeros_like(B)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    op.rhs_device(B, None, fdat)
+    r = op.solve_multi(B, Xs, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=args.iters)
+    torch.cuda.synchronize()
+    t_lock = time.perf_counter() - t1
+    log("greens lockstep %.2fs" % t_lock, {k: r[k] for k in ("iterations", "rel_residual")})
+    out["lockstep_seconds"] = t_lock
+    out["lockstep_iterations"] = int(r["iterations"])
+    out["lockstep_worst_rel_residual"] = float(np.max(r["column_rel_residual"]))
+    out["lockstep_rhs_iterations_per_second"] = nb * r["iterations"] / t_lock
+    # one right-hand side alone, same solver and budget
+    b = B[0].clone()
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    op.rhs_device(b, None, fdat)
-    r = op.solve(b, x, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=100000)
+    r1 = op.solve(b, x, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=args.iters)
     torch.cuda.synchronize()
-    out["single_rhs_seconds"] = time.perf_counter() - t1
-    out["single_rhs_iterations"] = r["iterations"]
-    out["batching_speedup_per_rhs"] = out["single_rhs_seconds"] * out["rhs_per_second"]
+    t_one = time.perf_counter() - t1
+    log("greens single column %.2fs" % t_one, {k: r1[k] for k in ("iterations", "rel_residual")})
+    out["single_seconds"] = t_one
+    out["single_iterations"] = int(r1["iterations"])
+    out["single_rel_residual"] = float(r1["rel_residual"])
+    out["single_rhs_iterations_per_second"] = r1["iterations"] / t_one
+    out["batching_speedup_per_rhs_iteration"] = out["lockstep_rhs_iterations_per_second"] / out[
+        "single_rhs_iterations_per_second"]
+    xs = Xs[0].cpu().numpy()
+    out["column0_rel_diff_lockstep_vs_single"] = float(np.linalg.norm(xs - x.cpu().numpy()) / np.linalg.norm(xs))
     return out
-
-
-class SurfaceSampler:
-    """B: the displacement at a seeded subset of the facet quadrature points of the top face z = 1
-    (the sample points are facet quadrature points, so B is the facet trace of the element basis)."""
-
-    def __init__(self, op, count, seed):
-        fct = op.facets()
-        Xf = op.facet_points()
-        top = np.nonzero((fct[:, 0] == fct[:, 1]) & np.all(np.abs(Xf[:, :, 2] - 1.0) < 1e-12, axis=1))[0]
-        rng = np.random.default_rng(seed)
-        pick = rng.choice(top.size * op.nq, size=count, replace=False)
-        self.f, self.q = top[pick // op.nq], pick % op.nq
-        self.el, self.loc = fct[self.f, 0], fct[self.f, 2]
-        tabs = [op.host_array("E_q%d" % l).reshape(op.nq, -1) for l in range(op.dim + 1)]   # (nq, Nbf)
-        self.phi = np.stack([tabs[l][q] for l, q in zip(self.loc, self.q)])                # (count, Nbf)
-        self.op = op
-
-    def apply(self, x):
-        U = x.view(self.op.E, 3, -1).cpu().numpy()[self.el]                                  # (count, 3, Nbf)
-        return np.einsum("snk,sk->sn", U, self.phi)
-
-    def transpose(self, r):
-        import torch
-        B = np.zeros((self.op.E, 3, self.phi.shape[1]))
-        np.add.at(B, self.el, r[:, :, None] * self.phi[:, None, :])
-        return torch.from_numpy(B.reshape(-1)).cuda()
 
 
 def run_multimesh(args):
@@ -316,8 +312,9 @@ def run_multimesh(args):
     log-uniform in [1000, 64000] from a seed, each with a unit along-fault slip right-hand side:
     all meshes held in one operator (their disjoint union) and solved as 32 CG recursions in
     lockstep, one apply launch over every mesh's element groups per iteration (FP32 MFMA + FP64
-    residual replacement, block Jacobi, to 1e-10); compared with the same 32 solves one mesh at a
-    time with an operator per mesh."""
+    residual replacement, block Jacobi, towards 1e-10 within args.iters iterations, each segment
+    stopping on its own); compared with the same 32 solves one mesh at a time with an operator per
+    mesh and the same budget."""
     import torch
     import tandem_amd as T
     from tandem_amd import scenario as S
@@ -351,18 +348,20 @@ def run_multimesh(args):
     offsets = eoff * bs
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    r = op.solve_segments(b, x, offsets, rtol=1e-10, mode="fp32-ru", precond="block-jacobi")
+    r = op.solve_segments(b, x, offsets, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=args.iters)
     torch.cuda.synchronize()
     t_lock = time.perf_counter() - t1
     log("multimesh lockstep %.2fs" % t_lock, {k: r[k] for k in ("iterations", "rel_residual", "converged")})
     out = {"workload": "%d fault-rectangle meshes, 2D P6 plane strain, %d..%d elements (%d in all, %d DoF), "
                        "one unit-slip right-hand side each" % (len(frs), counts.min(), counts.max(), op.E, op.n),
+           "iterations_budget": args.iters,
            "element_counts": [int(m.num_elements) for m in meshes],
            "lockstep_seconds": t_lock, "lockstep_iterations": int(r["iterations"]),
-           "segment_iterations": [int(v) for v in r["segment_iterations"]],
+           "lockstep_seconds_per_iteration": t_lock / max(1, int(r["iterations"])),
+           "segments_converged": int(np.sum(r["segment_rel_residual"] <= 1e-10)),
            "worst_rel_residual": float(np.max(r["segment_rel_residual"]))}
     # the same solves one mesh at a time
-    seq, diffs = 0.0, []
+    seq, diffs, seq_its = 0.0, [], 0
     xs = x.cpu().numpy()
     for i, m in enumerate(meshes):
         sub = on_stream(T.DGOperator(m, T.ELASTICITY, 6))
@@ -375,14 +374,16 @@ def run_multimesh(args):
         xi = torch.zeros_like(bi)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        sub.solve(bi, xi, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=200000)
+        ri = sub.solve(bi, xi, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=args.iters)
         torch.cuda.synchronize()
         seq += time.perf_counter() - t2
+        seq_its += int(ri["iterations"])
         ref = xi.cpu().numpy()
         seg = xs[offsets[i]:offsets[i + 1]]
         diffs.append(float(np.linalg.norm(seg - ref) / max(np.linalg.norm(ref), 1e-300)))
         del sub, bi, xi
     out["one_at_a_time_seconds"] = seq
+    out["one_at_a_time_iterations"] = seq_its
     out["speedup_lockstep_vs_one_at_a_time"] = seq / t_lock
     out["max_solution_rel_diff_vs_one_at_a_time"] = max(diffs)
     return out
@@ -520,6 +521,7 @@ def main():
                          "adjoint: misfit gradient with finite-difference checks")
     ap.add_argument("--blocks", type=int, default=2, help="greens: number of 64-column blocks timed")
     ap.add_argument("--meshes", type=int, default=32, help="multimesh: number of independent meshes")
+    ap.add_argument("--iters", type=int, default=2000, help="greens / multimesh: iteration budget of the solves")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy kernel (y = -x over BYTES of float64) --steps times; "
                          "its FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the apply kernel")
@@ -543,6 +545,7 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": args.steps}))
         return
     if args.workload != "apply":
+        os.environ.setdefault("TDG_PROGRESS_SECONDS", "30")  # the long solves report progress
         run = {"precision": run_precision, "greens": run_greens, "adjoint": run_adjoint,
                "multimesh": run_multimesh}[args.workload]
         with torch.cuda.stream(torch.cuda.Stream()):

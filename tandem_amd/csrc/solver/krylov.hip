@@ -7,6 +7,7 @@
 // `check_every` iterations only.  Reductions are two-stage with a fixed grid, so results are
 // bitwise reproducible run to run.
 #include "krylov.h"
+#include "progress.h"
 #include "replay.h"
 
 #include "../host/operator.h"
@@ -172,6 +173,7 @@ int cg(Operator& op, BlockJacobi const* M, T const* rhs, TX* z, T* r, T* p, T* q
     // check_every iterations between host checks; an even count keeps the rho slots in phase, so
     // the same captured sequence is replayed for every chunk
     Replay chunk(st, graphs && check_every % 2 == 0);
+    Progress progress(sizeof(T) == 8 ? "FP64 CG" : "FP32 CG");
     int it = 0, cur = 0;
     while (it < max_it) {
         if (it + check_every <= max_it) {
@@ -190,6 +192,7 @@ int cg(Operator& op, BlockJacobi const* M, T const* rhs, TX* z, T* r, T* p, T* q
             rr = read_scalar(s + (M ? 4 : cur), st);
             if (!std::isfinite(rr)) throw std::runtime_error("CG breakdown (non-finite residual)");
             if (rr <= target_rr) break;
+            progress.tick(it, 1, std::sqrt(rr));
         }
     }
     *rr_out = read_scalar(s + (M ? 4 : cur), st);

@@ -15,10 +15,12 @@
 
 #include "../host/operator.h"
 #include "block_jacobi.h"
+#include "progress.h"
 #include "replay.h"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <stdexcept>
@@ -205,6 +207,14 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
     res.iterations.assign(ncols, 0);
     MHIP(hipStreamSynchronize(st));
     auto t0 = std::chrono::steady_clock::now();
+    Progress progress(segments ? "segmented CG" : "multi-column CG");
+    // largest relative recursive residual over the active systems
+    auto worst = [&] {
+        double w = 0.0;
+        for (int c = 0; c < ncols; ++c)
+            if (hs[c * kS + 5] != 0.0 && hs[c * kS + 3] > 0.0) w = std::max(w, std::sqrt(hs[c * kS + 4] / hs[c * kS + 3]));
+        return w;
+    };
     auto pull = [&] {
         MHIP(hipMemcpyAsync(hs.data(), S.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         MHIP(hipStreamSynchronize(st));
@@ -287,6 +297,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
                 }
                 push();
                 if (!active) break;
+                progress.tick(it, active, worst());
             }
         }
         res.total_iterations = it;
@@ -397,6 +408,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
             for (int c = 0; c < ncols; ++c)
                 if (hs[c * kS + 5] != 0.0) ++active;
             if ((any_replace || it % o.check_every == 0) && !active) break;
+            if (it % o.check_every == 0) progress.tick(it, active, worst());
             half2(cur);
             cur = 1 - cur;
         }
