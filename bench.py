@@ -167,7 +167,7 @@ def _boundary_facets(op):
 def run_precision(args):
     """FP32 MFMA + FP64 residual replacement vs all-FP64 solve on the P4 tet mesh at cells^3 (the
     precision study: 40^3 = 384,000 elements by default), manufactured cosine solution with the
-    seeded variable Lame fields, block-Jacobi preconditioned CG to 1e-10.  Errors of the FP32-MFMA
+    seeded variable Lame fields, CG to 1e-10 (block Jacobi, or none with --precond none).  Errors of the FP32-MFMA
     solution against the FP64 solution (whose apply matches the CPU FP64 oracle to 1e-12 in the
     parity tests) for the solution and for the traction on the boundary faces, and both solutions'
     errors against the manufactured solution."""
@@ -188,16 +188,19 @@ def run_precision(args):
     op.rhs_device(b, force, gdat)
     del force
     log("precision: setup %.1fs, %d elements, %d DoF" % (time.perf_counter() - t0, op.E, op.n))
-    t1 = time.perf_counter()
-    ncol = op.block_jacobi_build()
-    torch.cuda.synchronize()
-    t_pc = time.perf_counter() - t1
+    pc = None if args.precond == "none" else "block-jacobi"
     out = {"workload": "P4 tet elasticity, %d^3 Kuhn cubes x 6 = %d elements, %d DoF" % (args.cells, op.E, op.n),
-           "block_jacobi_setup_s": t_pc, "colours": ncol}
+           "preconditioner": args.precond}
+    if pc:
+        t1 = time.perf_counter()
+        out["colours"] = op.block_jacobi_build()
+        torch.cuda.synchronize()
+        out["block_jacobi_setup_s"] = time.perf_counter() - t1
     xs = {}
     for mode in ("fp64", "fp32-ru"):
         x = torch.zeros_like(b)
-        r = op.solve(b, x, rtol=1e-10, mode=mode, precond="block-jacobi", max_iter=100000)
+        torch.cuda.synchronize()
+        r = op.solve(b, x, rtol=1e-10, mode=mode, precond=pc, max_iter=100000)
         log("precision solve", mode, r)
         out[mode] = {k: r[k] for k in ("seconds", "iterations", "outer_iterations", "rel_residual", "converged")}
         xs[mode] = x
@@ -305,6 +308,33 @@ def run_greens(args):
     xs = Xs[0].cpu().numpy()
     out["column0_rel_diff_lockstep_vs_single"] = float(np.linalg.norm(xs - x.cpu().numpy()) / np.linalg.norm(xs))
     return out
+
+
+class SurfaceSampler:
+    """B: the displacement at a seeded subset of the facet quadrature points of the top face z = 1
+    (the sample points are facet quadrature points, so B is the facet trace of the element basis)."""
+
+    def __init__(self, op, count, seed):
+        fct = op.facets()
+        Xf = op.facet_points()
+        top = np.nonzero((fct[:, 0] == fct[:, 1]) & np.all(np.abs(Xf[:, :, 2] - 1.0) < 1e-12, axis=1))[0]
+        rng = np.random.default_rng(seed)
+        pick = rng.choice(top.size * op.nq, size=count, replace=False)
+        self.f, self.q = top[pick // op.nq], pick % op.nq
+        self.el, self.loc = fct[self.f, 0], fct[self.f, 2]
+        tabs = [op.host_array("E_q%d" % l).reshape(op.nq, -1) for l in range(op.dim + 1)]   # (nq, Nbf)
+        self.phi = np.stack([tabs[l][q] for l, q in zip(self.loc, self.q)])                # (count, Nbf)
+        self.op = op
+
+    def apply(self, x):
+        U = x.view(self.op.E, 3, -1).cpu().numpy()[self.el]                                  # (count, 3, Nbf)
+        return np.einsum("snk,sk->sn", U, self.phi)
+
+    def transpose(self, r):
+        import torch
+        B = np.zeros((self.op.E, 3, self.phi.shape[1]))
+        np.add.at(B, self.el, r[:, :, None] * self.phi[:, None, :])
+        return torch.from_numpy(B.reshape(-1)).cuda()
 
 
 def run_multimesh(args):
@@ -521,6 +551,8 @@ def main():
                          "adjoint: misfit gradient with finite-difference checks")
     ap.add_argument("--blocks", type=int, default=2, help="greens: number of 64-column blocks timed")
     ap.add_argument("--meshes", type=int, default=32, help="multimesh: number of independent meshes")
+    ap.add_argument("--precond", choices=["block-jacobi", "none"], default="block-jacobi",
+                    help="precision: preconditioner of both solves")
     ap.add_argument("--iters", type=int, default=2000, help="greens / multimesh: iteration budget of the solves")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy kernel (y = -x over BYTES of float64) --steps times; "

@@ -86,14 +86,31 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
     const int64_t e0 = grp * 16;
     const int nel = live ? int(min(int64_t(16), a.num_elements - e0)) : 0;
 
-    for (int i = tid; i < nV; i += 64 * kWaves) sV[i] = a.V[i];
-    for (int i = tid; i < nT; i += 64 * kWaves) sT[i] = a.T0[i];
-    for (int i = tid; i < 2 * NF * Nbf; i += 64 * kWaves) sP[i] = a.perm[i];
+    // unrolled copies: every load of the preamble is in flight at once (a rolled loop waits for each)
+#pragma unroll
+    for (int i0 = 0; i0 < nV; i0 += 64 * kWaves) {
+        const int i = i0 + tid;
+        if (i < nV) sV[i] = a.V[i];
+    }
+#pragma unroll
+    for (int i0 = 0; i0 < nT; i0 += 64 * kWaves) {
+        const int i = i0 + tid;
+        if (i < nT) sT[i] = a.T0[i];
+    }
+#pragma unroll
+    for (int i0 = 0; i0 < 2 * NF * Nbf; i0 += 64 * kWaves) {
+        const int i = i0 + tid;
+        if (i < 2 * NF * Nbf) sP[i] = a.perm[i];
+    }
     T* const U = sU[wv];
     {
         T const* src = x + e0 * BS;
         const int n = nel * BS;
-        for (int i = lane; i < 16 * BS; i += 64) U[i] = i < n ? src[i] : T(0);
+#pragma unroll
+        for (int i0 = 0; i0 < 16 * BS; i0 += 64) {
+            const int i = i0 + lane;
+            if (i < 16 * BS) U[i] = i < n ? src[i] : T(0);
+        }
     }
     __syncthreads();
 
@@ -445,7 +462,11 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
     if (live) {
         T* dst = y + e0 * BS;
         const int n = nel * BS;
-        for (int i = lane; i < n; i += 64) dst[i] = U[i];
+#pragma unroll
+        for (int i0 = 0; i0 < 16 * BS; i0 += 64) {
+            const int i = i0 + lane;
+            if (i < n) dst[i] = U[i];
+        }
     }
 }
 
