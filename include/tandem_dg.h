@@ -185,10 +185,12 @@ int tdg_solve_segments(tdg_op* op, double const* b, double* x, int nseg, int64_t
  * to the element, Elasticity.cpp:317-519, Poisson.cpp:225-381).  tdg_block_jacobi_build builds it
  * (also done by the first preconditioned solve) and returns the number of colours used;
  * tdg_block_jacobi_apply computes z = D^-1 r on device FP64 vectors (asynchronous on the operator's
- * stream); tdg_block_jacobi_inverse copies the FP64 inverse block of one element (bs x bs,
- * column-major) to the host. */
+ * stream), tdg_block_jacobi_apply_batched the same on ncols columns at stride ld (each block read
+ * once for all columns); tdg_block_jacobi_inverse copies the FP64 inverse block of one element
+ * (bs x bs, column-major) to the host. */
 int tdg_block_jacobi_build(tdg_op* op, int* colours);
 int tdg_block_jacobi_apply(tdg_op* op, double const* r, double* z);
+int tdg_block_jacobi_apply_batched(tdg_op* op, double const* R, double* Z, int ncols, int64_t ld);
 int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block);
 
 /* Adjoint gradient kernel (elasticity): g_lam, g_mu (device, elements x Nq) = d/d(lam_Q, mu_Q) of

@@ -338,6 +338,14 @@ int tdg_block_jacobi_apply(tdg_op* op, double const* r, double* z) {
     });
 }
 
+int tdg_block_jacobi_apply_batched(tdg_op* op, double const* R, double* Z, int ncols, int64_t ld) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(R && Z && R != Z && ncols >= 1 && ld >= int64_t(o.local_size()), "bad block Jacobi arguments");
+        o.block_jacobi().apply(R, Z, o.stream(), ncols, ld);
+    });
+}
+
 int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block) {
     return guarded([&] {
         auto& o = ready(op);

@@ -54,8 +54,7 @@ constexpr int kWaves = 4;
 // FP32: two waves per SIMD (the LDS footprint allows two blocks per CU); FP64: one.
 template <int D, int N, class T>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
-void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
-                                                                     T* __restrict__ ys, int64_t ld) {
+void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restrict__ ys, int64_t ld, int ncols) {
     using L = MfmaLayout<D, N>;
     using acc = typename Mma<T>::acc;
     constexpr int Nbf = L::Nbf, Nq = L::Nq, nq = L::nq, Q = D, BS = Nbf * Q, DD = D * D;
@@ -75,14 +74,19 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
     __shared__ T sU[kWaves][16 * BS];
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, j = lane & 15, g = lane >> 4;
-    // Workgroups are dealt round-robin to the 8 XCDs; renumber them so that each XCD works through
-    // one contiguous range of element groups and neighbouring elements share its L2.  The grid
-    // is a multiple of 8 blocks.
-    const unsigned lb = a.xcd_remap ? (blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u : blockIdx.x;
+    // Workgroups are dealt round-robin to the 8 XCDs (workgroup w to XCD w % 8).  Several columns:
+    // workgroup w = 8 (ncols r + c) + x takes column c of block 8 r + x, so all columns of a block
+    // run on one XCD one after another and its point and facet data are read from HBM once into
+    // that XCD's L2 instead of once per column.  One column with the XCD variant: each XCD works
+    // through one contiguous range of blocks.  The grid is a multiple of 8 workgroups.
+    const unsigned w = blockIdx.x, xcd = w % 8u, r = w / 8u;
+    const unsigned col = ncols > 1 ? r % unsigned(ncols) : 0u;
+    const unsigned lb = ncols > 1 ? (r / unsigned(ncols)) * 8u + xcd
+                                  : (a.xcd_remap ? xcd * (gridDim.x / 8u) + r : w);
     const int64_t grp = int64_t(lb) * kWaves + wv;
     const bool live = grp < a.num_groups; // uniform per wave
-    T const* __restrict__ x = xs + int64_t(blockIdx.y) * ld;
-    T* __restrict__ y = ys + int64_t(blockIdx.y) * ld;
+    T const* __restrict__ x = xs + int64_t(col) * ld;
+    T* __restrict__ y = ys + int64_t(col) * ld;
     const int64_t e0 = grp * 16;
     const int nel = live ? int(min(int64_t(16), a.num_elements - e0)) : 0;
 
@@ -473,8 +477,10 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs,
 template <int D, int N, class T>
 hipError_t run(MfmaArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t s) {
     const int64_t blocks = (a.num_groups + kWaves - 1) / kWaves;
-    dim3 grid(unsigned((blocks + 7) / 8 * 8), unsigned(ncols));
-    hipLaunchKernelGGL((elasticity_mfma_kernel<D, N, T>), grid, dim3(64 * kWaves), 0, s, a, x, y, ld);
+    const int64_t total = (blocks + 7) / 8 * 8 * int64_t(ncols);
+    if (total > int64_t(0x7fffffff)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((elasticity_mfma_kernel<D, N, T>), dim3(unsigned(total)), dim3(64 * kWaves), 0, s, a, x, y, ld,
+                       ncols);
     return hipGetLastError();
 }
 

@@ -95,6 +95,50 @@ __global__ __launch_bounds__(128) void apply_blocks(T const* __restrict__ inv, T
     z[e * bs + i] = acc;
 }
 
+// Several columns: one workgroup per element holds the element's inverse block in LDS and applies
+// it to every column (columns at stride ld), so a block is read from HBM once per application
+// instead of once per column.  Each lane computes four columns of one row.
+constexpr int kCols = 16; // columns staged per pass
+template <class T, int BSMAX>
+__global__ __launch_bounds__(256) void apply_blocks_cols(T const* __restrict__ inv, T const* __restrict__ rs,
+                                                         T* __restrict__ zs, int bs, int ncols, int64_t ld) {
+    __shared__ T B[BSMAX * BSMAX];
+    __shared__ T R[kCols * BSMAX];
+    const int64_t e = blockIdx.x;
+    T const* Bg = inv + e * int64_t(bs) * bs;
+    const int nn = bs * bs;
+    for (int t = threadIdx.x; t < nn; t += 256) B[t] = Bg[t];
+    for (int c0 = 0; c0 < ncols; c0 += kCols) {
+        const int nc = min(kCols, ncols - c0);
+        __syncthreads(); // B is loaded / the previous pass's reads of R are done
+        for (int t = threadIdx.x; t < nc * bs; t += 256) {
+            const int c = t / bs, i = t % bs;
+            R[c * bs + i] = rs[int64_t(c0 + c) * ld + e * bs + i];
+        }
+        __syncthreads();
+        // outputs (row i, columns 4q..4q+3) of this pass
+        const int nq4 = (nc + 3) / 4;
+        for (int t = threadIdx.x; t < bs * nq4; t += 256) {
+            const int i = t % bs, q = t / bs;
+            T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+            T const* r0 = R + (4 * q) * bs;
+            for (int jj = 0; jj < bs; ++jj) {
+                const T b = B[i + jj * bs];
+                a0 += b * r0[jj];
+                a1 += b * r0[jj + bs];
+                a2 += b * r0[jj + 2 * bs];
+                a3 += b * r0[jj + 3 * bs];
+            }
+            const int c = c0 + 4 * q;
+            T* z = zs + e * bs + i;
+            z[int64_t(c) * ld] = a0;
+            if (4 * q + 1 < nc) z[int64_t(c + 1) * ld] = a1;
+            if (4 * q + 2 < nc) z[int64_t(c + 2) * ld] = a2;
+            if (4 * q + 3 < nc) z[int64_t(c + 3) * ld] = a3;
+        }
+    }
+}
+
 unsigned grid_for(int64_t items) { return unsigned(std::min<int64_t>((items + kThreads - 1) / kThreads, 65536)); }
 
 } // namespace
@@ -185,18 +229,28 @@ void BlockJacobi::build(Operator& op) {
     colours_ = nc;
 }
 
+namespace {
+template <class T> void launch_apply(T const* inv, T const* r, T* z, int64_t E, int bs, int ncols, int64_t ld,
+                                     hipStream_t s) {
+    if (ncols <= 1) {
+        hipLaunchKernelGGL((apply_blocks<T>), dim3(unsigned(E), 1u), dim3(128), 0, s, inv, r, z, bs, ld);
+    } else if (bs <= 64) {
+        hipLaunchKernelGGL((apply_blocks_cols<T, 64>), dim3(unsigned(E)), dim3(256), 0, s, inv, r, z, bs, ncols, ld);
+    } else {
+        hipLaunchKernelGGL((apply_blocks_cols<T, 112>), dim3(unsigned(E)), dim3(256), 0, s, inv, r, z, bs, ncols, ld);
+    }
+    BHIP(hipGetLastError());
+}
+} // namespace
+
 void BlockJacobi::apply(double const* r, double* z, hipStream_t s, int ncols, int64_t ld) const {
     if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
-    hipLaunchKernelGGL((apply_blocks<double>), dim3(unsigned(E_), unsigned(ncols)), dim3(128), 0, s, inv64_, r, z, bs_,
-                       ld);
-    BHIP(hipGetLastError());
+    launch_apply(inv64_, r, z, E_, bs_, ncols, ld, s);
 }
 
 void BlockJacobi::apply(float const* r, float* z, hipStream_t s, int ncols, int64_t ld) const {
     if (!inv32_) throw std::logic_error("block Jacobi preconditioner not built");
-    hipLaunchKernelGGL((apply_blocks<float>), dim3(unsigned(E_), unsigned(ncols)), dim3(128), 0, s, inv32_, r, z, bs_,
-                       ld);
-    BHIP(hipGetLastError());
+    launch_apply(inv32_, r, z, E_, bs_, ncols, ld, s);
 }
 
 void BlockJacobi::inverse_block(int64_t e, double* out) const {

@@ -316,6 +316,13 @@ class DGOperator:
         check(lib().tdg_block_jacobi_apply(self._h, C.c_void_p(r.data_ptr()), C.c_void_p(z.data_ptr())))
         return z
 
+    def block_jacobi_apply_batched(self, R, Z):
+        """Z[c] = D^-1 R[c] for the rows c of (ncols, ld) float64 GPU tensors."""
+        assert R.shape == Z.shape and R.dim() == 2 and R.shape[1] >= self.n and R.is_contiguous() and Z.is_contiguous()
+        check(lib().tdg_block_jacobi_apply_batched(self._h, C.c_void_p(R.data_ptr()), C.c_void_p(Z.data_ptr()),
+                                                   R.shape[0], R.shape[1]))
+        return Z
+
     def block_jacobi_inverse(self, element):
         out = np.zeros(self.bs * self.bs)
         check(lib().tdg_block_jacobi_inverse(self._h, element, out))

@@ -45,6 +45,17 @@ def test_block_inverses_match_assembled_diagonal(cuda, case):
     err = rel_l2(z.cpu().numpy(), ref)
     print("  D^-1 r rel L2 %.3e" % err, flush=True)
     assert err <= 1e-10
+    # several columns at a stride: each equals the single-column application (19 columns: a ragged
+    # last pass of the 16-column staging and of the 4-column lane tiles)
+    nc, ld = 19, c.n + 3
+    R = torch.from_numpy(np.random.default_rng(3).uniform(-1, 1, (nc, ld))).to(cuda)
+    Z = torch.full((nc, ld), 7.0, dtype=torch.float64, device=cuda)
+    c.op.block_jacobi_apply_batched(R, Z)
+    for col in (0, 5, 15, 16, 18):
+        zc = torch.zeros(c.n, dtype=torch.float64, device=cuda)
+        c.op.block_jacobi_apply(R[col, :c.n].contiguous(), zc)
+        assert rel_l2(Z[col, :c.n].cpu().numpy(), zc.cpu().numpy()) <= 1e-14
+    assert torch.all(Z[:, c.n:] == 7.0)
 
 
 def test_preconditioned_solves(cuda):
