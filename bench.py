@@ -226,13 +226,15 @@ def run_precision(args):
 
 def run_greens(args):
     """Discrete Green's function of a 2D plane-strain fault problem (P6 triangles, ~200k elements,
-    512 fault facets x 7 slip basis functions = 3,584 right-hand sides): a block of 64 slip
-    right-hand sides built on the GPU in one batched launch and advanced together by lockstep CG
+    512 fault facets x 7 slip basis functions = 3,584 right-hand sides): a block of 64 unit fault
+    states turned into facet slip (fault adapter) and right-hand sides on the GPU, one batched
+    launch each, advanced together by lockstep CG
     (one batched apply of 64 columns per iteration), FP32 MFMA + FP64 residual replacement, block
     Jacobi, against one right-hand side alone with the same solver.  Both run a fixed budget of
     args.iters iterations (block-Jacobi CG needs 10^4-10^5 iterations to reach 1e-10 on this mesh,
     more than a bounded run affords), so the comparison is throughput: right-hand-side iterations
-    per second, and the residual each reached."""
+    per second, and the residual each reached.  The 64 Green's-function columns (fault traction
+    in the fault basis) are then computed in one batched launch."""
     import torch
     import tandem_amd as T
     from tandem_amd import scenario as S
@@ -275,12 +277,18 @@ def run_greens(args):
     out["apply_fp32_64col_batched_ms"] = t_b
     out["apply_fp32_64col_one_by_one_ms"] = t_s
     out["apply_batching_speedup"] = t_s / t_b
-    cols = np.arange(nb)
-    fdat = torch.from_numpy(fr.slip_basis(op, cols).reshape(nb, -1)).cuda()
+    # the slip basis: unit fault states in the fault adapter's basis (nodal P6 per fault facet,
+    # along-fault component; up = +y, fault normal oriented to the hanging wall)
+    dip = np.radians(30.0)
+    nfault, nbf = op.fault_setup([0.0, 1.0], [np.sin(dip), np.cos(dip)])
+    out["green_function_columns"] = nfault * nbf
+    S = torch.zeros((nb, nfault * nbf), dtype=torch.float64, device="cuda")
+    S[torch.arange(nb), torch.arange(nb)] = 1.0
     B = torch.empty((nb, ld), dtype=torch.float64, device="cuda")
     Xs = torch.zeros_like(B)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    fdat = op.fault_slip(S)
     op.rhs_device(B, None, fdat)
     r = op.solve_multi(B, Xs, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=args.iters)
     torch.cuda.synchronize()
@@ -290,6 +298,13 @@ def run_greens(args):
     out["lockstep_iterations"] = int(r["iterations"])
     out["lockstep_worst_rel_residual"] = float(np.max(r["column_rel_residual"]))
     out["lockstep_rhs_iterations_per_second"] = nb * r["iterations"] / t_lock
+    # the Green's function columns: fault traction of the 64 solutions in the fault basis
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    G = op.fault_traction(Xs, fdat)
+    torch.cuda.synchronize()
+    out["fault_traction_64col_ms"] = (time.perf_counter() - t2) * 1e3
+    out["green_block_shape"] = list(G.shape)
     # one right-hand side alone, same solver and budget
     b = B[0].clone()
     x = torch.zeros_like(b)

@@ -204,6 +204,26 @@ int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block);
 int tdg_material_gradient(tdg_op* op, double const* u, double const* psi, double* g_lam, double* g_mu);
 int tdg_material_gradient_f32(tdg_op* op, float const* u, float const* psi, float* g_lam, float* g_mu);
 
+/* Fault adapter and the Green's-function columns (elasticity, FP64, device buffers).
+ * tdg_fault_setup prepares the adapter on the Fault facets in facet order (AdapterBase::prepare,
+ *   app/localoperator/AdapterBase.cpp:39-100, with Curvilinear::facetBasis,
+ *   src/geometry/Curvilinear.cpp:281-316): up and ref_normal are the scenario's vectors (dim
+ *   entries); sizes[0] = number of fault facets, sizes[1] = nbf of the fault space (nodal P_N on
+ *   the (dim-1)-simplex).  The fault state of one column is nfault x nbf x (dim-1) (per facet
+ *   (l, n) column-major), the fault traction nfault x nbf x dim (per facet (k, p) column-major, p = 0
+ *   the normal component).
+ * tdg_fault_slip: facet data F (facets x dim x nq per column, zero off the fault) of the slips S:
+ *   Adapter<Elasticity>::slip (app/localoperator/ElasticityAdapter.cpp:37-49) for every fault facet.
+ * tdg_fault_traction: T = fault-basis traction of the solutions X with facet data Fd (may be NULL):
+ *   the facet traction (Elasticity::traction_skeleton/boundary, app/localoperator/Elasticity.cpp:
+ *   949-1018) on the fault facets followed by Adapter<Elasticity>::traction (ElasticityAdapter.cpp:
+ *   21-34); with X the solutions for unit fault-state columns, T holds the corresponding columns of
+ *   the discrete Green's function (SeasQDDiscreteGreenOperator.cpp:536-545). */
+int tdg_fault_setup(tdg_op* op, double const* up, double const* ref_normal, int64_t* sizes);
+int tdg_fault_slip(tdg_op* op, double const* S, double* F, int ncols, int64_t ld_s, int64_t ld_f);
+int tdg_fault_traction(tdg_op* op, double const* X, double const* Fd, double* T, int ncols, int64_t ld_x,
+                       int64_t ld_f, int64_t ld_t);
+
 /* Elastic-wave pieces (elasticity, FP64, device vectors of local size).
  * tdg_inverse_mass: y = M^-1 x with the weight-adjusted inverse of the curvilinear element mass
  *   (unit density): Mhat^-1 E diag(-W/|J|) E^T Mhat^-1 per element and component; replaces the

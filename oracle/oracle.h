@@ -150,6 +150,25 @@ struct DGCommon {
     std::size_t nF() const { return topo.fct.size(); }
 };
 
+// Fault adapter (app/localoperator/AdapterBase.cpp:20-100, ElasticityAdapter.cpp:21-48, generated
+// from app/kernels/elasticity_adapter.py:5-30): the fault facets (BC::Fault, facet order), on each
+// the nodal P_N basis of the (D-1)-simplex at the facet rule's points, the normal length, the
+// fault basis (unit normal, tangents from the up vector; signs fixed by the reference normal) and
+// the inverse of the facet mass matrix weighted by the normal length.
+struct FaultAdapter {
+    unsigned D = 0, nbf = 0, nq = 0;
+    std::vector<int64_t> facets;   // fault facet -> facet
+    std::vector<double> e_q, w;    // nbf x nq (column-major), nq
+    std::vector<double> nl;        // nf x nq
+    std::vector<double> basis;     // nf x nq x D x D: basis[((m nq + q) D + i) D + j] = component i of vector j
+    std::vector<double> minv;      // nf x nbf x nbf (column-major)
+    void setup(DGCommon const& c, double const* up, double const* ref_normal);
+    // traction(k, p) (nbf x D column-major) of facet-point traction t(o, q) (D x nq)
+    void traction(std::size_t m, double const* t_q, double* out) const;
+    // slip(o, q) (D x nq) at the facet points of the fault-basis slip s(l, n) (nbf x (D-1) column-major)
+    void slip(std::size_t m, double const* s, double* slip_q) const;
+};
+
 struct LocalOp {
     virtual ~LocalOp() = default;
     virtual std::size_t block_size() const = 0;
@@ -189,6 +208,7 @@ struct Elasticity : LocalOp {
     // null), then apply_inverse_mass
     void wave_rhs(double const* x, double const* f_q, double* y) const;
     unsigned degree = 0;
+    FaultAdapter fault;
 };
 
 // NodalInterpolation::assemble (app/localoperator/NodalInterpolation.cpp:8-38): I (n_to x n_from,

@@ -217,6 +217,33 @@ int oracle_op_traction(void* h, int64_t fct, double const* x, double const* f_q,
         o->el->traction(fct, x, f_q, t);
     });
 }
+// Fault adapter: setup returns {fault facets, nbf, nq} in sizes
+int oracle_op_fault_setup(void* h, double const* up, double const* ref_normal, int64_t* sizes) {
+    return guard([&] {
+        auto o = static_cast<OpHandle*>(h);
+        if (o->kind != 0) throw std::runtime_error("fault adapter: elasticity only");
+        o->el->fault.setup(o->el->c, up, ref_normal);
+        sizes[0] = int64_t(o->el->fault.facets.size());
+        sizes[1] = o->el->fault.nbf;
+        sizes[2] = o->el->fault.nq;
+    });
+}
+int oracle_op_fault_traction(void* h, int64_t m, double const* t_q, double* out) {
+    return guard([&] { static_cast<OpHandle*>(h)->el->fault.traction(std::size_t(m), t_q, out); });
+}
+int oracle_op_fault_slip(void* h, int64_t m, double const* s, double* slip_q) {
+    return guard([&] { static_cast<OpHandle*>(h)->el->fault.slip(std::size_t(m), s, slip_q); });
+}
+// facets (nf), basis (nf x nq x D x D), nl (nf x nq), e_q (nbf x nq)
+int oracle_op_fault_arrays(void* h, int64_t* facets, double* basis, double* nl, double* e_q) {
+    return guard([&] {
+        auto const& a = static_cast<OpHandle*>(h)->el->fault;
+        std::copy(a.facets.begin(), a.facets.end(), facets);
+        std::copy(a.basis.begin(), a.basis.end(), basis);
+        std::copy(a.nl.begin(), a.nl.end(), nl);
+        std::copy(a.e_q.begin(), a.e_q.end(), e_q);
+    });
+}
 // Discrete solution at the volume quadrature points: out (E x Q x Nq) = sum_k E_Q(k,q) x(k,p).
 // Also returns the quadrature weights times |det J| (E x Nq) in wj when non-null.
 int oracle_op_eval_volume(void* h, double const* x, double* out, double* wj) {
@@ -302,6 +329,7 @@ int oracle_op_array(void* h, char const* name, double* out) {
         else if (n == "normalLength") v = &c.normalLength;
         else if (n == "fcoords") v = &c.fcoords;
         else if (n == "area") v = &c.area;
+        else if (n == "facetRuleWeights") v = &c.fctRule.wts;
         else if (o->kind == 0) {
             auto& e = *o->el;
             if (n == "lam") v = &e.lam;
@@ -335,7 +363,7 @@ int64_t oracle_op_array_size(void* h, char const* name) {
     if (n == "volume") return E;
     if (n == "JInv0" || n == "JInv1") return F * nq * D * D;
     if (n == "normal" || n == "unitNormal" || n == "fcoords") return F * nq * D;
-    if (n == "normalLength") return F * nq;
+    if (n == "normalLength" || n == "facetRuleWeights") return n == "normalLength" ? F * nq : nq;
     if (n == "area" || n == "penalty") return F;
     if (n == "lam" || n == "mu" || n == "K") return E * (o->kind == 0 ? o->el->Nbf : o->po->matSpace.nbf);
     if (n == "lamWJ" || n == "muWJ" || n == "JWK") return E * Nq;

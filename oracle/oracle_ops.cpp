@@ -1032,3 +1032,111 @@ void Poisson::rhs(double const* force_Q, double const* facet_q, double* b) const
 }
 
 } // namespace oracle
+
+namespace oracle {
+
+// AdapterBase::AdapterBase / prepare (app/localoperator/AdapterBase.cpp:20-100) with
+// Curvilinear::facetBasis (src/geometry/Curvilinear.cpp:281-316).
+void FaultAdapter::setup(DGCommon const& c, double const* up, double const* ref_normal) {
+    D = c.D;
+    RefElement space(D - 1, c.N, false);
+    nbf = space.nbf;
+    nq = unsigned(c.fctRule.wts.size());
+    e_q = space.eval_basis(c.fctRule.pts); // (l, q)
+    w = c.fctRule.wts;
+    facets.clear();
+    for (std::size_t f = 0; f < c.nF(); ++f)
+        if (c.topo.fct[f].bc == BC::Fault) facets.push_back(int64_t(f));
+    const std::size_t nf = facets.size();
+    nl.assign(nf * nq, 0.0);
+    basis.assign(nf * nq * D * D, 0.0);
+    minv.assign(nf * nbf * nbf, 0.0);
+    const double tol = 10000.0 * std::numeric_limits<double>::epsilon();
+    for (std::size_t m = 0; m < nf; ++m) {
+        const std::size_t f = std::size_t(facets[m]);
+        for (unsigned q = 0; q < nq; ++q) {
+            double n[3] = {0, 0, 0}, len = 0.0, dot = 0.0;
+            for (unsigned d = 0; d < D; ++d) {
+                n[d] = c.normal[(f * nq + q) * D + d];
+                len += n[d] * n[d];
+                dot += ref_normal[d] * n[d];
+            }
+            len = std::sqrt(len);
+            nl[m * nq + q] = len;
+            if (std::fabs(dot) < tol) throw std::logic_error("Normal and reference normal are almost perpendicular.");
+            const bool flipped = dot < 0.0;
+            if (flipped)
+                for (unsigned d = 0; d < D; ++d) n[d] = -n[d];
+            double* b = &basis[(m * nq + q) * D * D]; // b[i D + j]: component i of vector j
+            double nh[3] = {0, 0, 0};
+            for (unsigned d = 0; d < D; ++d) nh[d] = n[d] / len;
+            for (unsigned d = 0; d < D; ++d) b[d * D + 0] = nh[d];
+            if (D == 2) {
+                const double sg = up[0] * nh[1] - up[1] * nh[0];
+                if (std::fabs(sg) < tol) throw std::logic_error("Up vector and normal are almost colinear.");
+                const double s = sg > 0 ? 1.0 : -1.0;
+                b[0 * D + 1] = -s * nh[1];
+                b[1 * D + 1] = s * nh[0];
+            } else {
+                // s = normalize(up x n), d = normalize(s x n)
+                double sv[3] = {up[1] * nh[2] - up[2] * nh[1], up[2] * nh[0] - up[0] * nh[2],
+                                up[0] * nh[1] - up[1] * nh[0]};
+                double sl = std::sqrt(sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2]);
+                if (sl < tol) throw std::logic_error("Up vector and normal are almost colinear.");
+                for (double& v : sv) v /= sl;
+                double dv[3] = {sv[1] * nh[2] - sv[2] * nh[1], sv[2] * nh[0] - sv[0] * nh[2],
+                                sv[0] * nh[1] - sv[1] * nh[0]};
+                double dl = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+                for (unsigned d = 0; d < 3; ++d) {
+                    b[d * D + 1] = dv[d] / dl;
+                    b[d * D + 2] = sv[d];
+                }
+            }
+            if (flipped)
+                for (unsigned i = 0; i < D * D; ++i) b[i] = -b[i];
+        }
+        // m(i, j) = sum_q w_q nl_q e(i, q) e(j, q); minv = m^-1
+        std::vector<double> M(nbf * nbf, 0.0), I(nbf * nbf, 0.0);
+        for (unsigned i = 0; i < nbf; ++i)
+            for (unsigned j = 0; j < nbf; ++j)
+                for (unsigned q = 0; q < nq; ++q)
+                    M[i + j * nbf] += w[q] * nl[m * nq + q] * e_q[i + nbf * q] * e_q[j + nbf * q];
+        for (unsigned i = 0; i < nbf; ++i) I[i + i * nbf] = 1.0;
+        lu_solve(M, int(nbf), I.data(), int(nbf));
+        std::copy(I.begin(), I.end(), minv.begin() + m * nbf * nbf);
+    }
+}
+
+// evaluate_traction: traction(k, p) = sum_l minv(l, k) sum_q e_q(l, q) w_q nl_q sum_o t(o, q) basis(o, p, q)
+void FaultAdapter::traction(std::size_t m, double const* t_q, double* out) const {
+    std::vector<double> s(nbf * D, 0.0);
+    for (unsigned q = 0; q < nq; ++q) {
+        double const* b = &basis[(m * nq + q) * D * D];
+        for (unsigned p = 0; p < D; ++p) {
+            double tb = 0.0;
+            for (unsigned o = 0; o < D; ++o) tb += t_q[o * nq + q] * b[o * D + p];
+            for (unsigned l = 0; l < nbf; ++l) s[l + p * nbf] += e_q[l + nbf * q] * w[q] * nl[m * nq + q] * tb;
+        }
+    }
+    double const* Mi = &minv[m * nbf * nbf];
+    for (unsigned p = 0; p < D; ++p)
+        for (unsigned k = 0; k < nbf; ++k) {
+            double v = 0.0;
+            for (unsigned l = 0; l < nbf; ++l) v += Mi[l + k * nbf] * s[l + p * nbf];
+            out[k + p * nbf] = v;
+        }
+}
+
+// evaluate_slip: slip_q(p, q) = sum_l e_q(l, q) sum_n basis(p, n + 1, q) s(l, n)
+void FaultAdapter::slip(std::size_t m, double const* s, double* slip_q) const {
+    for (unsigned p = 0; p < D; ++p)
+        for (unsigned q = 0; q < nq; ++q) {
+            double const* b = &basis[(m * nq + q) * D * D];
+            double v = 0.0;
+            for (unsigned n = 0; n + 1 < D; ++n)
+                for (unsigned l = 0; l < nbf; ++l) v += e_q[l + nbf * q] * b[p * D + n + 1] * s[l + n * nbf];
+            slip_q[p * nq + q] = v;
+        }
+}
+
+} // namespace oracle

@@ -368,6 +368,32 @@ int tdg_material_gradient_f32(tdg_op* op, float const* u, float const* psi, floa
     });
 }
 
+int tdg_fault_setup(tdg_op* op, double const* up, double const* ref_normal, int64_t* sizes) {
+    return guarded([&] {
+        require(op && op->impl && up && ref_normal && sizes, "null argument");
+        auto r = op->impl->fault_setup(up, ref_normal);
+        sizes[0] = r[0];
+        sizes[1] = r[1];
+    });
+}
+
+int tdg_fault_slip(tdg_op* op, double const* S, double* F, int ncols, int64_t ld_s, int64_t ld_f) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(S && F && ncols >= 1, "bad fault slip arguments");
+        o.fault_slip_device(S, F, ncols, ld_s, ld_f);
+    });
+}
+
+int tdg_fault_traction(tdg_op* op, double const* X, double const* Fd, double* T, int ncols, int64_t ld_x,
+                       int64_t ld_f, int64_t ld_t) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(X && T && ncols >= 1, "bad fault traction arguments");
+        o.fault_traction_device(X, Fd, T, ncols, ld_x, ld_f, ld_t);
+    });
+}
+
 int tdg_inverse_mass(tdg_op* op, double const* x, double* y) {
     return guarded([&] {
         require(x && y, "null argument");

@@ -865,6 +865,131 @@ std::vector<double> Operator::interpolation_matrix(unsigned D, unsigned to, unsi
     return bf.values(bt.nodes()); // column-major (n_from x n_to) == row-major (n_to x n_from)
 }
 
+// Fault adapter: AdapterBase::prepare (app/localoperator/AdapterBase.cpp:39-100) with
+// Curvilinear::facetBasis (src/geometry/Curvilinear.cpp:281-316): per fault facet point the normal
+// length, the unit normal oriented along the reference normal, the tangents from the up vector, all
+// basis vectors negated where the orientation flipped the normal, and the inverse of the facet mass
+// weighted by the normal length, on the nodal P_N basis of the (D-1)-simplex.
+std::array<int64_t, 2> Operator::fault_setup(double const* up, double const* ref_normal) {
+    if (kind_ != Kind_Elasticity) throw std::invalid_argument("the fault adapter is built for elasticity");
+    const unsigned D = D_, nq = nq_;
+    Basis space(D - 1, N_, true);
+    const unsigned nbf = space.size();
+    std::vector<double> e_q = space.values(fct_.points); // (l + nbf q)
+    std::vector<int32_t> fac, fmap(topo_.facets.size(), -1);
+    for (std::size_t f = 0; f < topo_.facets.size(); ++f)
+        if (topo_.facets[f].bc == BC_Fault) {
+            fmap[f] = int32_t(fac.size());
+            fac.push_back(int32_t(f));
+        }
+    const std::size_t nf = fac.size();
+    std::vector<double> nl(nf * nq), bas(nf * nq * D * D, 0.0), minv(nf * nbf * nbf, 0.0);
+    const double tol = 10000.0 * std::numeric_limits<double>::epsilon();
+    for (std::size_t m = 0; m < nf; ++m) {
+        const std::size_t f = std::size_t(fac[m]);
+        for (unsigned q = 0; q < nq; ++q) {
+            double n[3] = {0, 0, 0}, len = 0.0, dot = 0.0;
+            for (unsigned d = 0; d < D; ++d) {
+                n[d] = normal_[(f * nq + q) * D + d];
+                len += n[d] * n[d];
+            }
+            len = std::sqrt(len);
+            for (unsigned d = 0; d < D; ++d) dot += ref_normal[d] * n[d];
+            nl[m * nq + q] = len;
+            if (std::fabs(dot) < tol) throw std::invalid_argument("normal and reference normal are almost perpendicular");
+            const double flip = dot < 0.0 ? -1.0 : 1.0;
+            double nh[3] = {0, 0, 0};
+            for (unsigned d = 0; d < D; ++d) nh[d] = flip * n[d] / len; // oriented along the reference normal
+            double v1[3] = {0, 0, 0}, v2[3] = {0, 0, 0};
+            if (D == 2) {
+                const double c = up[0] * nh[1] - up[1] * nh[0];
+                if (std::fabs(c) < tol) throw std::invalid_argument("up vector and normal are almost colinear");
+                const double sg = c > 0.0 ? 1.0 : -1.0;
+                v1[0] = -sg * nh[1];
+                v1[1] = sg * nh[0];
+            } else {
+                double sv[3] = {up[1] * nh[2] - up[2] * nh[1], up[2] * nh[0] - up[0] * nh[2],
+                                up[0] * nh[1] - up[1] * nh[0]};
+                const double sl = std::sqrt(sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2]);
+                if (sl < tol) throw std::invalid_argument("up vector and normal are almost colinear");
+                for (double& v : sv) v /= sl;
+                double dv[3] = {sv[1] * nh[2] - sv[2] * nh[1], sv[2] * nh[0] - sv[0] * nh[2],
+                                sv[0] * nh[1] - sv[1] * nh[0]};
+                const double dl = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+                for (unsigned d = 0; d < 3; ++d) {
+                    v1[d] = dv[d] / dl;
+                    v2[d] = sv[d];
+                }
+            }
+            double* b = &bas[(m * nq + q) * D * D]; // b[i D + j]: component i of vector j
+            for (unsigned d = 0; d < D; ++d) {
+                b[d * D + 0] = flip * nh[d];
+                b[d * D + 1] = flip * v1[d];
+                if (D == 3) b[d * D + 2] = flip * v2[d];
+            }
+        }
+        std::vector<double> M(std::size_t(nbf) * nbf, 0.0), I(std::size_t(nbf) * nbf, 0.0);
+        for (unsigned q = 0; q < nq; ++q) {
+            const double wq = fct_.weights[q] * nl[m * nq + q];
+            for (unsigned j = 0; j < nbf; ++j)
+                for (unsigned i = 0; i < nbf; ++i) M[i + j * nbf] += wq * e_q[i + nbf * q] * e_q[j + nbf * q];
+        }
+        for (unsigned i = 0; i < nbf; ++i) I[i + i * nbf] = 1.0;
+        dense_solve(M, int(nbf), I.data(), int(nbf));
+        std::copy(I.begin(), I.end(), minv.begin() + m * nbf * nbf);
+    }
+    auto fd = std::make_unique<DeviceFault>();
+    fd->fac.upload(fac);
+    fd->fmap.upload(fmap);
+    fd->e_q.upload(e_q);
+    fd->w.upload(fct_.weights);
+    fd->nl.upload(nl);
+    fd->bas.upload(bas);
+    fd->minv.upload(minv);
+    fd->nfault = int64_t(nf);
+    fd->nbf = int(nbf);
+    fault_ = std::move(fd);
+    return {int64_t(nf), int64_t(nbf)};
+}
+
+FaultArgs Operator::fault_args() const {
+    if (!fault_) throw std::logic_error("fault_setup() must be called first");
+    FaultArgs a;
+    a.fac = fault_->fac.ptr;
+    a.fmap = fault_->fmap.ptr;
+    a.e_q = fault_->e_q.ptr;
+    a.w = fault_->w.ptr;
+    a.nl = fault_->nl.ptr;
+    a.bas = fault_->bas.ptr;
+    a.minv = fault_->minv.ptr;
+    a.nbf = fault_->nbf;
+    a.nq = int(nq_);
+    a.dim = int(D_);
+    a.nfault = fault_->nfault;
+    a.num_facets = int64_t(topo_.facets.size());
+    return a;
+}
+
+void Operator::fault_slip_device(double const* S, double* F, int ncols, int64_t ld_s, int64_t ld_f) {
+    FaultArgs a = fault_args();
+    TDG_HIP(launch_fault_slip(a, S, F, ncols, ld_s, ld_f, stream_));
+}
+
+// Traction of the solution columns on the fault facets only, then its projection onto the fault
+// basis: the column blocks of the Green's function (SeasQDDiscreteGreenOperator.cpp:536-545).
+void Operator::fault_traction_device(double const* X, double const* Fd, double* T, int ncols, int64_t ld_x,
+                                     int64_t ld_f, int64_t ld_t) {
+    if (!ready_) throw std::logic_error("set_material() must be called first");
+    FaultArgs a = fault_args();
+    RhsArgs r = rhs_args();
+    const int64_t per = a.nfault * int64_t(D_) * nq_;
+    const std::size_t need = std::size_t(per) * std::size_t(ncols);
+    if (fault_->work.n < need) fault_->work.alloc(need);
+    TDG_HIP(launch_elasticity_traction_subset(D_, N_, r, X, Fd, fault_->work.ptr, a.fac, a.nfault, ncols, ld_x, ld_f,
+                                              per, stream_));
+    TDG_HIP(launch_fault_project(a, fault_->work.ptr, T, ncols, per, ld_t, stream_));
+}
+
 BlockJacobi& Operator::block_jacobi() {
     if (!bj_) bj_ = std::make_unique<BlockJacobi>();
     if (!bj_->built()) bj_->build(*this);

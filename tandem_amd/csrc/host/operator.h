@@ -68,6 +68,14 @@ struct DeviceWave {
     DeviceArray<double> Minv, E, w, ax, b;
 };
 
+// Fault adapter data (FP64) and a work buffer for the facet-point traction of the fault facets.
+struct DeviceFault {
+    DeviceArray<int32_t> fac, fmap;
+    DeviceArray<double> e_q, w, nl, bas, minv, work;
+    int64_t nfault = 0;
+    int nbf = 0;
+};
+
 // Kernel selection: Variant_Auto uses the MFMA kernel where one is compiled (elasticity), the
 // thread-per-point kernel otherwise; Variant_Points forces the thread-per-point kernel;
 // Variant_MfmaXcd the MFMA kernel with workgroups renumbered per XCD.
@@ -146,6 +154,16 @@ public:
     // Nodal interpolation matrix from degree `from` to degree `to`, row-major (to x from).
     static std::vector<double> interpolation_matrix(unsigned D, unsigned to, unsigned from);
 
+    // Fault adapter (AdapterBase.cpp:20-100): set up on the Fault facets (facet order) with the up
+    // vector and reference normal of the scenario; returns {fault facets, nbf of the fault space}.
+    std::array<int64_t, 2> fault_setup(double const* up, double const* ref_normal);
+    // Facet data (F x Q x nq per column, zero off the fault) of fault-basis slips S (nfault x nbf x
+    // (D-1) per column), and the fault-basis traction T (nfault x nbf x D per column) of solutions X
+    // with facet data Fd (may be null): ncols columns at the given strides, device buffers.
+    void fault_slip_device(double const* S, double* F, int ncols, int64_t ld_s, int64_t ld_f);
+    void fault_traction_device(double const* X, double const* Fd, double* T, int ncols, int64_t ld_x, int64_t ld_f,
+                               int64_t ld_t);
+
     // named host arrays for inspection
     std::vector<double> const* host_array(std::string const& name) const;
 
@@ -201,6 +219,8 @@ private:
     std::unique_ptr<DeviceGrad<double>> g64_;
     std::unique_ptr<DeviceGrad<float>> g32_;
     std::unique_ptr<DeviceWave> wave_;
+    std::unique_ptr<DeviceFault> fault_;
+    FaultArgs fault_args() const;
     WaveArgs wave_args();
     template <class T> void upload_grad(DeviceGrad<T>& g);
     template <class T> void grad_dispatch(T const* u, T const* psi, T* gl, T* gm);

@@ -114,15 +114,22 @@ __global__ __launch_bounds__(kThreads) void elasticity_rhs_kernel(RhsArgs a, dou
 }
 
 template <int D, int N>
-__global__ __launch_bounds__(kThreads) void elasticity_traction_kernel(RhsArgs a, double const* __restrict__ x,
-                                                                       double const* __restrict__ fdata,
-                                                                       double* __restrict__ tq) {
+__global__ __launch_bounds__(kThreads) void elasticity_traction_kernel(RhsArgs a, double const* __restrict__ xs,
+                                                                       double const* __restrict__ fds,
+                                                                       double* __restrict__ tqs,
+                                                                       int32_t const* __restrict__ sub, int64_t nsub,
+                                                                       int64_t ld_x, int64_t ld_f, int64_t ld_t) {
     constexpr int Nbf = binom_(N + D, D), nq = fct_points_(D, N);
     constexpr int Q = D, BS = Nbf * Q, DD = D * D, C = D + 1;
     const int64_t item = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (item >= a.num_facets * nq) return;
-    const int64_t fct = item / nq;
+    if (item >= nsub * nq) return;
+    // facet `fct` of the list (all facets, or the subset sub), output row `row`, column blockIdx.y
+    const int64_t row = item / nq;
+    const int64_t fct = sub ? int64_t(sub[row]) : row;
     const int q = int(item % nq);
+    double const* __restrict__ x = xs + int64_t(blockIdx.y) * ld_x;
+    double const* __restrict__ fdata = fds ? fds + int64_t(blockIdx.y) * ld_f : nullptr;
+    double* __restrict__ tq = tqs + int64_t(blockIdx.y) * ld_t;
     const int4 fi = a.fct[fct]; // {element 0, element 1, local face 0 | local face 1 << 4, -}
     const int fb = a.fbc[fct];
     const int bc = fb & 0xff;
@@ -194,7 +201,7 @@ __global__ __launch_bounds__(kThreads) void elasticity_traction_kernel(RhsArgs a
 #pragma unroll
     for (int p = 0; p < Q; ++p) {
         const double f = data ? scale * fdata[(fct * Q + p) * nq + q] : 0.0;
-        tq[(fct * Q + p) * nq + q] = tr[p] + c00 * (jump[p] - f);
+        tq[(row * Q + p) * nq + q] = tr[p] + c00 * (jump[p] - f);
     }
 }
 
@@ -210,11 +217,14 @@ hipError_t run_rhs(RhsArgs const& a, double const* force, double const* fdata, d
 }
 
 template <int D, int N>
-hipError_t run_traction(RhsArgs const& a, double const* x, double const* fdata, double* t, hipStream_t s) {
+hipError_t run_traction(RhsArgs const& a, double const* x, double const* fdata, double* t, int32_t const* sub,
+                        int64_t nsub, int ncols, int64_t ld_x, int64_t ld_f, int64_t ld_t, hipStream_t s) {
     constexpr int nq = fct_points_(D, N);
-    const int64_t items = a.num_facets * nq;
-    dim3 grid(unsigned((items + kThreads - 1) / kThreads));
-    hipLaunchKernelGGL((elasticity_traction_kernel<D, N>), grid, dim3(kThreads), 0, s, a, x, fdata, t);
+    const int64_t items = nsub * nq;
+    if (items == 0) return hipSuccess;
+    dim3 grid(unsigned((items + kThreads - 1) / kThreads), unsigned(ncols));
+    hipLaunchKernelGGL((elasticity_traction_kernel<D, N>), grid, dim3(kThreads), 0, s, a, x, fdata, t, sub, nsub, ld_x,
+                       ld_f, ld_t);
     return hipGetLastError();
 }
 
@@ -253,7 +263,17 @@ hipError_t launch_elasticity_rhs(int dim, int degree, RhsArgs const& a, double c
 hipError_t launch_elasticity_traction(int dim, int degree, RhsArgs const& a, double const* x, double const* fdata,
                                       double* t, hipStream_t s) {
     return by_degree(dim, degree, [&](auto Dc, auto Nc) {
-        return run_traction<decltype(Dc)::value, decltype(Nc)::value>(a, x, fdata, t, s);
+        return run_traction<decltype(Dc)::value, decltype(Nc)::value>(a, x, fdata, t, nullptr, a.num_facets, 1, 0, 0,
+                                                                       0, s);
+    });
+}
+
+hipError_t launch_elasticity_traction_subset(int dim, int degree, RhsArgs const& a, double const* x,
+                                             double const* fdata, double* t, int32_t const* sub, int64_t nsub,
+                                             int ncols, int64_t ld_x, int64_t ld_f, int64_t ld_t, hipStream_t s) {
+    return by_degree(dim, degree, [&](auto Dc, auto Nc) {
+        return run_traction<decltype(Dc)::value, decltype(Nc)::value>(a, x, fdata, t, sub, nsub, ncols, ld_x, ld_f,
+                                                                       ld_t, s);
     });
 }
 

@@ -97,5 +97,33 @@ hipError_t launch_elasticity_rhs(int dim, int degree, RhsArgs const& a, double c
 // t (F x Q x nq) = facet traction of x with facet data fdata (may be null).
 hipError_t launch_elasticity_traction(int dim, int degree, RhsArgs const& a, double const* x, double const* fdata,
                                       double* t, hipStream_t stream);
+// Traction on the facets sub[0..nsub) only (rows of t in list order), for ncols columns of x, facet
+// data and t at strides ld_x, ld_f, ld_t.
+hipError_t launch_elasticity_traction_subset(int dim, int degree, RhsArgs const& a, double const* x,
+                                             double const* fdata, double* t, int32_t const* sub, int64_t nsub,
+                                             int ncols, int64_t ld_x, int64_t ld_f, int64_t ld_t, hipStream_t s);
+
+// Fault adapter (one precision, FP64): per fault facet m (facet fac[m]) the fault-space basis at the
+// facet points e_q[q][l], weights w, normal length nl, fault basis vectors bas[m][q][i][j] (component
+// i of vector j) and the inverse weighted facet mass minv[m][k][l].
+struct FaultArgs {
+    int32_t const* fac;
+    int32_t const* fmap; // facet -> fault facet or -1
+    double const* e_q;
+    double const* w;
+    double const* nl;
+    double const* bas;
+    double const* minv;
+    int nbf, nq, dim;
+    int64_t nfault, num_facets;
+};
+// T[c] (nfault x nbf x D, (k, p) column-major per facet) = projection of the facet-point traction
+// tq[c] (nfault x D x nq) onto the fault basis; ncols columns at strides ld_q, ld_t.
+hipError_t launch_fault_project(FaultArgs const& a, double const* tq, double* T, int ncols, int64_t ld_q,
+                                int64_t ld_t, hipStream_t s);
+// Facet data F[c] (num_facets x D x nq; zero off the fault) of the fault-basis slip S[c]
+// (nfault x nbf x (D-1), (l, n) column-major per facet).
+hipError_t launch_fault_slip(FaultArgs const& a, double const* S, double* F, int ncols, int64_t ld_s,
+                             int64_t ld_f, hipStream_t s);
 
 } // namespace tdg

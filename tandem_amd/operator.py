@@ -341,6 +341,43 @@ class DGOperator:
                 C.c_void_p(g_mu.data_ptr())))
         return g_lam.view(self.E, self.Nq), g_mu.view(self.E, self.Nq)
 
+    def fault_setup(self, up, ref_normal):
+        """Fault adapter on the Fault facets; returns (number of fault facets, nbf of the fault space)."""
+        sizes = np.zeros(2, dtype=np.int64)
+        check(lib().tdg_fault_setup(self._h, np.ascontiguousarray(up, dtype=np.float64),
+                                    np.ascontiguousarray(ref_normal, dtype=np.float64), sizes))
+        self.nfault, self.fault_nbf = int(sizes[0]), int(sizes[1])
+        return self.nfault, self.fault_nbf
+
+    def fault_slip(self, S, F=None):
+        """Facet data (F*D*nq per column) of fault-basis slips S: (nfault*nbf*(D-1)) or (ncols, ...)
+        float64 GPU tensors."""
+        torch = _torch()
+        ncols = 1 if S.dim() == 1 else S.shape[0]
+        width = self.F * self.dim * self.nq
+        if F is None:
+            F = torch.empty((ncols, width) if S.dim() == 2 else (width,), dtype=torch.float64, device=S.device)
+        ld_s = S.shape[-1] if S.dim() == 2 else 0
+        ld_f = F.shape[-1] if F.dim() == 2 else 0
+        check(lib().tdg_fault_slip(self._h, C.c_void_p(S.data_ptr()), C.c_void_p(F.data_ptr()), ncols, ld_s, ld_f))
+        return F
+
+    def fault_traction(self, X, Fd=None, T=None):
+        """Fault-basis traction (nfault*nbf*D per column) of solutions X (n) or (ncols, >= n) with
+        facet data Fd (same column layout, may be None)."""
+        torch = _torch()
+        ncols = 1 if X.dim() == 1 else X.shape[0]
+        width = self.nfault * self.fault_nbf * self.dim
+        if T is None:
+            T = torch.empty((ncols, width) if X.dim() == 2 else (width,), dtype=torch.float64, device=X.device)
+        ld_x = X.shape[-1] if X.dim() == 2 else 0
+        ld_f = Fd.shape[-1] if (Fd is not None and Fd.dim() == 2) else 0
+        ld_t = T.shape[-1] if T.dim() == 2 else 0
+        fp = None if Fd is None else C.c_void_p(Fd.data_ptr())
+        check(lib().tdg_fault_traction(self._h, C.c_void_p(X.data_ptr()), fp, C.c_void_p(T.data_ptr()), ncols, ld_x,
+                                       ld_f, ld_t))
+        return T
+
     def inverse_mass(self, x, y=None):
         """y = M^-1 x (weight-adjusted inverse mass with the reference's negative sign) for a float64
         GPU vector x."""

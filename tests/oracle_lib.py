@@ -63,6 +63,10 @@ def lib():
     L.oracle_op_assemble.argtypes = [C.c_void_p, dp]
     L.oracle_op_rhs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, dp]
     L.oracle_op_traction.argtypes = [C.c_void_p, C.c_int64, dp, C.c_void_p, dp]
+    L.oracle_op_fault_setup.argtypes = [C.c_void_p, dp, dp, ip64]
+    L.oracle_op_fault_traction.argtypes = [C.c_void_p, C.c_int64, dp, dp]
+    L.oracle_op_fault_slip.argtypes = [C.c_void_p, C.c_int64, dp, dp]
+    L.oracle_op_fault_arrays.argtypes = [C.c_void_p, ip64, dp, dp, dp]
     L.oracle_op_material_gradient.argtypes = [C.c_void_p, dp, dp, dp, dp]
     L.oracle_op_wave_rhs.argtypes = [C.c_void_p, dp, C.c_void_p, dp]
     L.oracle_op_inverse_mass.argtypes = [C.c_void_p, dp, dp]
@@ -266,6 +270,38 @@ class Operator:
         if lib().oracle_op_traction(self.h, fct, np.ascontiguousarray(x, dtype=np.float64), _ptr(fq), t) != 0:
             _err()
         return t.reshape(self.D, self.nq)
+
+    def fault_setup(self, up, ref_normal):
+        """Fault adapter on the BC::Fault facets; returns (facets, basis (nf, nq, D, D) with
+        basis[m, q, i, j] = component i of vector j, nl (nf, nq), e_q (nq, nbf))."""
+        sz = np.zeros(3, dtype=np.int64)
+        if lib().oracle_op_fault_setup(self.h, np.asarray(up, dtype=np.float64), np.asarray(ref_normal, dtype=np.float64),
+                                       sz) != 0:
+            _err()
+        nf, nbf, nq = (int(v) for v in sz)
+        self.fault_nbf = nbf
+        fac = np.zeros(nf, dtype=np.int64)
+        basis = np.zeros(nf * nq * self.D * self.D)
+        nl = np.zeros(nf * nq)
+        e_q = np.zeros(nbf * nq)
+        if lib().oracle_op_fault_arrays(self.h, fac, basis, nl, e_q) != 0:
+            _err()
+        return fac, basis.reshape(nf, nq, self.D, self.D), nl.reshape(nf, nq), e_q.reshape(nq, nbf)
+
+    def fault_traction(self, m, t_q):
+        """Fault-basis traction (nbf, D) of facet-point traction t_q (D, nq) on fault facet m."""
+        out = np.zeros(self.fault_nbf * self.D)
+        if lib().oracle_op_fault_traction(self.h, m, np.ascontiguousarray(t_q, dtype=np.float64), out) != 0:
+            _err()
+        return out.reshape(self.D, self.fault_nbf).T
+
+    def fault_slip(self, m, s):
+        """Facet-point slip (D, nq) of the fault-basis slip s (nbf, D-1) on fault facet m."""
+        out = np.zeros(self.D * self.nq)
+        if lib().oracle_op_fault_slip(self.h, m, np.ascontiguousarray(np.asarray(s).T, dtype=np.float64).ravel(),
+                                      out) != 0:
+            _err()
+        return out.reshape(self.D, self.nq)
 
     def material_gradient(self, u, psi):
         """d/d(lam_Q, mu_Q) of psi^T A u, each (E, Nq)."""

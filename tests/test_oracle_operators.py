@@ -402,3 +402,66 @@ def test_nodal_interpolation_reproduces_polynomials(D, to, frm):
             return v
         np.testing.assert_allclose(I @ poly(xf), poly(xt), atol=1e-12)
     np.testing.assert_allclose(I.sum(axis=1), 1.0, atol=1e-13)
+
+
+def _curved_fault_operator(D, N, scheme):
+    """Fault plane x = 1/2 in a box; the geometry map bends it (x-displacement that does not vanish
+    on the plane), so the normal length varies along every fault facet."""
+    m = _fault_mesh(D, 2, scheme)
+    nodes = m.affine_nodes(N)
+    X = nodes
+    bend = 0.04 * np.sin(np.pi * X[..., 1]) * (np.sin(np.pi * X[..., 2]) if D == 3 else 1.0)
+    nodes = nodes.copy()
+    nodes[..., 0] += bend * np.sin(np.pi * X[..., 0])
+    vc, _ = m.coords(N, nodes)
+    op = O.Operator(0, m, N, np.full(vc.shape[:2], 2.0), np.full(vc.shape[:2], 1.0), nodes=nodes)
+    return op
+
+
+@pytest.mark.parametrize("D,N,scheme", [(2, 3, 0), (3, 2, 1)])
+def test_fault_adapter_basis_and_projection(D, N, scheme):
+    """Fault adapter (AdapterBase.cpp:20-100, ElasticityAdapter.cpp:21-48): the first basis vector is
+    side 0's unit normal, the basis is orthonormal, and (2D) the tangent points against `up` after
+    orienting the normal along the reference normal; the traction projection returns the nodal
+    values of any field of the fault space exactly, its residual is orthogonal to the space in the
+    physical facet measure (a dropped normal length fails this on the curved fault), and projecting
+    an evaluated slip returns that slip with a zero normal component."""
+    op = _curved_fault_operator(D, N, scheme)
+    up = np.array([0.0, 1.0] if D == 2 else [0.0, 0.0, 1.0])
+    refn = np.array([1.0, 0.0] if D == 2 else [1.0, 0.0, 0.0])
+    fac, basis, nl, e_q = op.fault_setup(up, refn)
+    nf, nq = nl.shape
+    nbf = e_q.shape[1]
+    assert nf > 0 and nq == op.nq
+    normals = op.array("normal").reshape(op.F, nq, D)
+    rng = np.random.default_rng(4)
+    assert np.ptp(nl) > 1e-3  # the fault is curved
+    for m in range(nf):
+        n = normals[fac[m]]
+        nh = n / np.linalg.norm(n, axis=1)[:, None]
+        assert np.allclose(nl[m], np.linalg.norm(n, axis=1), rtol=0, atol=1e-14)
+        B = basis[m]                                   # (nq, D comp, D vec)
+        assert np.abs(B[:, :, 0] - nh).max() < 1e-13
+        assert np.abs(np.einsum("qij,qik->qjk", B, B) - np.eye(D)).max() < 1e-13
+        if D == 2:
+            sgn = np.sign(nh @ refn)
+            assert np.all(sgn[:, None] * np.einsum("qi,i->q", B[:, :, 1], up)[:, None] < 0)
+        # exactness on the fault space
+        g = rng.normal(size=(nbf, D))
+        tq = np.einsum("qk,kp,qop->oq", e_q, g, B)
+        assert np.abs(op.fault_traction(m, tq) - g).max() < 1e-11
+        # Galerkin orthogonality in the physical measure for a field outside the space
+        tq = rng.normal(size=(D, nq))
+        P = op.fault_traction(m, tq)                  # (nbf, D)
+        wq = _facet_weights(op)
+        resid = np.einsum("oq,qop->pq", tq, B) - (e_q @ P).T   # (D, nq)
+        assert np.abs(np.einsum("q,q,ql,pq->lp", wq, nl[m], e_q, resid)).max() < 1e-12 * np.abs(tq).max()
+        # slip round trip
+        s = rng.normal(size=(nbf, D - 1))
+        sq = op.fault_slip(m, s)
+        back = op.fault_traction(m, sq)
+        assert np.abs(back[:, 0]).max() < 1e-12 and np.abs(back[:, 1:] - s).max() < 1e-11
+
+
+def _facet_weights(op):
+    return op.array("facetRuleWeights")
