@@ -243,7 +243,7 @@ class DGOperator:
         chosen = C.c_int()
         ms = np.zeros(3)
         check(lib().tdg_autotune(self._h, ncols, 8 if precision == "fp64" else 4, C.byref(chosen), ms))
-        names = ["mfma", "mfma-xcd", "points"]
+        names = ["mfma", "points", "mfma-xcd"]  # indexed by TDG_VARIANT_*
         return names[chosen.value], {names[v]: float(ms[v]) for v in range(3) if ms[v] >= 0}
 
     def solve_segments(self, b, x, offsets, rtol=1e-10, mode="fp64", max_iter=200000, check_every=10,
