@@ -434,6 +434,71 @@ def run_multimesh(args):
     return out
 
 
+def run_poisson(args):
+    """c0: 2D Poisson P2 on the unit square split into 32 x 32 x 2 triangles (2,048 elements), K = 1,
+    u = sin(pi x) sin(pi y) with zero Dirichlet data: one apply on a seeded uniform[-1, 1] vector and
+    a CG solve to 1e-10, on the GPU and with the CPU FP64 oracle (apply: warm-up then 100 applies,
+    mean and min as in tandem's static.cpp; solve: scipy CG on the oracle's apply)."""
+    import torch
+    import tandem_amd as T
+    from tandem_amd import scenario as S
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from scipy.sparse.linalg import LinearOperator, cg
+
+    mesh = T.Mesh.box(2, 32, 0, T.BC_DIRICHLET)
+    op = on_stream(T.Poisson(mesh, 2, K=1.0))
+    X = op.volume_points()
+    _, f = S.poisson_sine(X)
+    b = op.rhs(f[:, None, :], np.zeros((op.F, 1, op.nq)))
+    x = torch.from_numpy(np.random.default_rng(SEED).uniform(-1, 1, op.n)).cuda()
+    y = torch.zeros_like(x)
+    stream = torch.cuda.current_stream()
+    t_ev, _ = time_applies(op, x, y, 100, 10, stream)
+    bd = torch.from_numpy(b).cuda()
+    xs = torch.zeros_like(bd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = op.solve(bd, xs, rtol=1e-10, mode="fp64")
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    # L2 error against the manufactured solution at the volume points
+    U = xs.view(op.E, -1).cpu().numpy()                                   # (E, Nbf) modal
+    EQ = op.host_array("E_Q").reshape(op.Nq, -1).T                        # (Nbf, Nq)
+    wj = op.host_array("absDetJ").reshape(op.E, op.Nq) * op.host_array("volWeights")[None, :]
+    ue, _ = S.poisson_sine(X)
+    uh = U @ EQ
+    err = float(np.sqrt(np.sum(wj * (uh - ue) ** 2) / np.sum(wj * ue ** 2)))
+    out = {"workload": "c0: Poisson P2, 32x32x2 triangles, %d elements, %d DoF" % (op.E, op.n),
+           "gpu_apply_us": t_ev * 1e6, "gpu_apply_gdofs": op.n / t_ev * 1e-9,
+           "gpu_solve_s": t_solve, "gpu_solve_iterations": int(r["iterations"]),
+           "gpu_rel_residual": float(r["rel_residual"]), "l2_error_vs_manufactured": err}
+    # CPU oracle
+    om = O.Mesh.box(2, [32, 32], 0, [5] * 4)
+    vc, _ = om.coords(2)
+    oo = O.Operator(1, om, 2, np.ones(vc.shape[:2]), None)
+    threads = min(os.cpu_count() or 1, 16)
+    xv = x.cpu().numpy()
+    for _ in range(10):
+        oo.apply(xv, threads)
+    ts = []
+    for _ in range(100):
+        t0 = time.perf_counter()
+        oo.apply(xv, threads)
+        ts.append(time.perf_counter() - t0)
+    out["cpu_apply_us_mean"] = float(np.mean(ts)) * 1e6
+    out["cpu_apply_us_min"] = float(np.min(ts)) * 1e6
+    out["cpu_threads"] = threads
+    A = LinearOperator((oo.n, oo.n), matvec=lambda v: oo.apply(np.ascontiguousarray(v), threads))
+    t0 = time.perf_counter()
+    its = [0]
+    xc, info = cg(A, b, rtol=1e-10, maxiter=100000, callback=lambda _: its.__setitem__(0, its[0] + 1))
+    out["cpu_solve_s"] = time.perf_counter() - t0
+    out["cpu_solve_iterations"] = its[0]
+    out["cpu_vs_gpu_solution_rel_diff"] = float(np.linalg.norm(xc - xs.cpu().numpy()) / np.linalg.norm(xc))
+    return out
+
+
 def run_adjoint(args):
     """Adjoint gradient of the misfit J(lam, mu) = 1/2 |B u - d|^2 on the P4 tet mesh (c1 size by
     default): B samples the displacement at 2,048 seeded points of the free surface z = 1 (Dirichlet
@@ -560,7 +625,7 @@ def main():
     ap.add_argument("--cells", type=int, default=24)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["apply", "precision", "greens", "adjoint", "multimesh"], default="apply",
+    ap.add_argument("--workload", choices=["apply", "precision", "greens", "adjoint", "multimesh", "poisson"], default="apply",
                     help="apply: the headline line (default); precision: FP32-MFMA vs FP64 solve at --cells "
                          "(40 for the 384,000-element study); greens: batched Green's-function solves; "
                          "adjoint: misfit gradient with finite-difference checks")
@@ -594,7 +659,7 @@ def main():
     if args.workload != "apply":
         os.environ.setdefault("TDG_PROGRESS_SECONDS", "30")  # the long solves report progress
         run = {"precision": run_precision, "greens": run_greens, "adjoint": run_adjoint,
-               "multimesh": run_multimesh}[args.workload]
+               "multimesh": run_multimesh, "poisson": run_poisson}[args.workload]
         with torch.cuda.stream(torch.cuda.Stream()):
             print(json.dumps(run(args)), flush=True)
         return

@@ -1,5 +1,5 @@
 // Element-block Jacobi preconditioner of the DG operator: the inverse of every element's diagonal
-// block A_KK (bs x bs, symmetrised), stored as packed upper triangles in FP64 and FP32.
+// block A_KK (bs x bs), in FP64 and FP32.
 //
 // The blocks are probed with the matrix-free apply on a colouring of the elements (no two elements
 // sharing a facet have the same colour): with x = e_j on every element of one colour and zero
@@ -43,7 +43,7 @@ private:
     void release();
     int bs_ = 0, colours_ = 0;
     int64_t E_ = 0;
-    double* inv64_ = nullptr; // [E][bs (bs + 1) / 2]: upper triangles of the symmetric inverse blocks
+    double* inv64_ = nullptr; // [E][bs][bs] column-major blocks
     float* inv32_ = nullptr;
 };
 

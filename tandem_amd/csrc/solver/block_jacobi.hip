@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kThreads) void gather_columns(double const* __restr
 // In-place inverse of each (symmetrised) SPD block by Gauss-Jordan elimination without pivoting,
 // the block held in LDS; writes the FP64 inverse back and an FP32 copy.
 template <int BSMAX>
-__global__ __launch_bounds__(kThreads) void invert_blocks(double* __restrict__ Dblk, int bs) {
+__global__ __launch_bounds__(kThreads) void invert_blocks(double* __restrict__ Dblk, float* __restrict__ inv32, int bs) {
     __shared__ double A[BSMAX * BSMAX];
     __shared__ double colk[BSMAX];
     const int64_t e = blockIdx.x;
@@ -70,49 +70,33 @@ __global__ __launch_bounds__(kThreads) void invert_blocks(double* __restrict__ D
         }
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < nn; t += kThreads) g[t] = A[t];
-}
-
-// Upper triangle (i <= j) of every symmetric inverse block, packed column by column:
-// B(i, j) = B(j, i) at j (j + 1) / 2 + i.  Halves the bytes every application reads and makes the
-// applied operator exactly symmetric.
-__device__ __forceinline__ int packed_index(int i, int j) { return i <= j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j; }
-
-__global__ __launch_bounds__(kThreads) void pack_blocks(double const* __restrict__ full, double* __restrict__ p64,
-                                                        float* __restrict__ p32, int bs) {
-    const int64_t e = blockIdx.x;
-    const int np = bs * (bs + 1) / 2;
-    double const* g = full + e * int64_t(bs) * bs;
-    for (int t = threadIdx.x; t < np; t += kThreads) {
-        // column j of the packed upper triangle holds rows 0..j
-        int j = int((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
-        while (j * (j + 1) / 2 > t) --j;
-        while ((j + 1) * (j + 2) / 2 <= t) ++j;
-        const int i = t - j * (j + 1) / 2;
-        const double v = g[i + j * bs];
-        p64[e * np + t] = v;
-        p32[e * np + t] = float(v);
+    // symmetrise the inverse (Gauss-Jordan leaves rounding-level asymmetry): CG sees an exactly
+    // symmetric preconditioner
+    float* g32 = inv32 + e * int64_t(bs) * bs;
+    for (int t = threadIdx.x; t < nn; t += kThreads) {
+        const int i = t % bs, j = t / bs;
+        const double v = 0.5 * (A[i + j * bs] + A[j + i * bs]);
+        g[t] = v;
+        g32[t] = float(v);
     }
 }
 
-// z_e = inv_e r_e, one workgroup per element: the packed block and r_e staged in LDS, lane i
-// computes row i
-template <class T, int BSMAX>
-__global__ __launch_bounds__(128) void apply_blocks(T const* __restrict__ inv, T const* __restrict__ rs,
-                                                    T* __restrict__ zs, int bs) {
-    __shared__ T P[BSMAX * (BSMAX + 1) / 2];
-    __shared__ T r[BSMAX];
+// z_e = inv_e r_e, one workgroup per element (column-major blocks: lanes read consecutive rows)
+template <class T>
+__global__ __launch_bounds__(128) void apply_blocks(T const* __restrict__ inv, T const* __restrict__ rs_, T* __restrict__ zs,
+                                                    int bs, int64_t ld) {
+    __shared__ T rs[128];
     const int64_t e = blockIdx.x;
-    const int np = bs * (bs + 1) / 2;
-    T const* Pg = inv + e * int64_t(np);
-    for (int t = threadIdx.x; t < np; t += 128) P[t] = Pg[t];
-    for (int t = threadIdx.x; t < bs; t += 128) r[t] = rs[e * bs + t];
-    __syncthreads();
     const int i = threadIdx.x;
+    T const* __restrict__ r = rs_ + int64_t(blockIdx.y) * ld;
+    T* __restrict__ z = zs + int64_t(blockIdx.y) * ld;
+    if (i < bs) rs[i] = r[e * bs + i];
+    __syncthreads();
     if (i >= bs) return;
+    T const* B = inv + e * int64_t(bs) * bs;
     T acc = T(0);
-    for (int jj = 0; jj < bs; ++jj) acc += P[packed_index(i, jj)] * r[jj];
-    zs[e * bs + i] = acc;
+    for (int j = 0; j < bs; ++j) acc += B[i + j * bs] * rs[j];
+    z[e * bs + i] = acc;
 }
 
 // Several columns: one workgroup per element holds the element's inverse block in LDS and applies
@@ -122,12 +106,12 @@ constexpr int kCols = 16; // columns staged per pass
 template <class T, int BSMAX>
 __global__ __launch_bounds__(256) void apply_blocks_cols(T const* __restrict__ inv, T const* __restrict__ rs,
                                                          T* __restrict__ zs, int bs, int ncols, int64_t ld) {
-    __shared__ T B[BSMAX * (BSMAX + 1) / 2];
+    __shared__ T B[BSMAX * BSMAX];
     __shared__ T R[kCols * BSMAX];
     const int64_t e = blockIdx.x;
-    const int np = bs * (bs + 1) / 2;
-    T const* Bg = inv + e * int64_t(np);
-    for (int t = threadIdx.x; t < np; t += 256) B[t] = Bg[t];
+    T const* Bg = inv + e * int64_t(bs) * bs;
+    const int nn = bs * bs;
+    for (int t = threadIdx.x; t < nn; t += 256) B[t] = Bg[t];
     for (int c0 = 0; c0 < ncols; c0 += kCols) {
         const int nc = min(kCols, ncols - c0);
         __syncthreads(); // B is loaded / the previous pass's reads of R are done
@@ -143,7 +127,7 @@ __global__ __launch_bounds__(256) void apply_blocks_cols(T const* __restrict__ i
             T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
             T const* r0 = R + (4 * q) * bs;
             for (int jj = 0; jj < bs; ++jj) {
-                const T b = B[packed_index(i, jj)];
+                const T b = B[i + jj * bs];
                 a0 += b * r0[jj];
                 a1 += b * r0[jj + bs];
                 a2 += b * r0[jj + 2 * bs];
@@ -206,15 +190,13 @@ void BlockJacobi::build(Operator& op) {
     int nc = 0;
     std::vector<int32_t> col = colour_elements(op, &nc);
     int32_t* dcol = nullptr;
-    double *X = nullptr, *Y = nullptr, *full = nullptr;
-    const int64_t np = int64_t(bs) * (bs + 1) / 2;
+    double *X = nullptr, *Y = nullptr;
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(bs, (int64_t(1) << 30) / (n * 8)));
     try {
         BHIP(hipMalloc(reinterpret_cast<void**>(&dcol), col.size() * sizeof(int32_t)));
         BHIP(hipMemcpy(dcol, col.data(), col.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        BHIP(hipMalloc(reinterpret_cast<void**>(&full), std::size_t(E) * bs * bs * sizeof(double)));
-        BHIP(hipMalloc(reinterpret_cast<void**>(&inv64_), std::size_t(E) * np * sizeof(double)));
-        BHIP(hipMalloc(reinterpret_cast<void**>(&inv32_), std::size_t(E) * np * sizeof(float)));
+        BHIP(hipMalloc(reinterpret_cast<void**>(&inv64_), std::size_t(E) * bs * bs * sizeof(double)));
+        BHIP(hipMalloc(reinterpret_cast<void**>(&inv32_), std::size_t(E) * bs * bs * sizeof(float)));
         BHIP(hipMalloc(reinterpret_cast<void**>(&X), std::size_t(chunk) * n * sizeof(double)));
         BHIP(hipMalloc(reinterpret_cast<void**>(&Y), std::size_t(chunk) * n * sizeof(double)));
         for (int c = 0; c < nc; ++c)
@@ -225,31 +207,27 @@ void BlockJacobi::build(Operator& op) {
                 BHIP(hipGetLastError());
                 op.apply_batched(X, Y, m, n);
                 hipLaunchKernelGGL(gather_columns, dim3(grid_for(m * n)), dim3(kThreads), 0, st, Y, dcol, c, c0, m, n,
-                                   bs, full);
+                                   bs, inv64_);
                 BHIP(hipGetLastError());
             }
         if (bs <= 32)
-            hipLaunchKernelGGL((invert_blocks<32>), dim3(unsigned(E)), dim3(kThreads), 0, st, full, bs);
+            hipLaunchKernelGGL((invert_blocks<32>), dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, inv32_, bs);
         else if (bs <= 64)
-            hipLaunchKernelGGL((invert_blocks<64>), dim3(unsigned(E)), dim3(kThreads), 0, st, full, bs);
+            hipLaunchKernelGGL((invert_blocks<64>), dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, inv32_, bs);
         else
-            hipLaunchKernelGGL((invert_blocks<112>), dim3(unsigned(E)), dim3(kThreads), 0, st, full, bs);
-        BHIP(hipGetLastError());
-        hipLaunchKernelGGL(pack_blocks, dim3(unsigned(E)), dim3(kThreads), 0, st, full, inv64_, inv32_, bs);
+            hipLaunchKernelGGL((invert_blocks<112>), dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, inv32_, bs);
         BHIP(hipGetLastError());
         BHIP(hipStreamSynchronize(st));
     } catch (...) {
         if (dcol) (void)hipFree(dcol);
         if (X) (void)hipFree(X);
         if (Y) (void)hipFree(Y);
-        if (full) (void)hipFree(full);
         release();
         throw;
     }
     (void)hipFree(dcol);
     (void)hipFree(X);
     (void)hipFree(Y);
-    (void)hipFree(full);
     bs_ = bs;
     E_ = E;
     colours_ = nc;
@@ -259,8 +237,7 @@ namespace {
 template <class T> void launch_apply(T const* inv, T const* r, T* z, int64_t E, int bs, int ncols, int64_t ld,
                                      hipStream_t s) {
     if (ncols <= 1) {
-        if (bs <= 64) hipLaunchKernelGGL((apply_blocks<T, 64>), dim3(unsigned(E)), dim3(128), 0, s, inv, r, z, bs);
-        else hipLaunchKernelGGL((apply_blocks<T, 112>), dim3(unsigned(E)), dim3(128), 0, s, inv, r, z, bs);
+        hipLaunchKernelGGL((apply_blocks<T>), dim3(unsigned(E), 1u), dim3(128), 0, s, inv, r, z, bs, ld);
     } else if (bs <= 64) {
         hipLaunchKernelGGL((apply_blocks_cols<T, 64>), dim3(unsigned(E)), dim3(256), 0, s, inv, r, z, bs, ncols, ld);
     } else {
@@ -283,11 +260,8 @@ void BlockJacobi::apply(float const* r, float* z, hipStream_t s, int ncols, int6
 void BlockJacobi::inverse_block(int64_t e, double* out) const {
     if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
     if (e < 0 || e >= E_) throw std::out_of_range("element index out of range");
-    const int64_t np = int64_t(bs_) * (bs_ + 1) / 2;
-    std::vector<double> p(static_cast<std::size_t>(np));
-    BHIP(hipMemcpy(p.data(), inv64_ + e * np, std::size_t(np) * sizeof(double), hipMemcpyDeviceToHost));
-    for (int j = 0; j < bs_; ++j)
-        for (int i = 0; i <= j; ++i) out[i + j * bs_] = out[j + i * bs_] = p[std::size_t(j * (j + 1) / 2 + i)];
+    BHIP(hipMemcpy(out, inv64_ + e * int64_t(bs_) * bs_, std::size_t(bs_) * bs_ * sizeof(double),
+                   hipMemcpyDeviceToHost));
 }
 
 } // namespace tdg
