@@ -155,6 +155,9 @@ Topology build_topology(SimplexMesh const& m) {
             }
         }
     }
+    for (auto const& kv : bcmap)
+        if (fid.find(kv.first) == fid.end())
+            throw std::invalid_argument("a boundary facet is not a face of any element");
     t.sides.assign(ne * nf, Side{-1, -1, 0, 0, 0});
     for (std::size_t i = 0; i < t.facets.size(); ++i) {
         Facet const& fc = t.facets[i];

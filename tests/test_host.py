@@ -168,3 +168,39 @@ def test_segmented_solve_rejects_coupled_segments():
                                            np.array([0, half + 1, op.n], dtype=np.int64),
                                            C.byref(L.SolveOpts(1e-10, 10, 0, 1e-4, 1, 10, 0.1, 0, 0)),
                                            C.byref(L.SolveResult()), np.zeros(2, dtype=np.int32), np.zeros(2)))
+
+
+def test_more_boundary_errors():
+    """Empty and malformed inputs and calls out of order are reported as error codes with a
+    message, before any device work."""
+    lib = L.lib()
+    m = T.Mesh.box(2, 2, 0, 5)
+    # no elements
+    empty = T.Mesh(m.vertices, np.zeros((0, 3), dtype=np.int64), None, None)
+    with pytest.raises(L.TdgError):
+        T.DGOperator(empty, 0, 2)
+    # a boundary facet that is not a face of any element
+    bad = T.Mesh(m.vertices, m.elements, np.array([[0, 8]]), np.array([5], dtype=np.int32))
+    with pytest.raises(L.TdgError):
+        T.DGOperator(bad, 0, 2)
+    # p-multigrid transfer needs 1 <= from < to
+    with pytest.raises(L.TdgError):
+        T.interpolation_matrix(2, 2, 2)
+    with pytest.raises(L.TdgError):
+        T.interpolation_matrix(3, 2, 0)
+    # operations that need set_material first
+    op = T.DGOperator(m, 0, 2)
+    assert lib.tdg_autotune(op._h, 1, 8, None, np.zeros(3)) == -2
+    assert lib.tdg_inverse_mass(op._h, C.c_void_p(16), C.c_void_p(32)) == -2
+    assert lib.tdg_fault_slip(op._h, C.c_void_p(16), C.c_void_p(32), 1, 0, 0) == -2
+    # the fault adapter is built for elasticity
+    pois = T.DGOperator(m, 1, 2)
+    with pytest.raises(L.TdgError, match="elasticity"):
+        pois.fault_setup([0.0, 1.0], [1.0, 0.0])
+    # the solver rejects a bad column layout before touching the device
+    op.set_material(np.full((op.E, op.Nq), 2.0), np.full((op.E, op.Nq), 1.0))
+    opts = L.SolveOpts(1e-10, 10, 0, 1e-4, 1, 10, 0.1, 0, 0)
+    res = L.SolveResult()
+    rc = lib.tdg_solve_multi(op._h, C.c_void_p(16), C.c_void_p(32), 2, op.n - 1, C.byref(opts), C.byref(res),
+                             np.zeros(2, dtype=np.int32), np.zeros(2))
+    assert rc == -1 and b"bad" in lib.tdg_last_error()
