@@ -24,26 +24,13 @@
 // the compiler can overlap the point arithmetic of one tile with the MFMAs of the next.
 #include "kernels.h"
 #include "mfma_layout.h"
+#include "mfma_ops.h"
 
 #include <type_traits>
 
 namespace tdg {
 
 namespace {
-
-template <class T> struct Mma;
-template <> struct Mma<double> {
-    using acc = double __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-};
-template <> struct Mma<float> {
-    using acc = float __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-};
 
 template <class T> __device__ __forceinline__ T rsqrt_(T v);
 template <> __device__ __forceinline__ double rsqrt_<double>(double v) { return 1.0 / sqrt(v); }
@@ -183,6 +170,15 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
         for (int i = 0; i < 4; ++i) {
             if (LAST && 16 * (QB - 1) + mfma_row<T>(0, i) >= Nq) continue; // compile-time: all padded
             const bool act = valid && 16 * b + mfma_row<T>(g, i) < Nq;
+            // this K step's A operands, read before the point arithmetic they do not depend on
+            T la[D][MM];
+            {
+                const int qa2 = min(16 * b + mfma_row<T>(g, i), Nq - 1) * Nbf;
+#pragma unroll
+                for (int e = 0; e < D; ++e)
+#pragma unroll
+                    for (int m = 0; m < MM; ++m) la[e][m] = sV[e * Nq * Nbf + qa2 + min(16 * m + j, Nbf - 1)];
+            }
             T const* G = pd[i];
             const T lw = pd[i][DD], mw = pd[i][DD + 1];
             T Ju[D][Q]; // Ju[s][r] = du_r/dx_s
@@ -216,7 +212,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
             for (int e = 0; e < D; ++e) {
 #pragma unroll
                 for (int m = 0; m < MM; ++m) {
-                    const T av = sV[e * Nq * Nbf + qr + min(16 * m + j, Nbf - 1)];
+                    const T av = la[e][m];
 #pragma unroll
                     for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, C[e][u][i], Y[m][u]);
                 }
@@ -300,13 +296,17 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
             // lift of tile t's point weights: Y(k) += sum_(q,c) T0(c, iperm_f(k), q) W(q, c)
             auto lift = [&](int t, T const (&W)[D + 1][Q]) {
                 const int qc = min(4 * t + g, nq - 1);
+                // A operands one (c, m) step ahead of their MFMAs
+                T av = sT[qc * Nbf + il[0]];
 #pragma unroll
                 for (int c = 0; c <= D; ++c) {
 #pragma unroll
                     for (int m = 0; m < MM; ++m) {
-                        const T av = sT[(c * nq + qc) * Nbf + il[m]];
+                        const int nx = c * MM + m + 1;
+                        const T avn = nx < (D + 1) * MM ? sT[((nx / MM) * nq + qc) * Nbf + il[nx % MM]] : av;
 #pragma unroll
                         for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, W[c][u], Y[m][u]);
+                        av = avn;
                     }
                     if constexpr (VTAIL) {
 #pragma unroll
@@ -331,23 +331,45 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
 #pragma unroll
                 for (int r = 0; r < Q; ++r) Co[r] = Cn[r] = acc{0, 0, 0, 0};
                 const int ab = (tc * nq + min(4 * t + tq, nq - 1)) * Nbf;
+                if constexpr (STAGE) {
 #pragma unroll
-                for (int s = 0; s < KS; ++s) {
-                    const int k = 4 * s + g;
-                    const T av = sT[ab + min(k, Nbf - 1)];
+                    for (int s = 0; s < KS; ++s) {
+                        const int k = 4 * s + g;
+                        const T av = sT[ab + min(k, Nbf - 1)];
 #pragma unroll
-                    for (int r = 0; r < Q; ++r) {
-                        const bool kin = 4 * s + 3 < Nbf || k < Nbf;
-                        if constexpr (STAGE) {
+                        for (int r = 0; r < Q; ++r) {
+                            const bool kin = 4 * s + 3 < Nbf || k < Nbf;
                             const T v = Uj[(pp[s] & 0xffff) + r * Nbf];
                             const T vn = Nj[(pp[s] >> 16) + r * Nbf];
                             Co[r] = Mma<T>::mma(av, kin ? v : T(0), Co[r]);
                             Cn[r] = Mma<T>::mma(av, kin ? vn : T(0), Cn[r]);
-                        } else {
-                            const T v = Uj[pp[s] + r * Nbf];
-                            Co[r] = Mma<T>::mma(av, kin ? v : T(0), Co[r]);
+                        }
+                    }
+                } else {
+                    // LDS operands one K step ahead of the MFMAs that use them
+                    T av = sT[ab + min(g, Nbf - 1)], bo[Q];
+#pragma unroll
+                    for (int r = 0; r < Q; ++r) bo[r] = Uj[pp[0] + r * Nbf];
+#pragma unroll
+                    for (int s = 0; s < KS; ++s) {
+                        const int k = 4 * s + g;
+                        const bool kin = 4 * s + 3 < Nbf || k < Nbf;
+                        T avn = av, bon[Q];
+#pragma unroll
+                        for (int r = 0; r < Q; ++r) bon[r] = bo[r];
+                        if (s + 1 < KS) {
+                            avn = sT[ab + min(k + 4, Nbf - 1)];
+#pragma unroll
+                            for (int r = 0; r < Q; ++r) bon[r] = Uj[pp[s + 1] + r * Nbf];
+                        }
+#pragma unroll
+                        for (int r = 0; r < Q; ++r) {
+                            Co[r] = Mma<T>::mma(av, kin ? bo[r] : T(0), Co[r]);
                             Cn[r] = Mma<T>::mma(av, Nb[s][r], Cn[r]);
                         }
+                        av = avn;
+#pragma unroll
+                        for (int r = 0; r < Q; ++r) bo[r] = bon[r];
                     }
                 }
                 // this lane: facet point q; register c = value (0) / d/dxi_e (1 + e)
