@@ -601,19 +601,37 @@ def run_adjoint(args):
         log("adjoint J-FD direction", k, fd, an)
     out["J_fd_directions"] = 2
     out["J_fd_max_rel_err"] = max(jerr)
-    # timing of the gradient kernel (FP64 and FP32)
+    # timing of the gradient kernels: matrix-core ("mfma", the default) and thread-per-point, FP64
+    # and FP32; the FP32 gradients' error against the FP64 matrix-core gradient
     op.set_material(lam, mu)
-    for dt, name in ((torch.float64, "fp64"), (torch.float32, "fp32")):
-        uu, pp = u.to(dt), psi.to(dt)
-        op.material_gradient(uu, pp)
-        torch.cuda.synchronize()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        ev[0].record()
-        for _ in range(10):
-            op.material_gradient(uu, pp)
-        ev[1].record()
-        torch.cuda.synchronize()
-        out["gradient_kernel_ms_" + name] = ev[0].elapsed_time(ev[1]) / 10
+    ref = None
+    for variant in ("auto", "points"):
+        op.set_variant(variant)
+        vname = "mfma" if variant == "auto" else variant
+        for dt, name in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+            uu, pp = u.to(dt), psi.to(dt)
+            gl_, gm_ = op.material_gradient(uu, pp)
+            torch.cuda.synchronize()
+            cat = torch.cat([gl_.double(), gm_.double()])
+            if ref is None:
+                ref = cat
+            else:
+                out["gradient_rel_l2_%s_%s_vs_mfma_fp64" % (vname, name)] = float(torch.linalg.norm(cat - ref) /
+                                                                                   torch.linalg.norm(ref))
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            for _ in range(10):
+                op.material_gradient(uu, pp)
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms = ev[0].elapsed_time(ev[1]) / 10
+            out["gradient_kernel_ms_%s_%s" % (vname, name)] = ms
+            # algorithmic bytes: u, psi, J^-1 and W|J| per volume point, M^-1 per element, facet G'
+            # of both sides and the normal, the two outputs
+            E, Nbf, Nq, nq = op.E, op.bs // 3, op.Nq, op.nq
+            nbytes = dt.itemsize * (E * (2 * Nbf * 3 + Nq * 10 + Nbf * Nbf + 2 * Nq) + op.F * nq * (2 * 9 + 3))
+            out["gradient_alg_gbs_%s_%s" % (vname, name)] = nbytes / (ms * 1e-3) / 1e9
+    op.set_variant("auto")
     return out
 
 

@@ -751,7 +751,11 @@ template <class T> void Operator::upload_grad(DeviceGrad<T>& g) {
             pcoef[e * NF + f] = (D + 1) * cN1 * area_[fct] / volume_[e];
         }
     }
+    std::vector<double> tabEt(tabE.size());
+    for (unsigned k = 0; k < Nbf_; ++k)
+        for (unsigned q = 0; q < Nq_; ++q) tabEt[q * Nbf_ + k] = tabE[k * Nq_ + q];
     g.tabE.upload(cast<T>(tabE));
+    g.tabEt.upload(cast<T>(tabEt));
     g.jw.upload(cast<T>(jw));
     g.Minv.upload(cast<T>(Minv));
     g.pc.upload(cast<T>(pc));
@@ -772,6 +776,24 @@ template <class T> void Operator::grad_dispatch(T const* u, T const* psi, T* gl,
         upload_grad(*g);
         slot = std::move(g);
     }
+    GradArgs<T> g;
+    g.tabE = slot->tabE.ptr;
+    g.tabEt = slot->tabEt.ptr;
+    g.jw = slot->jw.ptr;
+    g.Minv = slot->Minv.ptr;
+    g.pc = slot->pc.ptr;
+    g.pinfo = reinterpret_cast<int4 const*>(slot->pinfo.ptr);
+    g.pcoef = slot->pcoef.ptr;
+    int kq = 0, kf = 0;
+    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
+        throw std::runtime_error("no kernel compiled for this dimension/degree");
+    if (uses_mfma()) {
+        MfmaArgs<T> b;
+        if constexpr (std::is_same<T, double>::value) b = mfma_args(*m64_);
+        else b = mfma_args(*m32_);
+        TDG_HIP(launch_material_gradient_mfma<T>(D_, N_, b, g, u, psi, gl, gm, stream_));
+        return;
+    }
     DeviceOperator<T> const& d = device<T>();
     ApplyArgs<T> a;
     a.tabV = d.tabV.ptr;
@@ -791,16 +813,6 @@ template <class T> void Operator::grad_dispatch(T const* u, T const* psi, T* gl,
     a.fpen = d.fpen.ptr;
     a.ef = reinterpret_cast<int4 const*>(d.ef.ptr);
     a.num_elements = int64_t(E_);
-    GradArgs<T> g;
-    g.tabE = slot->tabE.ptr;
-    g.jw = slot->jw.ptr;
-    g.Minv = slot->Minv.ptr;
-    g.pc = slot->pc.ptr;
-    g.pinfo = reinterpret_cast<int4 const*>(slot->pinfo.ptr);
-    g.pcoef = slot->pcoef.ptr;
-    int kq = 0, kf = 0;
-    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
-        throw std::runtime_error("no kernel compiled for this dimension/degree");
     TDG_HIP(launch_material_gradient<T>(D_, N_, a, g, u, psi, gl, gm, stream_));
 }
 
@@ -1090,11 +1102,7 @@ template <> DeviceOperator<float> const& Operator::device<float>() {
     return *d32_;
 }
 
-template <class T>
-void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld) {
-    int kq = 0, kf = 0;
-    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
-        throw std::runtime_error("no kernel compiled for this dimension/degree");
+template <class T> MfmaArgs<T> Operator::mfma_args(DeviceMfma<T> const& d) const {
     MfmaArgs<T> b;
     b.V = d.V.ptr;
     b.T0 = d.T0.ptr;
@@ -1107,7 +1115,15 @@ void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, 
     b.num_elements = int64_t(E_);
     b.num_groups = int64_t((E_ + 15) / 16);
     b.xcd_remap = variant_ == Variant_MfmaXcd ? 1 : 0;
-    TDG_HIP(launch_elasticity_mfma<T>(D_, N_, b, x, y, ncols, ld, stream_));
+    return b;
+}
+
+template <class T>
+void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld) {
+    int kq = 0, kf = 0;
+    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
+        throw std::runtime_error("no kernel compiled for this dimension/degree");
+    TDG_HIP(launch_elasticity_mfma<T>(D_, N_, mfma_args(d), x, y, ncols, ld, stream_));
 }
 
 template <class T> void Operator::dispatch(T const* x, T* y, int ncols, int64_t ld) {

@@ -43,6 +43,8 @@ template <class T> struct DeviceOperator {
     DeviceArray<int32_t> ef;                   // per element face: facet, neighbour, flags
 };
 
+template <class T> struct MfmaArgs;
+
 // Operand tables and element-grouped point data of the MFMA elasticity kernel (one precision).
 template <class T> struct DeviceMfma {
     DeviceArray<T> V, T0, w;       // canonical reference tables, facet weights
@@ -52,7 +54,7 @@ template <class T> struct DeviceMfma {
 
 // Extra data of the material-gradient kernel (one precision).
 template <class T> struct DeviceGrad {
-    DeviceArray<T> tabE, jw, Minv, pc, pcoef;
+    DeviceArray<T> tabE, tabEt, jw, Minv, pc, pcoef;
     DeviceArray<int32_t> pinfo; // int4 per element
 };
 
@@ -177,6 +179,7 @@ private:
     template <class T> void upload(DeviceOperator<T>& d);
     template <class T> void upload_mfma(DeviceMfma<T>& d);
     template <class T> void launch(DeviceOperator<T> const& d, T const* x, T* y, int ncols, int64_t ld);
+    template <class T> MfmaArgs<T> mfma_args(DeviceMfma<T> const& d) const;
     template <class T> void launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld);
     template <class T> void dispatch(T const* x, T* y, int ncols, int64_t ld);
 

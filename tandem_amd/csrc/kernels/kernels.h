@@ -40,6 +40,7 @@ hipError_t launch_poisson(int dim, int degree, ApplyArgs<T> const& a, T const* x
 // Extra device data of the material-gradient (adjoint) kernel, one precision.
 template <class T> struct GradArgs {
     T const* tabE;      // [k][q] basis values at the volume points (= the nodal material basis)
+    T const* tabEt;     // [q][k] the same table transposed (matrix-core operand order)
     T const* jw;        // [E][Nq] W |det J|
     T const* Minv;      // [E][Nbf][Nbf] inverse material mass matrix (column-major)
     T const* pc;        // [E][2] c0 = min 2 mu_i, c1 = max (D lam_i + 2 mu_i) of the nodal coefficients

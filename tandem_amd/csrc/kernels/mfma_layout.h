@@ -68,4 +68,11 @@ template <class T>
 hipError_t launch_elasticity_mfma(int dim, int degree, MfmaArgs<T> const& a, T const* x, T* y, int ncols,
                                  int64_t ld, hipStream_t stream);
 
+template <class T> struct GradArgs;
+// Adjoint material gradient on the matrix cores (same dimensions and degrees as the apply):
+// g_lam, g_mu (E x Nq) = d/d(lam_Q, mu_Q) of psi^T A u.
+template <class T>
+hipError_t launch_material_gradient_mfma(int dim, int degree, MfmaArgs<T> const& a, GradArgs<T> const& g,
+                                         T const* u, T const* psi, T* g_lam, T* g_mu, hipStream_t stream);
+
 } // namespace tdg

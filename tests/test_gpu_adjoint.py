@@ -1,7 +1,8 @@
 """Adjoint gradient kernel (through the C ABI) against the oracle's material gradient.
 
 The oracle's gradient (d/d(lam_Q, mu_Q) of psi^T A u) is pinned by Richardson-extrapolated central
-differences of the oracle's own apply in the CPU tests.  Bar: FP64 relative L2 <= 1e-11; FP32 <= 1e-4.
+differences of the oracle's own apply in the CPU tests.  Bar: FP64 relative L2 <= 1e-11; FP32 <= 1e-4.  Both kernels (matrix-core and thread-per-point) are
+checked; the element counts are not multiples of the 16 elements of one wave.
 """
 import numpy as np
 import pytest
@@ -10,20 +11,24 @@ from cases import Case, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(3, 2, 1, 4, [5] * 6), (3, 2, 0, 2, [5, 1, 1, 5, 3, 1]), (2, 3, 0, 6, [5, 1, 3, 5]), (2, 4, 0, 2, [5] * 4)]
+CASES = [(3, 3, 1, 4, [5] * 6), (3, 2, 0, 2, [5, 1, 1, 5, 3, 1]), (2, 3, 0, 6, [5, 1, 3, 5]), (2, 5, 0, 2, [5] * 4)]
 
 
+@pytest.mark.parametrize("variant", ["auto", "mfma-xcd", "points"])
 @pytest.mark.parametrize("case", CASES, ids=["D%d-n%d-s%d-P%d" % c[:4] for c in CASES])
-def test_material_gradient_matches_oracle(cuda, case):
+def test_material_gradient_matches_oracle(cuda, case, variant):
+    """auto and mfma-xcd run the matrix-core gradient kernel (every case here has one), points the
+    thread-per-point kernel."""
     import torch
 
     D, n, scheme, N, bc = case
     c = Case(D, n, scheme, N, 0, bc=bc, seed=9)
+    c.op.set_variant(variant)
     rng = np.random.default_rng(12)
     u = rng.uniform(-1, 1, c.n)
     psi = rng.uniform(-1, 1, c.n)
     rl, rm = c.oracle.material_gradient(u, psi)
-    print("case", case, flush=True)
+    print("case", case, variant, "elements", c.op.E, flush=True)
     gl, gm = c.op.material_gradient(torch.from_numpy(u).to(cuda), torch.from_numpy(psi).to(cuda))
     el, em = rel_l2(gl.cpu().numpy(), rl), rel_l2(gm.cpu().numpy(), rm)
     print("  fp64 rel L2 lam %.3e mu %.3e" % (el, em), flush=True)
