@@ -14,7 +14,7 @@ of FP64 CG versus FP32 CG with FP64 residual replacement on the same mesh and bo
 and with the element-block Jacobi preconditioner.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-solve] [--no-cpu-baseline]
-    python bench.py --workload precision|greens|adjoint   (the other studies; one JSON line each)
+    python bench.py --workload precision|greens|adjoint|multimesh|poisson|gradient   (the other studies; one JSON line each)
 """
 import argparse
 import json
@@ -601,9 +601,17 @@ def run_adjoint(args):
         log("adjoint J-FD direction", k, fd, an)
     out["J_fd_directions"] = 2
     out["J_fd_max_rel_err"] = max(jerr)
-    # timing of the gradient kernels: matrix-core ("mfma", the default) and thread-per-point, FP64
-    # and FP32; the FP32 gradients' error against the FP64 matrix-core gradient
     op.set_material(lam, mu)
+    time_gradients(op, u, psi, out, 10)
+    return out
+
+
+def time_gradients(op, u, psi, out, reps):
+    """Times the gradient kernels, matrix-core ("mfma", the default) and thread-per-point, in FP64 and
+    FP32 (HIP events on the operator's stream, torch's current one), and records the FP32 gradients'
+    error against the FP64 matrix-core gradient and the algorithmic bandwidth of each."""
+    import torch
+
     ref = None
     for variant in ("auto", "points"):
         op.set_variant(variant)
@@ -620,11 +628,11 @@ def run_adjoint(args):
                                                                                    torch.linalg.norm(ref))
             ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
             ev[0].record()
-            for _ in range(10):
+            for _ in range(reps):
                 op.material_gradient(uu, pp)
             ev[1].record()
             torch.cuda.synchronize()
-            ms = ev[0].elapsed_time(ev[1]) / 10
+            ms = ev[0].elapsed_time(ev[1]) / reps
             out["gradient_kernel_ms_%s_%s" % (vname, name)] = ms
             # algorithmic bytes: u, psi, J^-1 and W|J| per volume point, M^-1 per element, facet G'
             # of both sides and the normal, the two outputs
@@ -632,6 +640,20 @@ def run_adjoint(args):
             nbytes = dt.itemsize * (E * (2 * Nbf * 3 + Nq * 10 + Nbf * Nbf + 2 * Nq) + op.F * nq * (2 * 9 + 3))
             out["gradient_alg_gbs_%s_%s" % (vname, name)] = nbytes / (ms * 1e-3) / 1e9
     op.set_variant("auto")
+
+
+def run_gradient(args):
+    """The adjoint gradient kernels alone on the P4 tet apply workload (seeded random u and psi in
+    [-1, 1]): one evaluation of each kernel and precision, then --steps timed evaluations."""
+    import torch
+
+    op, _, _, _ = build_workload(args.cells)
+    on_stream(op)
+    rng = np.random.default_rng(SEED + 17)
+    u = torch.from_numpy(rng.uniform(-1.0, 1.0, op.n)).cuda()
+    psi = torch.from_numpy(rng.uniform(-1.0, 1.0, op.n)).cuda()
+    out = {"workload": "adjoint gradient kernels, P4 tets, %d^3 Kuhn cubes x 6 = %d elements" % (args.cells, op.E)}
+    time_gradients(op, u, psi, out, args.steps)
     return out
 
 
@@ -643,10 +665,11 @@ def main():
     ap.add_argument("--cells", type=int, default=24)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["apply", "precision", "greens", "adjoint", "multimesh", "poisson"], default="apply",
+    ap.add_argument("--workload", choices=["apply", "precision", "greens", "adjoint", "multimesh", "poisson", "gradient"],
+                    default="apply",
                     help="apply: the headline line (default); precision: FP32-MFMA vs FP64 solve at --cells "
                          "(40 for the 384,000-element study); greens: batched Green's-function solves; "
-                         "adjoint: misfit gradient with finite-difference checks")
+                         "adjoint: misfit gradient with finite-difference checks; gradient: the gradient kernels alone")
     ap.add_argument("--blocks", type=int, default=2, help="greens: number of 64-column blocks timed")
     ap.add_argument("--meshes", type=int, default=32, help="multimesh: number of independent meshes")
     ap.add_argument("--precond", choices=["block-jacobi", "none"], default="block-jacobi",
@@ -677,7 +700,7 @@ def main():
     if args.workload != "apply":
         os.environ.setdefault("TDG_PROGRESS_SECONDS", "30")  # the long solves report progress
         run = {"precision": run_precision, "greens": run_greens, "adjoint": run_adjoint,
-               "multimesh": run_multimesh, "poisson": run_poisson}[args.workload]
+               "multimesh": run_multimesh, "poisson": run_poisson, "gradient": run_gradient}[args.workload]
         with torch.cuda.stream(torch.cuda.Stream()):
             print(json.dumps(run(args)), flush=True)
         return

@@ -56,6 +56,11 @@ char const* tdg_last_error(void);
  * (src/form/AbstractDGOperator.h:26-30).  out[0..6] = block size, elements, local size,
  * facets, volume points per element, facet points per facet, dim. */
 int tdg_sizes(tdg_op const* op, int64_t* out);
+/* The same two sizes one at a time: *bs = block_size() (Nbf x components,
+ * src/form/AbstractDGOperator.h:26), *n = number_of_local_dofs() = block size x elements
+ * (AbstractDGOperator.h:28; the MatShell's local rows and columns, app/common/PetscDGShell.cpp:14-17). */
+int tdg_block_size(tdg_op const* op, int64_t* bs);
+int tdg_local_size(tdg_op const* op, int64_t* n);
 
 /* Coordinates of the volume quadrature points (elements x Nq x dim) and of the facet quadrature
  * points (facets x nq x dim), where tandem evaluates its coefficient and BC functionals

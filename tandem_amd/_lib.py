@@ -49,7 +49,8 @@ class MeshDesc(C.Structure):
 
 
 EXPORTS = [
-    "tdg_create", "tdg_destroy", "tdg_last_error", "tdg_sizes", "tdg_volume_points", "tdg_facet_points",
+    "tdg_create", "tdg_destroy", "tdg_last_error", "tdg_sizes", "tdg_block_size", "tdg_local_size",
+    "tdg_volume_points", "tdg_facet_points",
     "tdg_facets", "tdg_set_material", "tdg_upload", "tdg_mult", "tdg_mult_f32", "tdg_mult_batched", "tdg_mult_batched_f32",
     "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_solve_multi", "tdg_solve_segments", "tdg_autotune", "tdg_rhs", "tdg_rhs_device", "tdg_traction", "tdg_block_jacobi_build", "tdg_block_jacobi_apply", "tdg_block_jacobi_apply_batched", "tdg_block_jacobi_inverse", "tdg_material_gradient",
     "tdg_material_gradient_f32", "tdg_fault_setup", "tdg_fault_slip", "tdg_fault_traction", "tdg_inverse_mass", "tdg_wave_rhs", "tdg_interpolation_matrix", "tdg_interpolate",
@@ -70,6 +71,8 @@ def lib():
     L.tdg_create.argtypes = [C.POINTER(MeshDesc), C.c_int, C.POINTER(vp)]
     L.tdg_destroy.argtypes = [vp]
     L.tdg_sizes.argtypes = [vp, _ip64]
+    L.tdg_block_size.argtypes = [vp, _ip64]
+    L.tdg_local_size.argtypes = [vp, _ip64]
     L.tdg_volume_points.argtypes = [vp, _dp]
     L.tdg_facet_points.argtypes = [vp, _dp]
     L.tdg_facets.argtypes = [vp, _ip64]

@@ -107,6 +107,20 @@ int tdg_sizes(tdg_op const* op, int64_t* out) {
     });
 }
 
+int tdg_block_size(tdg_op const* op, int64_t* bs) {
+    return guarded([&] {
+        require(op && op->impl && bs, "null argument");
+        *bs = int64_t(op->impl->block_size());
+    });
+}
+
+int tdg_local_size(tdg_op const* op, int64_t* n) {
+    return guarded([&] {
+        require(op && op->impl && n, "null argument");
+        *n = int64_t(op->impl->local_size());
+    });
+}
+
 int tdg_volume_points(tdg_op const* op, double* coords) {
     return guarded([&] {
         require(op && op->impl && coords, "null argument");

@@ -64,6 +64,18 @@ def test_mesh_and_precompute_match_oracle(D, n, scheme, N):
         assert np.abs(rb - ro).max() <= 1e-12 * np.abs(ro).max()
 
 
+def test_size_queries_agree():
+    """tdg_block_size / tdg_local_size (the MatShell's block size and local rows) against tdg_sizes."""
+    from tandem_amd._lib import lib
+
+    for D, N, kind, bs in [(3, 4, 0, 105), (2, 6, 0, 56), (2, 2, 1, 6)]:
+        op = T.DGOperator(T.Mesh.box(D, 2, 0, 5), kind, N)
+        b, n = np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int64)
+        assert lib().tdg_block_size(op._h, b) == 0 and lib().tdg_local_size(op._h, n) == 0
+        assert b[0] == op.bs == bs and n[0] == op.n == bs * op.E
+    assert lib().tdg_block_size(None, b) != 0
+
+
 def test_apply_counts_match_closed_form():
     """Algorithmic bytes / flops of the P4 tet apply per element (the roofline's numerator)."""
     m = T.Mesh.box(3, 4, 1, 5)
