@@ -204,8 +204,10 @@ int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block);
  * prepare_volume_post_skeleton :227-253), the facet traces (:189-225) and the penalty
  * (:255-292, differentiated at its extremal nodes).  With A psi = dJ/du it gives
  * dJ/dm = -(g_lam, g_mu) for a misfit J(u(m)) whose right-hand side does not depend on m.
- * Strains are recomputed from u and psi.  No counterpart in the reference (it has no adjoint);
- * the FP64 form is checked against central differences of the reference's apply. */
+ * Strains are recomputed from u and psi.  Runs on the matrix cores wherever the MFMA apply exists
+ * (variants 0 and 2 of tdg_set_variant), thread-per-point with variant 1.  No counterpart in the
+ * reference (it has no adjoint); the FP64 form is checked against central differences of the
+ * FP64 apply along seeded directions. */
 int tdg_material_gradient(tdg_op* op, double const* u, double const* psi, double* g_lam, double* g_mu);
 int tdg_material_gradient_f32(tdg_op* op, float const* u, float const* psi, float* g_lam, float* g_mu);
 

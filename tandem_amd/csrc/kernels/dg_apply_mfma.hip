@@ -111,10 +111,9 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
     // Output rows k of the lift: the first MM tiles of 16 on the matrix cores; when the last tile
     // holds at most 4 real rows (3D P3: 4 of 16, P4: 3 of 16) those rows are summed on the vector
     // ALU instead (each lane over its own K entries, then across the four lane groups), which
-    // removes a quarter of the lift MFMAs that would mostly multiply padding.  (FP64 P4 has no
-    // registers left for the tail accumulators and keeps the matrix-core path.)
+    // removes a third of the lift MFMAs, which would mostly multiply padding.
     constexpr int TAIL = Nbf - 16 * (MT - 1);
-    constexpr bool VTAIL = D == 3 && MT > 1 && TAIL <= 4 && (sizeof(T) == 4 || N < 4); // FP64 P4: no registers left
+    constexpr bool VTAIL = D == 3 && MT > 1 && TAIL <= 4;
     constexpr int MM = VTAIL ? MT - 1 : MT;
     constexpr int NT = VTAIL ? TAIL : 1;
     acc Y[MT][Q];
