@@ -113,7 +113,7 @@ def cpu_baseline(cells=10, reps=10):
 
 
 def traffic_from_profile():
-    p = os.path.join(ROOT, "profiles", "elasticity_mfma_p4_c1_traffic.json")
+    p = os.path.join(ROOT, "profiles", "elasticity_mfma_p4_box24_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -435,7 +435,7 @@ def run_multimesh(args):
 
 
 def run_poisson(args):
-    """c0: 2D Poisson P2 on the unit square split into 32 x 32 x 2 triangles (2,048 elements), K = 1,
+    """2D Poisson P2 on the unit square split into 32 x 32 x 2 triangles (2,048 elements), K = 1,
     u = sin(pi x) sin(pi y) with zero Dirichlet data: one apply on a seeded uniform[-1, 1] vector and
     a CG solve to 1e-10, on the GPU and with the CPU FP64 oracle (apply: warm-up then 100 applies,
     mean and min as in tandem's static.cpp; solve: scipy CG on the oracle's apply)."""
@@ -469,7 +469,7 @@ def run_poisson(args):
     ue, _ = S.poisson_sine(X)
     uh = U @ EQ
     err = float(np.sqrt(np.sum(wj * (uh - ue) ** 2) / np.sum(wj * ue ** 2)))
-    out = {"workload": "c0: Poisson P2, 32x32x2 triangles, %d elements, %d DoF" % (op.E, op.n),
+    out = {"workload": "Poisson P2, 32x32x2 triangles, %d elements, %d DoF" % (op.E, op.n),
            "gpu_apply_us": t_ev * 1e6, "gpu_apply_gdofs": op.n / t_ev * 1e-9,
            "gpu_solve_s": t_solve, "gpu_solve_iterations": int(r["iterations"]),
            "gpu_rel_residual": float(r["rel_residual"]), "l2_error_vs_manufactured": err}
@@ -500,7 +500,7 @@ def run_poisson(args):
 
 
 def run_adjoint(args):
-    """Adjoint gradient of the misfit J(lam, mu) = 1/2 |B u - d|^2 on the P4 tet mesh (c1 size by
+    """Adjoint gradient of the misfit J(lam, mu) = 1/2 |B u - d|^2 on the P4 tet mesh (the 24^3-cube apply workload by
     default): B samples the displacement at 2,048 seeded points of the free surface z = 1 (Dirichlet
     zero on the other faces, seeded smooth body force), d comes from Lame fields perturbed by a
     seeded 10% smooth field.  Gradient per volume quadrature point from the adjoint kernel, checked

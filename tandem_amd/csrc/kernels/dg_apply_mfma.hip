@@ -6,7 +6,8 @@
 //   sV[e][q][k]   = d phi_k / d xi_e at volume point q          (gradient and divergence operand)
 //   sT[c][q][k]   = face-0 value (c = 0) / d/dxi_(c-1) at facet point q (trace and lift operand)
 //   sP            = the facet relabellings perm[f][k] and their inverses,
-//   sU[wave]      = the 16 element vectors of each wave (one contiguous slice of x),
+//   sU[wave]      = the 16 element vectors of each wave (one contiguous slice of x; FP32 pads
+//                   the table rows and the vectors to bank-friendly strides NbfP, BSP),
 // so each MFMA A operand is one LDS read whatever the tile orientation, and B operands of the own
 // element are LDS reads in any relabelled order.
 //
@@ -48,17 +49,24 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
     constexpr int KS = L::KS, QB = L::QB, MT = L::MT, FT = L::FT, NF = D + 1;
     constexpr int NCH = L::NCH, NFC = L::NFC;
     constexpr int nV = D * Nq * Nbf, nT = NF * nq * Nbf;
+    // FP32 LDS strides of the table rows and of the element vectors: twice an odd number, so the
+    // 32 lanes of a half wave that read 16 rows (or elements) at 2 consecutive offsets hit distinct
+    // banks (measured: 30% fewer bank-conflict cycles than the unpadded 35 / 105, while multiples of
+    // 4 such as 36 / 108 add conflicts).  FP64 keeps the unpadded layout.
+    constexpr int NbfP = sizeof(T) == 4 ? (Nbf + 1) / 4 * 4 + 2 : Nbf;
+    constexpr int BSP = sizeof(T) == 4 ? (BS + 1) / 4 * 4 + 2 : BS;
+    constexpr int nVP = D * Nq * NbfP, nTP = NF * nq * NbfP;
 
     // FP32 stages the 16 neighbour vectors of each face in LDS, in the volume table's space once the
     // volume phase is done (contiguous loads; the FP32 kernel has no registers to hold them); FP64
     // keeps them in registers, gathered per lane
     constexpr bool STAGE = sizeof(T) == 4;
-    constexpr int nVN = STAGE && kWaves * 16 * BS > nV ? kWaves * 16 * BS : nV;
+    constexpr int nVN = STAGE && kWaves * 16 * BSP > nVP ? kWaves * 16 * BSP : nVP;
     __shared__ T sVN[nVN];
     T* const sV = sVN;
-    __shared__ T sT[nT];
+    __shared__ T sT[nTP];
     __shared__ int sP[2 * NF * Nbf];
-    __shared__ T sU[kWaves][16 * BS];
+    __shared__ T sU[kWaves][16 * BSP];
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, j = lane & 15, g = lane >> 4;
     // Workgroups are dealt round-robin to the 8 XCDs (workgroup w to XCD w % 8).  Several columns:
@@ -81,12 +89,12 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
 #pragma unroll
     for (int i0 = 0; i0 < nV; i0 += 64 * kWaves) {
         const int i = i0 + tid;
-        if (i < nV) sV[i] = a.V[i];
+        if (i < nV) sV[(i / Nbf) * NbfP + i % Nbf] = a.V[i];
     }
 #pragma unroll
     for (int i0 = 0; i0 < nT; i0 += 64 * kWaves) {
         const int i = i0 + tid;
-        if (i < nT) sT[i] = a.T0[i];
+        if (i < nT) sT[(i / Nbf) * NbfP + i % Nbf] = a.T0[i];
     }
 #pragma unroll
     for (int i0 = 0; i0 < 2 * NF * Nbf; i0 += 64 * kWaves) {
@@ -100,14 +108,14 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
 #pragma unroll
         for (int i0 = 0; i0 < 16 * BS; i0 += 64) {
             const int i = i0 + lane;
-            if (i < 16 * BS) U[i] = i < n ? src[i] : T(0);
+            if (i < 16 * BS) U[(i / BS) * BSP + i % BS] = i < n ? src[i] : T(0);
         }
     }
     __syncthreads();
 
     const bool valid = j < nel;
     const int64_t gj = live ? grp : 0; // point data row of this lane (in bounds for every lane)
-    T const* const Uj = U + j * BS;    // element j: U(k, r) at Uj[k + r Nbf]
+    T const* const Uj = U + j * BSP;    // element j: U(k, r) at Uj[k + r Nbf]
     // Output rows k of the lift: the first MM tiles of 16 on the matrix cores; when the last tile
     // holds at most 4 real rows (3D P3: 4 of 16, P4: 3 of 16) those rows are summed on the vector
     // ALU instead (each lane over its own K entries, then across the four lane groups), which
@@ -145,7 +153,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
         for (int e = 0; e < D; ++e)
 #pragma unroll
             for (int r = 0; r < Q; ++r) C[e][r] = acc{0, 0, 0, 0};
-        const int qa = min(16 * b + j, Nq - 1) * Nbf; // A row: point 16 b + j
+        const int qa = min(16 * b + j, Nq - 1) * NbfP; // A row: point 16 b + j
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int k = 4 * s + g;
@@ -158,7 +166,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
             }
 #pragma unroll
             for (int e = 0; e < D; ++e) {
-                const T av = sV[e * Nq * Nbf + qa + kc];
+                const T av = sV[e * Nq * NbfP + qa + kc];
 #pragma unroll
                 for (int r = 0; r < Q; ++r) C[e][r] = Mma<T>::mma(av, bu[r], C[e][r]);
             }
@@ -172,11 +180,11 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
             // this K step's A operands, read before the point arithmetic they do not depend on
             T la[D][MM];
             {
-                const int qa2 = min(16 * b + mfma_row<T>(g, i), Nq - 1) * Nbf;
+                const int qa2 = min(16 * b + mfma_row<T>(g, i), Nq - 1) * NbfP;
 #pragma unroll
                 for (int e = 0; e < D; ++e)
 #pragma unroll
-                    for (int m = 0; m < MM; ++m) la[e][m] = sV[e * Nq * Nbf + qa2 + min(16 * m + j, Nbf - 1)];
+                    for (int m = 0; m < MM; ++m) la[e][m] = sV[e * Nq * NbfP + qa2 + min(16 * m + j, Nbf - 1)];
             }
             T const* G = pd[i];
             const T lw = pd[i][DD], mw = pd[i][DD + 1];
@@ -206,7 +214,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
                     }
                     C[e][u][i] = act ? v : T(0);
                 }
-            const int qr = min(16 * b + mfma_row<T>(g, i), Nq - 1) * Nbf;
+            const int qr = min(16 * b + mfma_row<T>(g, i), Nq - 1) * NbfP;
 #pragma unroll
             for (int e = 0; e < D; ++e) {
 #pragma unroll
@@ -220,7 +228,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
                     // tail rows (C = 0 on padded points)
 #pragma unroll
                     for (int kt = 0; kt < NT; ++kt) {
-                        const T a = sV[e * Nq * Nbf + qr + 16 * MM + kt];
+                        const T a = sV[e * Nq * NbfP + qr + 16 * MM + kt];
 #pragma unroll
                         for (int u = 0; u < Q; ++u) Yt[kt][u] += a * C[e][u][i];
                     }
@@ -235,7 +243,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
         volume_block(QB - 1, std::true_type{});
     }
     if constexpr (STAGE) __syncthreads(); // every read of the volume table is complete
-    T* const NbU = sVN + (STAGE ? wv * 16 * BS : 0); // this wave's 16 neighbour vectors (FP32)
+    T* const NbU = sVN + (STAGE ? wv * 16 * BSP : 0); // this wave's 16 neighbour vectors (FP32)
     if (live) {
         // -------------------------------------------------------------- facets
         // trace A operand row rho = j: facet point 4 t + mfma_group(rho), channel mfma_reg(rho)
@@ -263,7 +271,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
                     const int d = d0 + lane;
                     const int jj = min(d / BS, 15);
                     const uint32_t off = uint32_t(__shfl(nb32, jj, 64)) * uint32_t(BS) + uint32_t(d - jj * BS);
-                    if (d < 16 * BS) NbU[d] = x[off];
+                    if (d < 16 * BS) NbU[jj * BSP + (d - jj * BS)] = x[off];
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -285,7 +293,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
                     }
                 }
             }
-            T const* const Nj = NbU + j * BS;
+            T const* const Nj = NbU + j * BSP;
             // lift A operand columns: face-0 label of row k = 16 m + j
             int il[MT];
 #pragma unroll
@@ -296,13 +304,13 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
             auto lift = [&](int t, T const (&W)[D + 1][Q]) {
                 const int qc = min(4 * t + g, nq - 1);
                 // A operands one (c, m) step ahead of their MFMAs
-                T av = sT[qc * Nbf + il[0]];
+                T av = sT[qc * NbfP + il[0]];
 #pragma unroll
                 for (int c = 0; c <= D; ++c) {
 #pragma unroll
                     for (int m = 0; m < MM; ++m) {
                         const int nx = c * MM + m + 1;
-                        const T avn = nx < (D + 1) * MM ? sT[((nx / MM) * nq + qc) * Nbf + il[nx % MM]] : av;
+                        const T avn = nx < (D + 1) * MM ? sT[((nx / MM) * nq + qc) * NbfP + il[nx % MM]] : av;
 #pragma unroll
                         for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, W[c][u], Y[m][u]);
                         av = avn;
@@ -310,7 +318,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
                     if constexpr (VTAIL) {
 #pragma unroll
                         for (int kt = 0; kt < NT; ++kt) {
-                            const T a = sT[(c * nq + qc) * Nbf + iperm_f[min(16 * MM + kt, Nbf - 1)]];
+                            const T a = sT[(c * nq + qc) * NbfP + iperm_f[min(16 * MM + kt, Nbf - 1)]];
 #pragma unroll
                             for (int u = 0; u < Q; ++u) Yt[kt][u] += a * W[c][u];
                         }
@@ -329,7 +337,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
                 acc Co[Q], Cn[Q];
 #pragma unroll
                 for (int r = 0; r < Q; ++r) Co[r] = Cn[r] = acc{0, 0, 0, 0};
-                const int ab = (tc * nq + min(4 * t + tq, nq - 1)) * Nbf;
+                const int ab = (tc * nq + min(4 * t + tq, nq - 1)) * NbfP;
                 if constexpr (STAGE) {
 #pragma unroll
                     for (int s = 0; s < KS; ++s) {
@@ -471,7 +479,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
                 const int k = 16 * m + mfma_row<T>(g, i);
                 if (k < Nbf) {
 #pragma unroll
-                    for (int u = 0; u < Q; ++u) U[j * BS + k + u * Nbf] = Y[m][u][i];
+                    for (int u = 0; u < Q; ++u) U[j * BSP + k + u * Nbf] = Y[m][u][i];
                 }
             }
         if constexpr (VTAIL) {
@@ -479,7 +487,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
 #pragma unroll
                 for (int kt = 0; kt < NT; ++kt)
 #pragma unroll
-                    for (int u = 0; u < Q; ++u) U[j * BS + 16 * MM + kt + u * Nbf] = Yt[kt][u];
+                    for (int u = 0; u < Q; ++u) U[j * BSP + 16 * MM + kt + u * Nbf] = Yt[kt][u];
             }
         }
     }
@@ -490,7 +498,7 @@ void elasticity_mfma_kernel(MfmaArgs<T> a, T const* __restrict__ xs, T* __restri
 #pragma unroll
         for (int i0 = 0; i0 < 16 * BS; i0 += 64) {
             const int i = i0 + lane;
-            if (i < n) dst[i] = U[i];
+            if (i < n) dst[i] = U[(i / BS) * BSP + i % BS];
         }
     }
 }

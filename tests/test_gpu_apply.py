@@ -1,6 +1,6 @@
 """Parity of the HIP operator apply (through the C ABI) with the CPU FP64 oracle.
 
-Bar: relative L2 difference <= 1e-12 in FP64 (the north-star tolerance for the plain apply);
+Bar: relative L2 difference <= 1e-12 in FP64 (the parity bar of the plain apply);
 <= 1e-5 for the FP32 kernels (FP32 copies of the FP64 precompute, FP32 arithmetic).
 """
 import numpy as np
@@ -12,14 +12,14 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     # (dim, cells, scheme, degree, kind, bc)
-    (3, 2, 1, 4, 0, BC_D),          # P4 tet elasticity, Kuhn cubes (the c1 element type)
+    (3, 2, 1, 4, 0, BC_D),          # P4 tet elasticity, Kuhn cubes (the benchmark element type)
     (3, 2, 0, 4, 0, [BC_D, BC_N, BC_N, BC_D, BC_D, BC_N]),  # tandem GenMesh 5-tet cubes, mixed BCs
     (3, 2, 1, 1, 0, BC_D),
     (3, 2, 1, 2, 0, BC_D),
     (3, 2, 1, 3, 0, BC_D),
     (2, 4, 0, 2, 0, BC_D),
-    (2, 3, 0, 6, 0, [BC_D, BC_N, BC_D, BC_N]),  # P6 triangles (the c3 element type)
-    (2, 32, 0, 2, 1, BC_D),         # c0: Poisson P2 on the 32x32x2 unit square
+    (2, 3, 0, 6, 0, [BC_D, BC_N, BC_D, BC_N]),  # P6 triangles (the Green's-function element type)
+    (2, 32, 0, 2, 1, BC_D),         # Poisson P2 on the 32x32x2 unit square
     (2, 3, 0, 4, 1, [BC_D, BC_N, BC_N, BC_D]),
     (3, 2, 1, 2, 1, BC_D),
     (3, 2, 0, 4, 1, BC_D),

@@ -30,9 +30,9 @@ def _operator(bc):
 
 
 @pytest.fixture(scope="module")
-def c1(cuda):
+def box24(cuda):
     op, nodes = _operator(T.BC_DIRICHLET)
-    print("c1 operator: %d elements, %d DoF" % (op.E, op.n), flush=True)
+    print("benchmark operator: %d elements, %d DoF" % (op.E, op.n), flush=True)
     return op, nodes
 
 
@@ -40,10 +40,10 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_c1_apply_matches_oracle(cuda, c1):
+def test_full_size_apply_matches_oracle(cuda, box24):
     import torch
 
-    op, nodes = c1
+    op, nodes = box24
     om = O.Mesh.box(3, np.broadcast_to(CELLS, (3,)), 1, [T.BC_DIRICHLET] * 6)
     vc, _ = om.coords(N, nodes)
     lam, mu = S.lame_fields(3, SEED)
@@ -59,15 +59,15 @@ def test_c1_apply_matches_oracle(cuda, c1):
     yf = torch.full((op.n,), float("nan"), dtype=torch.float32, device=cuda)
     op.apply(xd.float(), yf)
     e32 = _rel(yf.double().cpu().numpy(), ref)
-    print("c1 rel L2 vs oracle: fp64 %.3e  fp32 %.3e" % (e64, e32), flush=True)
+    print("full-size rel L2 vs oracle: fp64 %.3e  fp32 %.3e" % (e64, e32), flush=True)
     assert e64 <= 1e-12
     assert e32 <= 1e-5
 
 
-def test_c1_symmetry_linearity_and_batching(cuda, c1):
+def test_full_size_symmetry_linearity_and_batching(cuda, box24):
     import torch
 
-    op, _ = c1
+    op, _ = box24
     rng = np.random.default_rng(SEED + 1)
     X = torch.from_numpy(rng.uniform(-1.0, 1.0, (3, op.n))).to(cuda)
     Y = torch.empty_like(X)
@@ -90,7 +90,7 @@ def test_c1_symmetry_linearity_and_batching(cuda, c1):
     assert torch.equal(YB, Y)
 
 
-def test_c1_rigid_body_null_space(cuda):
+def test_full_size_rigid_body_null_space(cuda):
     import torch
 
     op, nodes = _operator(T.BC_NATURAL)

@@ -34,8 +34,8 @@ def test_cg_matches_dense_solve(cuda):
         assert rel_l2(x, x_ref) <= 1e-7
 
 
-def test_poisson_c0_solve(cuda):
-    """c0: Poisson P2 on the 32x32x2 unit square, u = sin(pi x) sin(pi y), CG to 1e-10."""
+def test_poisson_unit_square_solve(cuda):
+    """Poisson P2 on the 32x32x2 unit square, u = sin(pi x) sin(pi y), CG to 1e-10."""
     import tandem_amd as T
     from tandem_amd import scenario as S
 

@@ -161,7 +161,7 @@ def _l2_error(op, x, exact):
 
 
 def test_poisson_mms_convergence_p2():
-    """c0's problem (K = 1, u = sin(pi x) sin(pi y)) on 4x4x2 .. 16x16x2 P2 triangles:
+    """The unit-square Poisson benchmark problem (K = 1, u = sin(pi x) sin(pi y)) on 4x4x2 .. 16x16x2 P2 triangles:
     L2 order in (N, N+1) = (2, 3) as test_convergence_static.py:70-93 requires."""
     errs = []
     for n in (4, 8, 16):
