@@ -151,8 +151,11 @@ def solve_comparison(op, mesh, nodes, fields, rtol=1e-10):
     Xf = op.facet_points()
     g, _ = S.cosine_solution(Xf)                            # (F, nq, 3)
     bd = torch.empty(op.n, dtype=torch.float64, device="cuda")
-    op.rhs_device(bd, torch.from_numpy(np.ascontiguousarray(f.transpose(0, 2, 1)).ravel()).cuda(),
-                  torch.from_numpy(np.ascontiguousarray(g.transpose(0, 2, 1)).ravel()).cuda())
+    fd = torch.from_numpy(np.ascontiguousarray(f.transpose(0, 2, 1)).ravel()).cuda()
+    gd = torch.from_numpy(np.ascontiguousarray(g.transpose(0, 2, 1)).ravel()).cuda()
+    op.rhs_device(bd, fd, gd)
+    torch.cuda.synchronize()  # fd, gd stay alive until the right-hand side is built
+    del fd, gd
     out = {}
     for pc in (None, "block-jacobi"):
         tag = "none" if pc is None else pc
@@ -728,6 +731,10 @@ def main():
     log("setup %.1fs, %d elements, %d DoF" % (time.perf_counter() - t0, op.E, op.n))
     stream = torch.cuda.Stream()
     op.set_stream(stream)
+    # torch's own work (vector set-up, conversions, temporaries handed to the library) goes on the
+    # operator's stream too: ordered with the library's kernels, and the caching allocator cannot
+    # reuse a temporary while a library kernel on another stream still reads it
+    torch.cuda.set_stream(stream)
     rng = np.random.default_rng(SEED)
     x = torch.from_numpy(rng.uniform(-1.0, 1.0, op.n)).cuda()
     y = torch.zeros_like(x)

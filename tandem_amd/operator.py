@@ -172,7 +172,10 @@ class DGOperator:
         check(lib().tdg_set_variant(self._h, {"auto": 0, "points": 1, "mfma-xcd": 2}[variant]))
 
     def set_stream(self, stream):
-        """stream: a torch.cuda.Stream (or None for the default stream)."""
+        """stream: a torch.cuda.Stream (or None for the default stream).  The library's calls are
+        asynchronous on that stream; make it torch's current stream as well (torch.cuda.set_stream
+        or a `with torch.cuda.stream(...)` block), so that the tensors passed in are produced, and
+        their memory recycled by torch's allocator, in the same stream order."""
         check(lib().tdg_set_stream(self._h, None if stream is None else C.c_void_p(stream.cuda_stream)))
 
     # --- apply
