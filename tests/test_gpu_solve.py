@@ -92,6 +92,7 @@ def test_graph_replayed_solve_equals_launched(cuda, mode, precond):
     out = {}
     for graphs in (False, True):
         x = torch.zeros_like(b)
+        torch.cuda.synchronize()  # b and x are made on the default stream; the solve runs on `stream`
         with torch.cuda.stream(stream):
             r = c.op.solve(b, x, rtol=1e-10, mode=mode, precond=precond, graphs=graphs)
         stream.synchronize()
@@ -118,6 +119,7 @@ def test_graph_replayed_multi_solve_equals_launched(cuda, mode, precond):
     out = {}
     for graphs in (False, True):
         X = torch.zeros_like(B)
+        torch.cuda.synchronize()  # B and X are made on the default stream; the solve runs on `stream`
         with torch.cuda.stream(stream):
             r = c.op.solve_multi(B, X, rtol=1e-10, mode=mode, precond=precond, graphs=graphs)
         stream.synchronize()
@@ -140,6 +142,7 @@ def test_graph_replayed_multi_solve_equals_launched(cuda, mode, precond):
     seg = {}
     for graphs in (False, True):
         x = torch.zeros_like(b)
+        torch.cuda.synchronize()  # b and x are made on the default stream; the solve runs on `stream`
         with torch.cuda.stream(stream):
             r = op.solve_segments(b, x, eoff * op.bs, rtol=1e-10, mode=mode, precond=precond, graphs=graphs)
         stream.synchronize()
