@@ -9,8 +9,8 @@ y = A x of a seeded uniform[-1,1] vector in FP64, inputs resident in HBM; each p
 after 0.3 s of untimed applies (the GPU's clocks ramp up under sustained load) and W warm-up steps.
 
 Prints one JSON line: GDoF/s of the FP64 apply (`value`), the roofline of the apply kernel measured
-with HIP events on its stream, the CPU baseline (the CPU FP64 oracle timed on the host cores on a
-smaller mesh of the same element type), the FP32 apply, and the time to a 1e-10 relative residual
+with HIP events on its stream, the CPU baseline (the CPU FP64 oracle timed on the host cores on the
+same mesh and vector), the FP32 apply, and the time to a 1e-10 relative residual
 of FP64 CG versus FP32 CG with FP64 residual replacement on the same mesh and body force, without
 and with the element-block Jacobi preconditioner.
 
@@ -60,7 +60,8 @@ def build_workload(cells, degree=4, seed=SEED):
     op = T.DGOperator(mesh, T.ELASTICITY, degree, nodes)
     X = op.volume_points()
     lam, mu = S.lame_fields(3, seed)
-    op.set_material(lam(X), mu(X))
+    op.material_Q = (lam(X), mu(X))
+    op.set_material(*op.material_Q)
     return op, mesh, nodes, (lam, mu)
 
 
@@ -97,35 +98,57 @@ def time_applies(op, x, y, steps, warmup, stream):
     return ev[0].elapsed_time(ev[1]) * 1e-3 / steps, wall / steps
 
 
-def cpu_baseline(cells=10, reps=10):
-    """The CPU FP64 oracle (element-centric restatement of tandem's apply) on a Kuhn mesh of
-    cells^3 cubes with the same element type, geometry and coefficients."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mesh, nodes, lam_Q, mu_Q, budget_s=20.0, max_reps=100):
+    """The CPU FP64 oracle (element-centric restatement of tandem's apply) on the same mesh,
+    geometry, coefficients and input vector as the GPU apply, timed as app/static.cpp:94-121 does:
+    one warm-up apply, then up to max_reps applies (as many as fit in budget_s of wall time) on the
+    host's cores, mean and min; plus one apply on a single thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    from tandem_amd import scenario as S
 
     threads = min(os.cpu_count() or 1, 16)
-    om = O.Mesh.box(3, [cells] * 3, 1, [5] * 6)
-    nodes = S.warp(om.affine_nodes(4))
-    vc, _ = om.coords(4, nodes)
-    lam, mu = S.lame_fields(3, SEED)
-    op = O.Operator(0, om, 4, lam(vc), mu(vc), nodes=nodes)
+    t0 = time.perf_counter()
+    om = O.Mesh.from_arrays(mesh.vertices, mesh.elements, mesh.boundary_facets, mesh.boundary_bc)
+    op = O.Operator(0, om, 4, lam_Q, mu_Q, nodes=nodes)
+    log("cpu baseline: oracle set-up %.1fs" % (time.perf_counter() - t0))
     x = np.random.default_rng(SEED).uniform(-1, 1, op.n)
     op.apply(x, threads)
-    t0 = time.perf_counter()
-    for _ in range(reps):
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < max_reps and (len(times) < 3 or time.perf_counter() - t_start < budget_s):
+        t0 = time.perf_counter()
         op.apply(x, threads)
-    tm = (time.perf_counter() - t0) / reps
+        times.append(time.perf_counter() - t0)
     t0 = time.perf_counter()
     op.apply(x, 1)
     t1 = time.perf_counter() - t0
+    tm = float(np.mean(times))
     return {
         "value": op.n / tm * 1e-9,
         "unit": "GDoF/s",
         "cores": threads,
         "kind": "port",
-        "sample": "CPU FP64 oracle apply, P4 tets on %d^3 Kuhn cubes (%d elements, %d DoF), mean of %d applies "
-                  "on %d OpenMP threads; 1 thread: %.4f GDoF/s" % (cells, op.E, op.n, reps, threads, op.n / t1 * 1e-9),
+        "sample": "CPU FP64 oracle apply on the full workload (%d elements, %d DoF, same x): 1 warm-up apply, then %d "
+                  "applies (as many as fit in %.0f s, at most %d) on %d OpenMP threads of %s: mean %.3f s "
+                  "(%.4f GDoF/s), min %.3f s (%.4f GDoF/s); 1 thread, one apply: %.3f s (%.5f GDoF/s)"
+                  % (op.E, op.n, len(times), budget_s, max_reps, threads, cpu_model(), tm, op.n / tm * 1e-9,
+                     min(times), op.n / min(times) * 1e-9, t1, op.n / t1 * 1e-9),
+        "mean_s": tm,
+        "min_s": float(min(times)),
+        "applies": len(times),
+        "one_thread_s": t1,
+        "cpu_model": cpu_model(),
     }
 
 
@@ -696,9 +719,6 @@ def main():
     ap.add_argument("--precond", choices=["block-jacobi", "none"], default="block-jacobi",
                     help="precision: preconditioner of both solves")
     ap.add_argument("--iters", type=int, default=2000, help="greens / multimesh: iteration budget of the solves")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
-                    help="only run a plain device copy kernel (y = -x over BYTES of float64) --steps times; "
-                         "its FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the apply kernel")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the operator is not sharded across GPUs); "
@@ -709,15 +729,6 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU available", file=sys.stderr)
         sys.exit(2)
-    if args.copy_probe:
-        n = args.copy_probe // 8
-        x = torch.ones(n, dtype=torch.float64, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(args.steps):
-            torch.neg(x, out=y)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": args.steps}))
-        return
     if args.workload != "apply":
         os.environ.setdefault("TDG_PROGRESS_SECONDS", "30")  # the long solves report progress
         run = {"precision": run_precision, "greens": run_greens, "adjoint": run_adjoint,
@@ -784,7 +795,7 @@ def main():
     }
     if not args.no_solve:
         res["solve"] = solve_comparison(op, mesh, nodes, fields)
-    res["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline()
+    res["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(mesh, nodes, *op.material_Q)
     print(json.dumps(res), flush=True)
 
 

@@ -225,7 +225,10 @@ int tdg_material_gradient_f32(tdg_op* op, float const* u, float const* psi, floa
  *   the facet traction (Elasticity::traction_skeleton/boundary, app/localoperator/Elasticity.cpp:
  *   949-1018) on the fault facets followed by Adapter<Elasticity>::traction (ElasticityAdapter.cpp:
  *   21-34); with X the solutions for unit fault-state columns, T holds the corresponding columns of
- *   the discrete Green's function (SeasQDDiscreteGreenOperator.cpp:536-545). */
+ *   the discrete Green's function (SeasQDDiscreteGreenOperator.cpp:536-545).
+ * With ncols > 1 every stride must hold one column (ld_s >= nfault*nbf*(dim-1), ld_f >=
+ * facets*dim*nq, ld_x >= local size, ld_t >= nfault*nbf*dim), else TDG_ERR_ARG; with ncols = 1
+ * the strides are ignored. */
 int tdg_fault_setup(tdg_op* op, double const* up, double const* ref_normal, int64_t* sizes);
 int tdg_fault_slip(tdg_op* op, double const* S, double* F, int ncols, int64_t ld_s, int64_t ld_f);
 int tdg_fault_traction(tdg_op* op, double const* X, double const* Fd, double* T, int ncols, int64_t ld_x,

@@ -303,8 +303,9 @@ void DGCommon::init(Mesh const& m, unsigned N_, std::vector<double> const* nodes
     JInv.assign(ne * Nq * DD, 0.0);
     vcoords.assign(ne * Nq * D, 0.0);
     volume.assign(ne, 0.0);
-    std::vector<double> J(std::max(Nq, nq) * DD);
+#pragma omp parallel for schedule(static)
     for (std::size_t el = 0; el < ne; ++el) {
+        std::vector<double> J(Nq * DD);
         cl.jacobian(el, geoDxi_Q, J.data());
         for (std::size_t q = 0; q < Nq; ++q) {
             absDetJ[el * Nq + q] = std::fabs(det(D, &J[q * DD]));
@@ -322,7 +323,9 @@ void DGCommon::init(Mesh const& m, unsigned N_, std::vector<double> const* nodes
     normalLength.assign(nf * nq, 0.0);
     fcoords.assign(nf * nq * D, 0.0);
     area.assign(nf, 0.0);
+#pragma omp parallel for schedule(static)
     for (std::size_t f = 0; f < nf; ++f) {
+        std::vector<double> J(nq * DD);
         auto const& info = topo.fct[f];
         cl.jacobian(info.up[0], geoDxi_q[info.localNo[0]], J.data());
         auto const& rn = cl.refNormals[info.localNo[0]];

@@ -96,6 +96,7 @@ void Elasticity::setup(Mesh const& mesh, unsigned N, std::vector<double> const* 
     lam.assign(ne * Nbf, 0.0);
     mu.assign(ne * Nbf, 0.0);
     JInvT.assign(ne * Nq * DD, 0.0);
+#pragma omp parallel for schedule(static)
     for (std::size_t el = 0; el < ne; ++el) {
         project_coefficient(Nbf, Nq, matE_Q_T, c.volRule.wts.data(), &c.absDetJ[el * Nq],
                             &lam_Q[el * Nq], &lam[el * Nbf]);
@@ -109,6 +110,7 @@ void Elasticity::setup(Mesh const& mesh, unsigned N, std::vector<double> const* 
     mu_q1.assign(nf * nq, 0.0);
     JInvT0.assign(nf * nq * DD, 0.0);
     JInvT1.assign(nf * nq * DD, 0.0);
+#pragma omp parallel for schedule(static)
     for (std::size_t f = 0; f < nf; ++f) {
         auto const& info = c.topo.fct[f];
         bool skel = info.up[0] != info.up[1];
@@ -133,6 +135,7 @@ void Elasticity::setup(Mesh const& mesh, unsigned N, std::vector<double> const* 
     // prepare_volume_post_skeleton (precomputeVolume, elasticity.py:45-49)
     lamWJ.assign(ne * Nq, 0.0);
     muWJ.assign(ne * Nq, 0.0);
+#pragma omp parallel for schedule(static)
     for (std::size_t el = 0; el < ne; ++el)
         for (unsigned q = 0; q < Nq; ++q) {
             double a = 0.0, b = 0.0;

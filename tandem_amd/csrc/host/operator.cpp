@@ -984,6 +984,13 @@ FaultArgs Operator::fault_args() const {
 
 void Operator::fault_slip_device(double const* S, double* F, int ncols, int64_t ld_s, int64_t ld_f) {
     FaultArgs a = fault_args();
+    if (ncols > 1) {
+        // columns must not overlap: each stride holds at least one column
+        if (ld_s < a.nfault * int64_t(a.nbf) * int64_t(D_ - 1))
+            throw std::invalid_argument("ld_s is below one column of fault-basis slips (nfault*nbf*(dim-1))");
+        if (ld_f < int64_t(num_facets()) * int64_t(D_) * nq_)
+            throw std::invalid_argument("ld_f is below one column of facet data (facets*dim*nq)");
+    }
     TDG_HIP(launch_fault_slip(a, S, F, ncols, ld_s, ld_f, stream_));
 }
 
@@ -993,6 +1000,13 @@ void Operator::fault_traction_device(double const* X, double const* Fd, double* 
                                      int64_t ld_f, int64_t ld_t) {
     if (!ready_) throw std::logic_error("set_material() must be called first");
     FaultArgs a = fault_args();
+    if (ncols > 1) {
+        if (ld_x < int64_t(local_size())) throw std::invalid_argument("ld_x is below the local size");
+        if (Fd && ld_f < int64_t(num_facets()) * int64_t(D_) * nq_)
+            throw std::invalid_argument("ld_f is below one column of facet data (facets*dim*nq)");
+        if (ld_t < a.nfault * int64_t(a.nbf) * int64_t(D_))
+            throw std::invalid_argument("ld_t is below one column of fault-basis traction (nfault*nbf*dim)");
+    }
     RhsArgs r = rhs_args();
     const int64_t per = a.nfault * int64_t(D_) * nq_;
     const std::size_t need = std::size_t(per) * std::size_t(ncols);

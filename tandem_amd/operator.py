@@ -94,19 +94,53 @@ def interpolate(I, x, num_quantities, transpose=False, y=None):
     torch = _torch()
     nt, nf = I.shape
     n_in, n_out = (nt, nf) if transpose else (nf, nt)
-    assert I.is_cuda and I.dtype == torch.float64 and I.is_contiguous()
-    assert x.is_cuda and x.dtype == torch.float64 and x.numel() % (n_in * num_quantities) == 0
+    ip = _check(I, "interpolate: I")
+    if x.numel() % (n_in * num_quantities) != 0:
+        raise ValueError("interpolate: x holds no whole number of element blocks")
+    xp = _check(x, "interpolate: x", like=I)
     E = x.numel() // (n_in * num_quantities)
     y = torch.empty(E * num_quantities * n_out, dtype=x.dtype, device=x.device) if y is None else y
+    yp = _check(y, "interpolate: y", numel=E * num_quantities * n_out, like=I)
     stream = torch.cuda.current_stream(x.device).cuda_stream
-    check(lib().tdg_interpolate(C.c_void_p(I.data_ptr()), nt, nf, num_quantities, E, 1 if transpose else 0,
-                                C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), C.c_void_p(stream)))
+    check(lib().tdg_interpolate(ip, nt, nf, num_quantities, E, 1 if transpose else 0, xp, yp, C.c_void_p(stream)))
     return y
 
 
 def _torch():
     import torch
     return torch
+
+
+_DTYPES = {"f64": ("float64",), "f32": ("float32",), "any": ("float64", "float32")}
+
+
+def _check(t, what, dtype="f64", numel=None, min_numel=None, cols=None, like=None):
+    """Validate a device tensor before its pointer goes to the library: dtype, contiguity, size
+    (exact numel, at least min_numel, or a (ncols, >= width) block given as cols=(ncols, width)),
+    GPU residency and the device of `like`.  Raises TypeError / ValueError; returns the pointer."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s: a torch tensor is required" % what)
+    if str(t.dtype).replace("torch.", "") not in _DTYPES[dtype]:
+        raise TypeError("%s: dtype %s, expected %s" % (what, t.dtype, " or ".join(_DTYPES[dtype])))
+    if like is not None and t.dtype != like.dtype:
+        raise TypeError("%s: dtype %s differs from %s" % (what, t.dtype, like.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s: a contiguous tensor is required" % what)
+    if numel is not None and t.numel() != numel:
+        raise ValueError("%s: %d entries, expected %d" % (what, t.numel(), numel))
+    if min_numel is not None and t.numel() < min_numel:
+        raise ValueError("%s: %d entries, expected at least %d" % (what, t.numel(), min_numel))
+    if cols is not None:
+        nc, width = cols
+        if t.dim() != 2 or (nc is not None and t.shape[0] != nc) or t.shape[1] < width:
+            raise ValueError("%s: shape %s, expected (%s, >= %d)" % (what, tuple(t.shape), nc if nc else "ncols",
+                                                                      width))
+    if not t.is_cuda:
+        raise ValueError("%s: a GPU tensor is required" % what)
+    if like is not None and t.device != like.device:
+        raise ValueError("%s: on %s, expected %s" % (what, t.device, like.device))
+    return C.c_void_p(t.data_ptr())
 
 
 class DGOperator:
@@ -182,23 +216,21 @@ class DGOperator:
     def apply(self, x, y):
         """y = A x on GPU tensors (float64 or float32, contiguous, number_of_local_dofs() long)."""
         torch = _torch()
-        assert x.is_cuda and y.is_cuda and x.dtype == y.dtype and x.is_contiguous() and y.is_contiguous()
-        assert x.numel() >= self.n and y.numel() >= self.n
-        if x.dtype == torch.float64:
-            check(lib().tdg_mult(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
-        elif x.dtype == torch.float32:
-            check(lib().tdg_mult_f32(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
-        else:
-            raise TypeError("apply: float64 or float32 tensors required")
+        xp = _check(x, "apply: x", "any", min_numel=self.n)
+        yp = _check(y, "apply: y", "any", min_numel=self.n, like=x)
+        f = lib().tdg_mult if x.dtype == torch.float64 else lib().tdg_mult_f32
+        check(f(self._h, xp, yp))
         return y
 
     def apply_batched(self, X, Y):
         """Y[:, c] = A X[:, c]; X, Y of shape (ncols, ld) row-major (each column contiguous)."""
         torch = _torch()
-        assert X.shape == Y.shape and X.dim() == 2 and X.shape[1] >= self.n
-        assert X.is_contiguous() and Y.is_contiguous() and X.dtype == Y.dtype
+        xp = _check(X, "apply_batched: X", "any", cols=(None, self.n))
+        yp = _check(Y, "apply_batched: Y", "any", cols=(X.shape[0], self.n), like=X)
+        if Y.shape != X.shape:
+            raise ValueError("apply_batched: X and Y shapes differ")
         f = lib().tdg_mult_batched if X.dtype == torch.float64 else lib().tdg_mult_batched_f32
-        check(f(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), X.shape[0], X.shape[1]))
+        check(f(self._h, xp, yp, X.shape[0], X.shape[1]))
         return Y
 
     def upload(self):
@@ -212,12 +244,12 @@ class DGOperator:
         replacement (refinement that keeps the Krylov direction).  precond: None or
         "block-jacobi".  graphs: replay the iterations between residual checks as a captured HIP
         graph when a stream is set.  Returns the statistics."""
-        assert b.is_cuda and x.is_cuda and b.dtype == x.dtype and b.numel() == self.n == x.numel()
+        bp = _check(b, "solve: b", numel=self.n)
+        xp = _check(x, "solve: x", numel=self.n, like=b)
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ir": 1, "fp32-ru": 2}[mode], inner_rtol, max_outer,
                               check_every, reliable_delta, {None: 0, "block-jacobi": 1}[precond], int(graphs))
         res = _lib.SolveResult()
-        check(lib().tdg_solve(self._h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), C.byref(opts),
-                              C.byref(res)))
+        check(lib().tdg_solve(self._h, bp, xp, C.byref(opts), C.byref(res)))
         return {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
 
     def solve_multi(self, B, X, rtol=1e-10, mode="fp64", max_iter=200000, check_every=10, reliable_delta=0.1,
@@ -225,16 +257,17 @@ class DGOperator:
         """Solve A X[c] = B[c] for the rows c of (ncols, ld) float64 GPU tensors together (one batched
         apply per iteration).  mode "fp64" or "fp32-ru".  Returns the statistics with per-column
         iterations and relative residuals."""
-        assert B.is_cuda and X.is_cuda and B.shape == X.shape and B.dim() == 2 and B.shape[1] >= self.n
-        assert B.dtype == X.dtype and B.is_contiguous() and X.is_contiguous()
+        bp = _check(B, "solve_multi: B", cols=(None, self.n))
+        xp = _check(X, "solve_multi: X", cols=(B.shape[0], self.n), like=B)
+        if X.shape != B.shape:
+            raise ValueError("solve_multi: B and X shapes differ")
         nc, ld = B.shape
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ru": 2}[mode], 1e-4, 1, check_every, reliable_delta,
                               {None: 0, "block-jacobi": 1}[precond], int(graphs))
         res = _lib.SolveResult()
         its = np.zeros(nc, dtype=np.int32)
         rel = np.zeros(nc)
-        check(lib().tdg_solve_multi(self._h, C.c_void_p(B.data_ptr()), C.c_void_p(X.data_ptr()), nc, ld,
-                                    C.byref(opts), C.byref(res), its, rel))
+        check(lib().tdg_solve_multi(self._h, bp, xp, nc, ld, C.byref(opts), C.byref(res), its, rel))
         out = {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
         out["column_iterations"] = its
         out["column_rel_residual"] = rel
@@ -253,16 +286,18 @@ class DGOperator:
                        reliable_delta=0.1, precond=None, graphs=True):
         """Independent meshes in one operator: solve each segment [offsets[c], offsets[c+1]) of the
         float64 GPU vectors b, x as its own system, all in lockstep (one apply per iteration)."""
-        assert b.is_cuda and x.is_cuda and b.shape == x.shape and b.numel() == self.n
         off = np.ascontiguousarray(offsets, dtype=np.int64)
+        if off.ndim != 1 or len(off) < 2 or off[0] < 0 or off[-1] > self.n or np.any(np.diff(off) <= 0):
+            raise ValueError("solve_segments: offsets must increase strictly within [0, n]")
+        bp = _check(b, "solve_segments: b", numel=self.n)
+        xp = _check(x, "solve_segments: x", numel=self.n, like=b)
         ns = len(off) - 1
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ru": 2}[mode], 1e-4, 1, check_every, reliable_delta,
                               {None: 0, "block-jacobi": 1}[precond], int(graphs))
         res = _lib.SolveResult()
         its = np.zeros(ns, dtype=np.int32)
         rel = np.zeros(ns)
-        check(lib().tdg_solve_segments(self._h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), ns, off,
-                                       C.byref(opts), C.byref(res), its, rel))
+        check(lib().tdg_solve_segments(self._h, bp, xp, ns, off, C.byref(opts), C.byref(res), its, rel))
         out = {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
         out["segment_iterations"] = its
         out["segment_rel_residual"] = rel
@@ -283,29 +318,30 @@ class DGOperator:
         torch = _torch()
         ncols = 1 if b.dim() == 1 else b.shape[0]
 
-        def view(t, width):
+        def view(t, width, what):
             if t is None:
                 return None, 0
-            assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
             if ncols == 1:
-                assert t.numel() >= width
-                return C.c_void_p(t.data_ptr()), width
-            assert t.dim() == 2 and t.shape[0] == ncols and t.shape[1] >= width
-            return C.c_void_p(t.data_ptr()), t.shape[1]
+                return _check(t, "rhs_device: " + what, min_numel=width, like=b), width
+            return _check(t, "rhs_device: " + what, cols=(ncols, width), like=b), t.shape[1]
 
-        fp, lf = view(force, self.E * self.Q * self.Nq)
-        dp, ld = view(facet, self.F * self.Q * self.nq)
-        bp, lb = view(b, self.n)
+        _check(b, "rhs_device: b")
+        fp, lf = view(force, self.E * self.Q * self.Nq, "force")
+        dp, ld = view(facet, self.F * self.Q * self.nq, "facet")
+        bp, lb = view(b, self.n, "b")
         check(lib().tdg_rhs_device(self._h, fp, dp, bp, ncols, lf, ld, lb))
         return b
 
     def traction(self, x, facet=None, t=None):
         """Facet traction (F, Q, nq) of the GPU vector x (float64) with facet data (may be None)."""
         torch = _torch()
+        width = self.F * self.Q * self.nq
+        xp = _check(x, "traction: x", numel=self.n)
         if t is None:
-            t = torch.empty(self.F * self.Q * self.nq, dtype=torch.float64, device=x.device)
-        fp = None if facet is None else C.c_void_p(facet.data_ptr())
-        check(lib().tdg_traction(self._h, C.c_void_p(x.data_ptr()), fp, C.c_void_p(t.data_ptr())))
+            t = torch.empty(width, dtype=torch.float64, device=x.device)
+        tp = _check(t, "traction: t", numel=width, like=x)
+        fp = None if facet is None else _check(facet, "traction: facet", numel=width, like=x)
+        check(lib().tdg_traction(self._h, xp, fp, tp))
         return t.view(self.F, self.Q, self.nq)
 
     def block_jacobi_build(self):
@@ -316,14 +352,18 @@ class DGOperator:
 
     def block_jacobi_apply(self, r, z):
         """z = D^-1 r (float64 GPU tensors)."""
-        check(lib().tdg_block_jacobi_apply(self._h, C.c_void_p(r.data_ptr()), C.c_void_p(z.data_ptr())))
+        rp = _check(r, "block_jacobi_apply: r", numel=self.n)
+        zp = _check(z, "block_jacobi_apply: z", numel=self.n, like=r)
+        check(lib().tdg_block_jacobi_apply(self._h, rp, zp))
         return z
 
     def block_jacobi_apply_batched(self, R, Z):
         """Z[c] = D^-1 R[c] for the rows c of (ncols, ld) float64 GPU tensors."""
-        assert R.shape == Z.shape and R.dim() == 2 and R.shape[1] >= self.n and R.is_contiguous() and Z.is_contiguous()
-        check(lib().tdg_block_jacobi_apply_batched(self._h, C.c_void_p(R.data_ptr()), C.c_void_p(Z.data_ptr()),
-                                                   R.shape[0], R.shape[1]))
+        rp = _check(R, "block_jacobi_apply_batched: R", cols=(None, self.n))
+        zp = _check(Z, "block_jacobi_apply_batched: Z", cols=(R.shape[0], self.n), like=R)
+        if Z.shape != R.shape:
+            raise ValueError("block_jacobi_apply_batched: R and Z shapes differ")
+        check(lib().tdg_block_jacobi_apply_batched(self._h, rp, zp, R.shape[0], R.shape[1]))
         return Z
 
     def block_jacobi_inverse(self, element):
@@ -335,13 +375,16 @@ class DGOperator:
         """d/d(lam_Q, mu_Q) of psi^T A u for GPU vectors u, psi (float64 or float32); returns two
         (E, Nq) tensors."""
         torch = _torch()
-        assert u.dtype == psi.dtype and u.is_cuda and psi.is_cuda
+        up = _check(u, "material_gradient: u", "any", numel=self.n)
+        pp = _check(psi, "material_gradient: psi", "any", numel=self.n, like=u)
         if g_lam is None:
             g_lam = torch.empty(self.E * self.Nq, dtype=u.dtype, device=u.device)
+        if g_mu is None:
             g_mu = torch.empty_like(g_lam)
+        gl = _check(g_lam, "material_gradient: g_lam", "any", numel=self.E * self.Nq, like=u)
+        gm = _check(g_mu, "material_gradient: g_mu", "any", numel=self.E * self.Nq, like=u)
         f = lib().tdg_material_gradient if u.dtype == torch.float64 else lib().tdg_material_gradient_f32
-        check(f(self._h, C.c_void_p(u.data_ptr()), C.c_void_p(psi.data_ptr()), C.c_void_p(g_lam.data_ptr()),
-                C.c_void_p(g_mu.data_ptr())))
+        check(f(self._h, up, pp, gl, gm))
         return g_lam.view(self.E, self.Nq), g_mu.view(self.E, self.Nq)
 
     def fault_setup(self, up, ref_normal):
@@ -358,11 +401,18 @@ class DGOperator:
         torch = _torch()
         ncols = 1 if S.dim() == 1 else S.shape[0]
         width = self.F * self.dim * self.nq
+        ws = self.nfault * self.fault_nbf * (self.dim - 1)
         if F is None:
             F = torch.empty((ncols, width) if S.dim() == 2 else (width,), dtype=torch.float64, device=S.device)
+        if S.dim() == 2:
+            sp = _check(S, "fault_slip: S", cols=(None, ws))
+            fp = _check(F, "fault_slip: F", cols=(ncols, width), like=S)
+        else:
+            sp = _check(S, "fault_slip: S", numel=ws)
+            fp = _check(F, "fault_slip: F", numel=width, like=S)
         ld_s = S.shape[-1] if S.dim() == 2 else 0
         ld_f = F.shape[-1] if F.dim() == 2 else 0
-        check(lib().tdg_fault_slip(self._h, C.c_void_p(S.data_ptr()), C.c_void_p(F.data_ptr()), ncols, ld_s, ld_f))
+        check(lib().tdg_fault_slip(self._h, sp, fp, ncols, ld_s, ld_f))
         return F
 
     def fault_traction(self, X, Fd=None, T=None):
@@ -371,33 +421,42 @@ class DGOperator:
         torch = _torch()
         ncols = 1 if X.dim() == 1 else X.shape[0]
         width = self.nfault * self.fault_nbf * self.dim
+        wf = self.F * self.dim * self.nq
         if T is None:
             T = torch.empty((ncols, width) if X.dim() == 2 else (width,), dtype=torch.float64, device=X.device)
+        if X.dim() == 2:
+            xp = _check(X, "fault_traction: X", cols=(None, self.n))
+            tp = _check(T, "fault_traction: T", cols=(ncols, width), like=X)
+            fp = None if Fd is None else _check(Fd, "fault_traction: Fd", cols=(ncols, wf), like=X)
+        else:
+            xp = _check(X, "fault_traction: X", numel=self.n)
+            tp = _check(T, "fault_traction: T", numel=width, like=X)
+            fp = None if Fd is None else _check(Fd, "fault_traction: Fd", numel=wf, like=X)
         ld_x = X.shape[-1] if X.dim() == 2 else 0
         ld_f = Fd.shape[-1] if (Fd is not None and Fd.dim() == 2) else 0
         ld_t = T.shape[-1] if T.dim() == 2 else 0
-        fp = None if Fd is None else C.c_void_p(Fd.data_ptr())
-        check(lib().tdg_fault_traction(self._h, C.c_void_p(X.data_ptr()), fp, C.c_void_p(T.data_ptr()), ncols, ld_x,
-                                       ld_f, ld_t))
+        check(lib().tdg_fault_traction(self._h, xp, fp, tp, ncols, ld_x, ld_f, ld_t))
         return T
 
     def inverse_mass(self, x, y=None):
         """y = M^-1 x (weight-adjusted inverse mass with the reference's negative sign) for a float64
         GPU vector x."""
         torch = _torch()
-        assert x.dtype == torch.float64 and x.is_cuda
+        xp = _check(x, "inverse_mass: x", numel=self.n)
         y = torch.empty_like(x) if y is None else y
-        check(lib().tdg_inverse_mass(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
+        yp = _check(y, "inverse_mass: y", numel=self.n, like=x)
+        check(lib().tdg_inverse_mass(self._h, xp, yp))
         return y
 
     def wave_rhs(self, x, facet=None, y=None):
         """y = M^-1 (A x - b_facet(facet)) for a float64 GPU vector x and optional GPU facet data
         (facets x quantities x facet points)."""
         torch = _torch()
-        assert x.dtype == torch.float64 and x.is_cuda
+        xp = _check(x, "wave_rhs: x", numel=self.n)
         y = torch.empty_like(x) if y is None else y
-        fp = None if facet is None else C.c_void_p(facet.data_ptr())
-        check(lib().tdg_wave_rhs(self._h, C.c_void_p(x.data_ptr()), fp, C.c_void_p(y.data_ptr())))
+        yp = _check(y, "wave_rhs: y", numel=self.n, like=x)
+        fp = None if facet is None else _check(facet, "wave_rhs: facet", numel=self.F * self.Q * self.nq, like=x)
+        check(lib().tdg_wave_rhs(self._h, xp, fp, yp))
         return y
 
     def apply_counts(self, precision_bytes=8):

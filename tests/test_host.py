@@ -216,3 +216,47 @@ def test_more_boundary_errors():
     rc = lib.tdg_solve_multi(op._h, C.c_void_p(16), C.c_void_p(32), 2, op.n - 1, C.byref(opts), C.byref(res),
                              np.zeros(2, dtype=np.int32), np.zeros(2))
     assert rc == -1 and b"bad" in lib.tdg_last_error()
+
+
+def test_tensor_arguments_are_checked_before_the_library_is_called():
+    """Every entry point of the Python mirror validates dtype, contiguity, size and residency of its
+    tensors and raises before handing a pointer to the library (a float32 vector given to an FP64
+    entry point would otherwise be read and written as n doubles)."""
+    import torch
+
+    m = T.Mesh.box(2, 2, 0, 5)
+    op = T.DGOperator(m, 0, 2)
+    op.set_material(np.full((op.E, op.Nq), 2.0), np.full((op.E, op.Nq), 1.0))
+    n = op.n
+    f64, f32 = torch.zeros(n, dtype=torch.float64), torch.zeros(n, dtype=torch.float32)
+    strided = torch.zeros(2 * n, dtype=torch.float64)[::2]
+    short = torch.zeros(n - 1, dtype=torch.float64)
+    i64 = torch.zeros(n, dtype=torch.int64)
+    calls = [
+        (TypeError, lambda: op.solve(f32, f32)),
+        (TypeError, lambda: op.solve(f32, f64)),
+        (ValueError, lambda: op.solve(strided, f64)),
+        (ValueError, lambda: op.solve(short, f64)),
+        (ValueError, lambda: op.solve(f64, f64.clone())),  # host memory
+        (TypeError, lambda: op.apply(i64, f64)),
+        (ValueError, lambda: op.apply(strided, f64)),
+        (ValueError, lambda: op.apply(short, f64)),
+        (ValueError, lambda: op.apply_batched(torch.zeros(2, n - 1, dtype=torch.float64), f64)),
+        (TypeError, lambda: op.solve_multi(f32.view(1, n), f32.view(1, n))),
+        (ValueError, lambda: op.solve_multi(torch.zeros(2, n - 1, dtype=torch.float64),
+                                            torch.zeros(2, n - 1, dtype=torch.float64))),
+        (ValueError, lambda: op.solve_segments(f64, f64, [0, n + 1])),
+        (ValueError, lambda: op.solve_segments(f64, f64, [0, 5, 5, n])),
+        (TypeError, lambda: op.solve_segments(f32, f64, [0, n])),
+        (ValueError, lambda: op.block_jacobi_apply(strided, f64)),
+        (TypeError, lambda: op.block_jacobi_apply(f32, f64)),
+        (TypeError, lambda: op.inverse_mass(f32)),
+        (ValueError, lambda: op.inverse_mass(short)),
+        (TypeError, lambda: op.wave_rhs(f32)),
+        (ValueError, lambda: op.traction(short)),
+        (TypeError, lambda: op.material_gradient(i64, f64)),
+        (TypeError, lambda: op.rhs_device(f32)),
+    ]
+    for i, (exc, call) in enumerate(calls):
+        with pytest.raises(exc):
+            call()
