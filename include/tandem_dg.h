@@ -94,20 +94,30 @@ int tdg_mult_batched_f32(tdg_op* op, float const* X, float* Y, int ncols, int64_
 /* Stream all device work of this operator is queued on (hipStream_t; NULL = default stream). */
 int tdg_set_stream(tdg_op* op, void* stream);
 
-/* Kernel choice: TDG_VARIANT_AUTO (default) runs the matrix-core (MFMA) kernel where one is
- * compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6) and the thread-per-point kernel
- * otherwise; TDG_VARIANT_POINTS always runs the thread-per-point kernel; TDG_VARIANT_MFMA_XCD runs
- * the MFMA kernel with its workgroups renumbered so that each XCD takes one contiguous range of
- * elements.  All compute the same operator (the reference's element-centric apply); the two
- * kernels differ in summation order only. */
-enum { TDG_VARIANT_AUTO = 0, TDG_VARIANT_POINTS = 1, TDG_VARIANT_MFMA_XCD = 2 };
+/* Kernel choice: TDG_VARIANT_AUTO (default) runs the two-kernel matrix-core (MFMA) apply where one
+ * is compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6): a facet kernel evaluates each
+ * facet once (both sides' traces, the flux and both lifts) into per-element-face slots, an element
+ * kernel applies the volume term and adds the slots; the thread-per-point kernel otherwise.
+ * TDG_VARIANT_POINTS always runs the thread-per-point kernel.  TDG_VARIANT_MFMA_ELEMENT runs the
+ * element-centric MFMA kernel (each element evaluates both sides of its faces, as the reference's
+ * element loop does), TDG_VARIANT_MFMA_XCD the same kernel with its workgroups renumbered so that
+ * each XCD takes one contiguous range of elements.  All compute the same operator; they differ in
+ * summation order only. */
+enum {
+    TDG_VARIANT_AUTO = 0,
+    TDG_VARIANT_POINTS = 1,
+    TDG_VARIANT_MFMA_XCD = 2,
+    TDG_VARIANT_MFMA_ELEMENT = 3,
+    TDG_NUM_VARIANTS = 4
+};
 int tdg_set_variant(tdg_op* op, int variant);
 
-/* Autotuner: times every variant that applies to the operator (the MFMA kernel with linear and
- * XCD-grouped workgroup order, the thread-per-point kernel) on ncols columns in FP64
- * (precision_bytes 8) or FP32 (4), keeps the fastest as the operator's variant and reports it in
- * *chosen and the milliseconds per apply in ms[variant] (3 entries, -1 where a variant does not
- * apply).  No counterpart in the reference (its kernels are generated for one CPU target). */
+/* Autotuner: times every variant that applies to the operator (the two-kernel MFMA apply, the
+ * element-centric MFMA kernel with linear and XCD-grouped workgroup order, the thread-per-point
+ * kernel) on ncols columns in FP64 (precision_bytes 8) or FP32 (4), keeps the fastest as the
+ * operator's variant and reports it in *chosen and the milliseconds per apply in ms[variant]
+ * (TDG_NUM_VARIANTS entries, -1 where a variant does not apply).  No counterpart in the
+ * reference (its kernels are generated for one CPU target). */
 int tdg_autotune(tdg_op* op, int ncols, int precision_bytes, int* chosen, double* ms);
 
 /* Right-hand side b (host, local size) from a body force at the volume points (elements x Q x Nq,
@@ -261,8 +271,9 @@ int tdg_apply_counts(tdg_op const* op, int precision_bytes, double* bytes, doubl
 
 /* Named FP64 precompute arrays (host copy), for inspection: absDetJ, JInv, vcoords, volume,
  * JInv0, JInv1, normal, normalLength, fcoords, area, penalty, mat0, mat1, c0Q, c1Q, fc0, fc1,
- * E_Q (basis at the volume points, k + Nbf q), E_q0 ... E_q3 (basis at the facet points of local
- * face f, k + Nbf q), volWeights, facetWeights.
+ * E_Q (basis at the volume points, k + Nbf q), Dxi_Q (reference gradients at the volume points,
+ * k + Nbf (e + D q)), E_q0 ... E_q3 (basis at the facet points of local face f, k + Nbf q),
+ * volWeights, facetWeights.
  * Returns the element count in *n; data copied if out != NULL. */
 int tdg_host_array(tdg_op const* op, char const* name, double* out, int64_t* n);
 

@@ -207,8 +207,7 @@ int tdg_set_stream(tdg_op* op, void* stream) {
 int tdg_set_variant(tdg_op* op, int variant) {
     return guarded([&] {
         require(op && op->impl, "null operator");
-        require(variant == TDG_VARIANT_AUTO || variant == TDG_VARIANT_POINTS || variant == TDG_VARIANT_MFMA_XCD,
-                "unknown kernel variant");
+        require(variant >= 0 && variant < TDG_NUM_VARIANTS, "unknown kernel variant");
         op->impl->set_variant(variant);
     });
 }
@@ -310,7 +309,7 @@ int tdg_autotune(tdg_op* op, int ncols, int precision_bytes, int* chosen, double
         auto t = o.autotune(ncols, precision_bytes);
         if (chosen) *chosen = o.variant();
         if (ms)
-            for (int v = 0; v < 3; ++v) ms[v] = t[std::size_t(v)];
+            for (int v = 0; v < TDG_NUM_VARIANTS; ++v) ms[v] = t[std::size_t(v)];
     });
 }
 

@@ -218,6 +218,20 @@ void Operator::build_geometry(std::vector<double> const* geometry_nodes) {
     if (bad) throw std::runtime_error("degenerate element (zero Jacobian determinant)");
     std::size_t F = topo_.facets.size();
     for (int s = 0; s < 2; ++s) fJInv_[s].assign(F * nq_ * DD, 0.0);
+    for (int s = 0; s < 2; ++s) fM_[s].assign(F * nq_ * DD, 0.0);
+    std::vector<std::array<double, 9>> faceJ;
+    for (unsigned f = 0; f <= D; ++f) faceJ.push_back(ref_.face_map_jacobian(f));
+    // M = J J_f: the Jacobian of the element map composed with the reference map carrying facet 0
+    // onto the local facet (columns 0 .. D-2 tangential to the facet, D-1 into the element)
+    auto facet_jacobian = [&](double const* J, int lf, double* M) {
+        auto const& Jf = faceJ[lf];
+        for (unsigned c = 0; c < D; ++c)
+            for (unsigned r = 0; r < D; ++r) {
+                double v = 0.0;
+                for (unsigned k = 0; k < D; ++k) v += J[r + D * k] * Jf[k + D * c];
+                M[r + D * c] = v;
+            }
+    };
     normal_.assign(F * nq_ * D, 0.0);
     nlen_.assign(F * nq_, 0.0);
     fcoords_.assign(F * nq_ * D, 0.0);
@@ -233,6 +247,7 @@ void Operator::build_geometry(std::vector<double> const* geometry_nodes) {
             double dj = det(D, &J[q * DD]);
             double* Ji = &fJInv_[0][(i * nq_ + q) * DD];
             inverse(D, &J[q * DD], Ji);
+            facet_jacobian(&J[q * DD], fc.local[0], &fM_[0][(i * nq_ + q) * DD]);
             // area-scaled normal n = |det J| J^{-T} N_ref (Curvilinear.cpp:256-267)
             double ss = 0.0;
             for (unsigned r = 0; r < D; ++r) {
@@ -248,7 +263,10 @@ void Operator::build_geometry(std::vector<double> const* geometry_nodes) {
         area_[i] = a;
         map(fc.elem[0], gE_q[fc.local[0]], nq_, &fcoords_[i * nq_ * D]);
         jac(fc.elem[1], gD_q[fc.local[1]], nq_, J.data());
-        for (unsigned q = 0; q < nq_; ++q) inverse(D, &J[q * DD], &fJInv_[1][(i * nq_ + q) * DD]);
+        for (unsigned q = 0; q < nq_; ++q) {
+            inverse(D, &J[q * DD], &fJInv_[1][(i * nq_ + q) * DD]);
+            facet_jacobian(&J[q * DD], fc.local[1], &fM_[1][(i * nq_ + q) * DD]);
+        }
     }
 }
 
@@ -377,6 +395,8 @@ void Operator::set_material(double const* c0_Q, double const* c1_Q) {
     d32_.reset();
     m64_.reset();
     m32_.reset();
+    s64_.reset();
+    s32_.reset();
     rhs64_.reset();
     bj_.reset();
     g64_.reset();
@@ -405,11 +425,55 @@ void Operator::upload_device() {
         m64_ = std::move(a);
         m32_ = std::move(b);
     }
+    if (uses_split()) {
+        if (!s64_ || !s32_) {
+            auto a = std::make_unique<DeviceSplit<double>>();
+            auto b = std::make_unique<DeviceSplit<float>>();
+            upload_split(*a);
+            upload_split(*b);
+            s64_ = std::move(a);
+            s32_ = std::move(b);
+        }
+    } else if (uses_mfma()) {
+        ensure_mfma_faces();
+    }
+}
+
+// The element-face records of the element-centric MFMA kernel (also read by the matrix-core
+// gradient kernel), built on first use.
+void Operator::ensure_mfma_faces() {
+    if (!m64_ || !m32_) {
+        auto a = std::make_unique<DeviceMfma<double>>();
+        auto b = std::make_unique<DeviceMfma<float>>();
+        upload_mfma(*a);
+        upload_mfma(*b);
+        m64_ = std::move(a);
+        m32_ = std::move(b);
+    }
+    if (m64_->FD.n == 0) upload_mfma_faces(*m64_);
+    if (m32_->FD.n == 0) upload_mfma_faces(*m32_);
+}
+
+void Operator::reserve_columns(int ncols) {
+    if (ncols < 1) throw std::invalid_argument("reserve_columns: ncols must be positive");
+    upload_device();
+    if (uses_split()) {
+        reserve_split(*s64_, ncols);
+        reserve_split(*s32_, ncols);
+    }
+}
+
+template <class T> void Operator::reserve_split(DeviceSplit<T>& ds, int ncols) {
+    if (ds.cap >= ncols) return;
+    const std::size_t per = E_ * (D_ + 1) * block_size();
+    ds.L.alloc(per * std::size_t(ncols));
+    TDG_HIP(hipMemsetAsync(ds.L.ptr, 0, ds.L.n * sizeof(T), stream_));
+    TDG_HIP(hipStreamSynchronize(stream_));
+    ds.cap = ncols;
 }
 
 void Operator::set_variant(int v) {
-    if (v != Variant_Auto && v != Variant_Points && v != Variant_MfmaXcd)
-        throw std::invalid_argument("unknown kernel variant");
+    if (v < 0 || v >= kNumVariants) throw std::invalid_argument("unknown kernel variant");
     variant_ = v;
 }
 
@@ -417,7 +481,7 @@ std::vector<double> Operator::autotune(int ncols, int pb) {
     if (!ready_) throw std::logic_error("set_material() must be called first");
     if (ncols < 1 || (pb != 8 && pb != 4)) throw std::invalid_argument("bad autotune arguments");
     const int saved = variant_;
-    std::vector<double> ms(3, -1.0);
+    std::vector<double> ms(kNumVariants, -1.0);
     const bool mfma = kind_ == Kind_Elasticity && mfma_supported(D_, N_);
     const std::size_t n = local_size() * std::size_t(ncols);
     void *X = nullptr, *Y = nullptr;
@@ -435,7 +499,7 @@ std::vector<double> Operator::autotune(int ncols, int pb) {
         TDG_HIP(hipEventCreate(&e0));
         TDG_HIP(hipEventCreate(&e1));
         const int64_t ld = int64_t(local_size());
-        for (int v : {int(Variant_Auto), int(Variant_MfmaXcd), int(Variant_Points)}) {
+        for (int v : {int(Variant_Auto), int(Variant_MfmaElement), int(Variant_MfmaXcd), int(Variant_Points)}) {
             if (v != Variant_Points && !mfma) continue;
             variant_ = v;
             upload_device();
@@ -461,7 +525,7 @@ std::vector<double> Operator::autotune(int ncols, int pb) {
     }
     cleanup();
     int best = -1;
-    for (int v = 0; v < 3; ++v)
+    for (int v = 0; v < kNumVariants; ++v)
         if (ms[std::size_t(v)] >= 0.0 && (best < 0 || ms[std::size_t(v)] < ms[std::size_t(best)])) best = v;
     variant_ = best;
     return ms;
@@ -470,6 +534,8 @@ std::vector<double> Operator::autotune(int ncols, int pb) {
 bool Operator::uses_mfma() const {
     return variant_ != Variant_Points && kind_ == Kind_Elasticity && mfma_supported(D_, N_);
 }
+
+bool Operator::uses_split() const { return variant_ == Variant_Auto && uses_mfma(); }
 
 // Tables and element-grouped point data of the MFMA kernel (kernels/dg_apply_lds.hip): canonical
 // reference tables V[e][q][k] and T0[c][q][k] (face 0, c = 0 value, 1 + e gradient), the facet
@@ -490,11 +556,9 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
             for (unsigned k = 0; k < Nbf_; ++k) TT[(c * nq_ + q) * Nbf_ + k] = T0(int(c), int(k), int(q));
     std::vector<int32_t> P2(perm_);
     P2.insert(P2.end(), iperm_.begin(), iperm_.end());
-    // element-grouped point data
+    // element-grouped volume point data
     const std::size_t ng = (E_ + 15) / 16;
-    std::vector<T> VD(ng * Nq_ * L.NCH * 16, T(0)), FP(ng * NF * 16, T(0));
-    std::vector<T> FD(ng * NF * nq_ * L.NFC * 16, T(0));
-    std::vector<int32_t> FI(ng * NF * 16 * 4, 0);
+    std::vector<T> VD(ng * Nq_ * L.NCH * 16, T(0));
 #pragma omp parallel for schedule(static)
     for (std::size_t e = 0; e < E_; ++e) {
         std::size_t grp = e / 16, j = e % 16;
@@ -504,6 +568,27 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
             o[DD * 16] = T(c0Q_[e * Nq_ + q]);
             o[(DD + 1) * 16] = T(c1Q_[e * Nq_ + q]);
         }
+    }
+    dm.V.upload(cast<T>(V));
+    dm.T0.upload(cast<T>(TT));
+    dm.perm2.upload(P2);
+    dm.w.upload(cast<T>(fct_.weights));
+    dm.VD.upload(VD);
+}
+
+// Two-sided element-face records of the element-centric MFMA kernel: per group of 16 elements and
+// local face, both sides' G' = J_f^-1 J^-1 and coefficient traces and the oriented normal (FD), the
+// penalty (FP) and the face info (FI).
+template <class T> void Operator::upload_mfma_faces(DeviceMfma<T>& dm) {
+    const unsigned D = D_, DD = D * D, NF = D + 1;
+    const MfmaDims L{int(D), int(Nbf_), int(Nq_), int(nq_)};
+    const std::size_t ng = (E_ + 15) / 16;
+    std::vector<T> FP(ng * NF * 16, T(0));
+    std::vector<T> FD(ng * NF * nq_ * L.NFC * 16, T(0));
+    std::vector<int32_t> FI(ng * NF * 16 * 4, 0);
+#pragma omp parallel for schedule(static)
+    for (std::size_t e = 0; e < E_; ++e) {
+        std::size_t grp = e / 16, j = e % 16;
         for (unsigned f = 0; f < NF; ++f) {
             int32_t const* ef = &ef_[(e * NF + f) * 4];
             int64_t fct = ef[0];
@@ -530,13 +615,83 @@ template <class T> void Operator::upload_mfma(DeviceMfma<T>& dm) {
         }
     }
     dm.FD.upload(FD);
-    dm.V.upload(cast<T>(V));
-    dm.T0.upload(cast<T>(TT));
-    dm.perm2.upload(P2);
-    dm.w.upload(cast<T>(fct_.weights));
-    dm.VD.upload(VD);
     dm.FP.upload(FP);
     dm.FI.upload(FI);
+}
+
+// Facet records of the two-kernel MFMA apply: the facets that carry a term, two-sided first, then
+// one-sided, each in mesh facet order, in groups of 8 (facet innermost).  Per facet point: the
+// tangent columns of side 0's facet Jacobian M_0 = J_0 J_f0 (shared by both sides: the facet
+// parametrisations of the two sides coincide, vertices being in ascending order), the third
+// columns of M_0 and M_1, and both sides' coefficient traces; per facet the penalty and
+// {element 0, element 1, face 0 | face 1 << 4 | kind << 8}.  Padding lanes get the identity
+// Jacobian so that their (discarded) arithmetic stays finite.
+template <class T> void Operator::upload_split(DeviceSplit<T>& ds) {
+    const unsigned D = D_, DD = D * D, NF = D + 1;
+    const unsigned NFR = DD + D + 4;
+    const std::size_t F = topo_.facets.size();
+    std::vector<int64_t> order;
+    order.reserve(F);
+    for (int pass = 0; pass < 2; ++pass)
+        for (std::size_t i = 0; i < F; ++i) {
+            Facet const& fc = topo_.facets[i];
+            int kind = ef_[(std::size_t(fc.elem[0]) * NF + fc.local[0]) * 4 + 2] & 3;
+            if ((pass == 0 && kind == Face_Skeleton) || (pass == 1 && kind == Face_Boundary))
+                order.push_back(int64_t(i));
+        }
+    std::size_t ntwo = 0;
+    for (int64_t i : order) {
+        Facet const& fc = topo_.facets[std::size_t(i)];
+        if ((ef_[(std::size_t(fc.elem[0]) * NF + fc.local[0]) * 4 + 2] & 3) == Face_Skeleton) ++ntwo;
+    }
+    constexpr std::size_t G = 8; // facets per group: the two sides of 8 facets are the 16 MFMA columns
+    const std::size_t nact = order.size(), ng = (nact + G - 1) / G;
+    std::vector<T> FR(ng * nq_ * NFR * G, T(0)), FPen(ng * G, T(0));
+    std::vector<int32_t> FInfo(ng * G * 4, 0);
+#pragma omp parallel for schedule(static)
+    for (std::size_t p = 0; p < ng * G; ++p) {
+        const std::size_t grp = p / G, j = p % G;
+        for (unsigned q = 0; q < nq_; ++q) {
+            T* o = &FR[((grp * nq_ + q) * NFR) * G + j];
+            if (p >= nact) {
+                for (unsigned d = 0; d < D; ++d) // identity: t_d = e_d, c_0 = c_1 = e_(D-1)
+                    for (unsigned r = 0; r < D; ++r) o[(d * D + r) * G] = T(r == d ? 1.0 : 0.0);
+                for (unsigned r = 0; r < D; ++r) o[(DD + r) * G] = T(r == D - 1 ? 1.0 : 0.0);
+                for (unsigned c = 0; c < 4; ++c) o[(DD + D + c) * G] = T(1.0);
+                continue;
+            }
+            const std::size_t i = std::size_t(order[p]);
+            const bool two = p < ntwo;
+            double const* M0 = &fM_[0][(i * nq_ + q) * DD];
+            double const* M1 = two ? &fM_[1][(i * nq_ + q) * DD] : M0;
+            for (unsigned d = 0; d < D; ++d)
+                for (unsigned r = 0; r < D; ++r) o[(d * D + r) * G] = T(M0[r + d * D]);
+            for (unsigned r = 0; r < D; ++r) o[(DD + r) * G] = T(M1[r + (D - 1) * D]);
+            const int s1 = two ? 1 : 0;
+            o[(DD + D) * G] = T(fc0_[(i * 2 + 0) * nq_ + q]);
+            o[(DD + D + 1) * G] = T(fc1_[(i * 2 + 0) * nq_ + q]);
+            o[(DD + D + 2) * G] = T(fc0_[(i * 2 + s1) * nq_ + q]);
+            o[(DD + D + 3) * G] = T(fc1_[(i * 2 + s1) * nq_ + q]);
+        }
+        if (p < nact) {
+            const std::size_t i = std::size_t(order[p]);
+            Facet const& fc = topo_.facets[i];
+            const bool two = p < ntwo;
+            FPen[p] = T(penalty_[i]);
+            int32_t* fi = &FInfo[p * 4];
+            fi[0] = int32_t(fc.elem[0]);
+            fi[1] = int32_t(two ? fc.elem[1] : fc.elem[0]);
+            fi[2] = fc.local[0] | ((two ? fc.local[1] : fc.local[0]) << 4) | ((two ? 1 : 2) << 8);
+            fi[3] = 0;
+        }
+    }
+    ds.FR.upload(FR);
+    ds.FPen.upload(FPen);
+    ds.FInfo.upload(FInfo);
+    ds.nfacets = int64_t(nact);
+    ds.ntwo = int64_t(ntwo);
+    ds.cap = 0;
+    reserve_split(ds, 1);
 }
 
 template <class T> static std::vector<T> cast(std::vector<double> const& v) {
@@ -788,6 +943,7 @@ template <class T> void Operator::grad_dispatch(T const* u, T const* psi, T* gl,
     if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
         throw std::runtime_error("no kernel compiled for this dimension/degree");
     if (uses_mfma()) {
+        ensure_mfma_faces();
         MfmaArgs<T> b;
         if constexpr (std::is_same<T, double>::value) b = mfma_args(*m64_);
         else b = mfma_args(*m32_);
@@ -1047,6 +1203,7 @@ std::vector<double> const* Operator::host_array(std::string const& n) const {
     if (n == "fc0") return &fc0_;
     if (n == "fc1") return &fc1_;
     if (n == "E_Q") return &E_Q_;
+    if (n == "Dxi_Q") return &Dxi_Q_;
     if (n.size() == 4 && n.compare(0, 3, "E_q") == 0 && n[3] >= '0' && n[3] < char('0' + E_q_.size()))
         return &E_q_[std::size_t(n[3] - '0')];
     if (n == "volWeights") return &vol_.weights;
@@ -1132,6 +1289,24 @@ template <class T> MfmaArgs<T> Operator::mfma_args(DeviceMfma<T> const& d) const
     return b;
 }
 
+template <class T> SplitArgs<T> Operator::split_args(DeviceMfma<T> const& m, DeviceSplit<T> const& d) const {
+    SplitArgs<T> b;
+    b.V = m.V.ptr;
+    b.T0 = m.T0.ptr;
+    b.perm = m.perm2.ptr;
+    b.w = m.w.ptr;
+    b.VD = m.VD.ptr;
+    b.FR = d.FR.ptr;
+    b.FPen = d.FPen.ptr;
+    b.FInfo = reinterpret_cast<int4 const*>(d.FInfo.ptr);
+    b.num_elements = int64_t(E_);
+    b.num_groups = int64_t((E_ + 15) / 16);
+    b.num_facets = d.nfacets;
+    b.num_fgroups = (d.nfacets + 7) / 8;
+    b.num_two = d.ntwo;
+    return b;
+}
+
 template <class T>
 void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld) {
     int kq = 0, kf = 0;
@@ -1143,7 +1318,23 @@ void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, 
 template <class T> void Operator::dispatch(T const* x, T* y, int ncols, int64_t ld) {
     if (!ready_) throw std::logic_error("set_material() must be called before apply");
     upload_device();
-    if (uses_mfma()) {
+    if (uses_split()) {
+        int kq = 0, kf = 0;
+        if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
+            throw std::runtime_error("no kernel compiled for this dimension/degree");
+        auto& ds = [&]() -> DeviceSplit<T>& {
+            if constexpr (std::is_same<T, double>::value) return *s64_;
+            else return *s32_;
+        }();
+        auto const& dm = [&]() -> DeviceMfma<T> const& {
+            if constexpr (std::is_same<T, double>::value) return *m64_;
+            else return *m32_;
+        }();
+        reserve_split(ds, ncols);
+        const int64_t ldL = int64_t(E_) * (D_ + 1) * int64_t(block_size());
+        TDG_HIP(launch_elasticity_split<T>(D_, N_, split_args(dm, ds), x, y, ds.L.ptr, ncols, ncols > 1 ? ld : 0,
+                                           ldL, stream_));
+    } else if (uses_mfma()) {
         if constexpr (std::is_same<T, double>::value) launch_mfma(*m64_, x, y, ncols, ld);
         else launch_mfma(*m32_, x, y, ncols, ld);
     } else {

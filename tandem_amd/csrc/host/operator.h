@@ -44,12 +44,24 @@ template <class T> struct DeviceOperator {
 };
 
 template <class T> struct MfmaArgs;
+template <class T> struct SplitArgs;
 
 // Operand tables and element-grouped point data of the MFMA elasticity kernel (one precision).
 template <class T> struct DeviceMfma {
     DeviceArray<T> V, T0, w;       // canonical reference tables, facet weights
     DeviceArray<T> VD, FD, FP;     // volume point data, two-sided facet records, penalties
     DeviceArray<int32_t> perm2, FI; // perm_f followed by iperm_f; face info
+};
+
+// Facet records, penalties and facet info of the two-kernel (facet + element) MFMA apply, and the
+// buffer of lifted facet contributions per element face (cap columns; cleared when allocated, so the
+// slots of faces without a facet term stay zero).
+template <class T> struct DeviceSplit {
+    DeviceArray<T> FR, FPen;
+    DeviceArray<int32_t> FInfo; // int4 per facet
+    DeviceArray<T> L;
+    int64_t nfacets = 0, ntwo = 0;
+    int cap = 0;
 };
 
 // Extra data of the material-gradient kernel (one precision).
@@ -78,10 +90,12 @@ struct DeviceFault {
     int nbf = 0;
 };
 
-// Kernel selection: Variant_Auto uses the MFMA kernel where one is compiled (elasticity), the
+// Variant_Auto uses the two-kernel (facet + element) MFMA apply where one is compiled and the
 // thread-per-point kernel otherwise; Variant_Points forces the thread-per-point kernel;
-// Variant_MfmaXcd the MFMA kernel with workgroups renumbered per XCD.
-enum Variant : int { Variant_Auto = 0, Variant_Points = 1, Variant_MfmaXcd = 2 };
+// Variant_MfmaXcd / Variant_MfmaElement run the element-centric MFMA kernel (every element
+// evaluates both sides of its faces) with workgroups renumbered per XCD / in linear order.
+enum Variant : int { Variant_Auto = 0, Variant_Points = 1, Variant_MfmaXcd = 2, Variant_MfmaElement = 3 };
+constexpr int kNumVariants = 4;
 
 class BlockJacobi;
 
@@ -123,6 +137,10 @@ public:
     // variant (negative where a variant does not apply).
     std::vector<double> autotune(int ncols, int precision_bytes);
     bool uses_mfma() const;
+    bool uses_split() const;
+    // Allocates the facet-slot buffers of both precisions for ncols columns (done by the apply when
+    // needed; solvers call it before capturing applies into a graph).
+    void reserve_columns(int ncols);
 
     // y = A x on device buffers (x, y of local_size() entries).
     void apply(double const* x, double* y);
@@ -178,6 +196,11 @@ private:
     void build_geometry(std::vector<double> const* geometry_nodes);
     template <class T> void upload(DeviceOperator<T>& d);
     template <class T> void upload_mfma(DeviceMfma<T>& d);
+    template <class T> void upload_mfma_faces(DeviceMfma<T>& d);
+    template <class T> void upload_split(DeviceSplit<T>& d);
+    template <class T> SplitArgs<T> split_args(DeviceMfma<T> const& m, DeviceSplit<T> const& s) const;
+    template <class T> void reserve_split(DeviceSplit<T>& s, int ncols);
+    void ensure_mfma_faces();
     template <class T> void launch(DeviceOperator<T> const& d, T const* x, T* y, int ncols, int64_t ld);
     template <class T> MfmaArgs<T> mfma_args(DeviceMfma<T> const& d) const;
     template <class T> void launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld);
@@ -204,6 +227,7 @@ private:
     // geometry (FP64)
     std::vector<double> absDetJ_, JInv_, vcoords_, volume_;
     std::vector<double> fJInv_[2], normal_, nlen_, fcoords_, area_;
+    std::vector<double> fM_[2];                             // facet Jacobians J J_f per side
 
     // material-dependent (FP64)
     std::vector<double> mat0_, mat1_;                       // projected nodal coefficients
@@ -217,6 +241,8 @@ private:
     std::unique_ptr<DeviceOperator<float>> d32_;
     std::unique_ptr<DeviceMfma<double>> m64_;
     std::unique_ptr<DeviceMfma<float>> m32_;
+    std::unique_ptr<DeviceSplit<double>> s64_;
+    std::unique_ptr<DeviceSplit<float>> s32_;
     std::unique_ptr<DeviceRhs> rhs64_;
     std::unique_ptr<BlockJacobi> bj_;
     std::unique_ptr<DeviceGrad<double>> g64_;

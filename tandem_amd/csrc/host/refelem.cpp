@@ -578,6 +578,14 @@ std::vector<double> RefSimplex::facet_points(unsigned f, std::vector<double> con
     return xi;
 }
 
+std::array<double, 9> RefSimplex::face_map_jacobian(unsigned f) const {
+    const double rv[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    std::array<double, 9> J{};
+    for (unsigned d = 0; d < D; ++d)
+        for (unsigned i = 0; i < D; ++i) J[i + d * D] = rv[faces[f][d + 1]][i] - rv[faces[f][0]][i];
+    return J;
+}
+
 std::array<double, 9> RefSimplex::face_map_jacobian_inverse(unsigned f) const {
     const double rv[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     // vertex map sigma: 0 -> faces[f][0], 1 -> faces[f][1], ..., D -> missing vertex

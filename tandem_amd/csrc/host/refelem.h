@@ -63,7 +63,10 @@ struct RefSimplex {
     explicit RefSimplex(unsigned D);
     std::vector<double> facet_points(unsigned f, std::vector<double> const& chi) const;
     // Affine map Phi_f of the reference simplex onto itself with Phi_f(facet 0 point) = facet f point;
-    // returns its (constant) Jacobian inverse, D x D column-major.
+    // returns its (constant) Jacobian, D x D column-major: columns 0 .. D-2 are the edges of facet f
+    // from its first vertex (tangential), column D-1 points from it to the opposite vertex.
+    std::array<double, 9> face_map_jacobian(unsigned f) const;
+    // ... and the inverse of that Jacobian.
     std::array<double, 9> face_map_jacobian_inverse(unsigned f) const;
 };
 

@@ -68,6 +68,34 @@ template <class T>
 hipError_t launch_elasticity_mfma(int dim, int degree, MfmaArgs<T> const& a, T const* x, T* y, int ncols,
                                  int64_t ld, hipStream_t stream);
 
+// Arguments of the two-kernel (facet + element) MFMA elasticity apply.  Facets are the ones that
+// carry a term (two-sided first, then one-sided), in groups of 8 with the facet innermost:
+//   FR    [fgroup][q][NFR][8]   NFR = D*D + D + 4: the tangent columns t_1 .. t_(D-1) of the facet
+//                               Jacobian M = J J_f of side 0 (shared by both sides), the third
+//                               column c_0 of side 0 and c_1 of side 1, lam_0, mu_0, lam_1, mu_1
+//   FPen  [fgroup][8]           penalty
+//   FInfo [fgroup][8]           {element 0, element 1, face 0 | face 1 << 4 | kind << 8, 0}
+//                               (kind 1 two-sided, 2 one-sided, 0 padding)
+// The element kernel reads the volume table V and point data VD of MfmaArgs, and the lifted facet
+// contributions L[element][face][Nbf * D].
+template <class T> struct SplitArgs {
+    T const* V;
+    T const* T0;
+    int32_t const* perm; // [2][f][k] perm_f, then iperm_f
+    T const* w;
+    T const* VD;
+    T const* FR;
+    T const* FPen;
+    int4 const* FInfo;
+    int64_t num_elements, num_groups, num_facets, num_fgroups, num_two;
+};
+
+// y = A x for ncols columns (stride ld) through the facet kernel (into the slot buffer L, column
+// stride ldL = E (D+1) Nbf D) and the element kernel.
+template <class T>
+hipError_t launch_elasticity_split(int dim, int degree, SplitArgs<T> const& a, T const* x, T* y, T* L, int ncols,
+                                   int64_t ld, int64_t ldL, hipStream_t stream);
+
 template <class T> struct GradArgs;
 // Adjoint material gradient on the matrix cores (same dimensions and degrees as the apply):
 // g_lam, g_mu (E x Nq) = d/d(lam_Q, mu_Q) of psi^T A u.

@@ -184,6 +184,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
     if (o.mode != Solve_FP64_CG && o.mode != Solve_FP32_CG_FP64_RELIABLE)
         throw std::invalid_argument("multi-column solves support FP64 CG and FP32 CG with FP64 replacement");
     op.upload_device();
+    op.reserve_columns(segments ? 1 : ncols); // workspace of the batched apply, outside any capture
     BlockJacobi const* M = nullptr;
     if (o.precond == Precond_BlockJacobi) M = &op.block_jacobi();
     else if (o.precond != Precond_None) throw std::invalid_argument("unknown preconditioner");

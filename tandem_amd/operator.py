@@ -143,6 +143,10 @@ def _check(t, what, dtype="f64", numel=None, min_numel=None, cols=None, like=Non
     return C.c_void_p(t.data_ptr())
 
 
+# kernel variants, indexed by TDG_VARIANT_*
+VARIANTS = ["auto", "points", "mfma-xcd", "mfma-element"]
+
+
 class DGOperator:
     """Matrix-free SIPG operator y = A x on one GPU."""
 
@@ -166,6 +170,7 @@ class DGOperator:
         check(lib().tdg_sizes(h, s))
         self.bs, self.E, self.n, self.F, self.Nq, self.nq, self.dim = (int(x) for x in s)
         self.Q = self.dim if kind == ELASTICITY else 1
+        self.Nbf = self.bs // self.Q
         self._material = False
 
     # --- AbstractDGOperator interface
@@ -201,9 +206,10 @@ class DGOperator:
         self._material = True
 
     def set_variant(self, variant):
-        """"auto" (matrix-core kernel where compiled), "points" (thread-per-point kernel) or
-        "mfma-xcd" (matrix-core kernel, workgroups renumbered so each XCD takes one element range)."""
-        check(lib().tdg_set_variant(self._h, {"auto": 0, "points": 1, "mfma-xcd": 2}[variant]))
+        """"auto" (two-kernel matrix-core apply where compiled: facet kernel + element kernel),
+        "points" (thread-per-point kernel), "mfma-element" (element-centric matrix-core kernel) or
+        "mfma-xcd" (the same, workgroups renumbered so each XCD takes one element range)."""
+        check(lib().tdg_set_variant(self._h, VARIANTS.index(variant)))
 
     def set_stream(self, stream):
         """stream: a torch.cuda.Stream (or None for the default stream).  The library's calls are
@@ -277,10 +283,9 @@ class DGOperator:
         """Time the applicable kernel variants on ncols columns, keep the fastest; returns the chosen
         variant name and the milliseconds per apply of each."""
         chosen = C.c_int()
-        ms = np.zeros(3)
+        ms = np.zeros(len(VARIANTS))
         check(lib().tdg_autotune(self._h, ncols, 8 if precision == "fp64" else 4, C.byref(chosen), ms))
-        names = ["mfma", "points", "mfma-xcd"]  # indexed by TDG_VARIANT_*
-        return names[chosen.value], {names[v]: float(ms[v]) for v in range(3) if ms[v] >= 0}
+        return VARIANTS[chosen.value], {VARIANTS[v]: float(ms[v]) for v in range(len(VARIANTS)) if ms[v] >= 0}
 
     def solve_segments(self, b, x, offsets, rtol=1e-10, mode="fp64", max_iter=200000, check_every=10,
                        reliable_delta=0.1, precond=None, graphs=True):

@@ -30,11 +30,12 @@ def _ids(c):
     return "D%d-n%s-s%d-P%d-%s" % (c[0], c[1], c[2], c[3], "elast" if c[4] == 0 else "poisson")
 
 
-@pytest.mark.parametrize("variant", ["auto", "points", "mfma-xcd"])
+@pytest.mark.parametrize("variant", ["auto", "points", "mfma-xcd", "mfma-element"])
 @pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
 def test_apply_matches_oracle(cuda, case, variant):
-    """variant "auto" runs the matrix-core kernel where one is compiled (elasticity 3D P2-P4, 2D
-    P2/P4/P6), "points" the thread-per-point kernel, "mfma-xcd" the matrix-core kernel with its
+    """variant "auto" runs the two-kernel matrix-core apply (facet kernel + element kernel) where
+    one is compiled (elasticity 3D P2-P4, 2D P2/P4/P6), "points" the thread-per-point kernel,
+    "mfma-element" the element-centric matrix-core kernel and "mfma-xcd" the same with its
     workgroups renumbered per XCD."""
     import torch
 
@@ -107,7 +108,7 @@ def test_autotuned_variant_matches_oracle(cuda, precision):
     c = Case(3, 2, 1, 4, 0, seed=6)
     name, ms = c.op.autotune(ncols=2, precision=precision)
     print("autotune", precision, name, ms, flush=True)
-    assert name in ms and set(ms) == {"mfma", "mfma-xcd", "points"}
+    assert name in ms and set(ms) == {"auto", "mfma-element", "mfma-xcd", "points"}
     assert all(v > 0 for v in ms.values())
     x = np.random.default_rng(7).uniform(-1, 1, c.n)
     ref = c.oracle.apply(x)

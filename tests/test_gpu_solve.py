@@ -2,8 +2,8 @@
 
 Checks: both reach the FP64 true relative residual 1e-10; their solutions agree; the FP64 solution
 matches a dense solve of the oracle's assembled matrix (independent restatement of the bilinear
-form) on a small mesh; and the manufactured cosine solution is recovered with the expected
-convergence order.
+form) on a small mesh.  (The convergence orders of the manufactured cosine solution are checked in
+test_gpu_convergence.py.)
 """
 import numpy as np
 import pytest

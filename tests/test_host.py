@@ -202,7 +202,7 @@ def test_more_boundary_errors():
         T.interpolation_matrix(3, 2, 0)
     # operations that need set_material first
     op = T.DGOperator(m, 0, 2)
-    assert lib.tdg_autotune(op._h, 1, 8, None, np.zeros(3)) == -2
+    assert lib.tdg_autotune(op._h, 1, 8, None, np.zeros(4)) == -2
     assert lib.tdg_inverse_mass(op._h, C.c_void_p(16), C.c_void_p(32)) == -2
     assert lib.tdg_fault_slip(op._h, C.c_void_p(16), C.c_void_p(32), 1, 0, 0) == -2
     # the fault adapter is built for elasticity
