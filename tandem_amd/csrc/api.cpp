@@ -4,6 +4,7 @@
 #include "host/mesh.h"
 #include "host/operator.h"
 #include "host/refelem.h"
+#include "kernels/mfma_layout.h"
 #include "solver/krylov.h"
 #include "solver/block_jacobi.h"
 
@@ -310,6 +311,29 @@ int tdg_autotune(tdg_op* op, int ncols, int precision_bytes, int* chosen, double
         if (chosen) *chosen = o.variant();
         if (ms)
             for (int v = 0; v < TDG_NUM_VARIANTS; ++v) ms[v] = t[std::size_t(v)];
+    });
+}
+
+int tdg_autotune_shapes(tdg_op* op, int ncols, int precision_bytes, int* shape, int* xcd, double* ms) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(ncols >= 1 && (precision_bytes == 8 || precision_bytes == 4), "bad autotune arguments");
+        static_assert(TDG_SPLIT_SHAPES == tdg::kSplitShapes, "shape count of the ABI and the library differ");
+        auto t = o.autotune_shapes(ncols, precision_bytes);
+        if (shape) *shape = o.split_shape(precision_bytes);
+        if (xcd) *xcd = o.split_xcd(precision_bytes);
+        if (ms)
+            for (int c = 0; c < 2 * TDG_SPLIT_SHAPES; ++c) ms[c] = t[std::size_t(c)];
+    });
+}
+
+int tdg_set_shape(tdg_op* op, int precision_bytes, int shape, int xcd) {
+    return guarded([&] {
+        require(op && op->impl, "null operator");
+        require((precision_bytes == 8 || precision_bytes == 4) && shape >= 0 && shape < TDG_SPLIT_SHAPES &&
+                    (xcd == 0 || xcd == 1),
+                "bad launch shape");
+        op->impl->set_split_shape(precision_bytes, shape, xcd);
     });
 }
 

@@ -9,6 +9,8 @@
 // are rounded from it.
 #include "operator.h"
 
+#include <functional>
+
 #include "../kernels/kernels.h"
 #include "../kernels/mfma_layout.h"
 #include "../solver/block_jacobi.h"
@@ -477,12 +479,11 @@ void Operator::set_variant(int v) {
     variant_ = v;
 }
 
-std::vector<double> Operator::autotune(int ncols, int pb) {
-    if (!ready_) throw std::logic_error("set_material() must be called first");
-    if (ncols < 1 || (pb != 8 && pb != 4)) throw std::invalid_argument("bad autotune arguments");
-    const int saved = variant_;
-    std::vector<double> ms(kNumVariants, -1.0);
-    const bool mfma = kind_ == Kind_Elasticity && mfma_supported(D_, N_);
+// Times applies of the operator on ncols zero columns (HIP events on the operator's stream) for each
+// configuration in `configs` (set by `select(i)`), keeps the fastest and returns the milliseconds
+// of each (negative where select returns false).
+double Operator::time_configs(int ncols, int pb, int count, std::function<bool(int)> const& select,
+                              std::vector<double>& ms) {
     const std::size_t n = local_size() * std::size_t(ncols);
     void *X = nullptr, *Y = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -492,6 +493,8 @@ std::vector<double> Operator::autotune(int ncols, int pb) {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
     };
+    ms.assign(std::size_t(count), -1.0);
+    double best = -1.0;
     try {
         TDG_HIP(hipMalloc(&X, n * pb));
         TDG_HIP(hipMalloc(&Y, n * pb));
@@ -499,9 +502,8 @@ std::vector<double> Operator::autotune(int ncols, int pb) {
         TDG_HIP(hipEventCreate(&e0));
         TDG_HIP(hipEventCreate(&e1));
         const int64_t ld = int64_t(local_size());
-        for (int v : {int(Variant_Auto), int(Variant_MfmaElement), int(Variant_MfmaXcd), int(Variant_Points)}) {
-            if (v != Variant_Points && !mfma) continue;
-            variant_ = v;
+        for (int c = 0; c < count; ++c) {
+            if (!select(c)) continue;
             upload_device();
             auto run = [&] {
                 if (pb == 8) dispatch(static_cast<double const*>(X), static_cast<double*>(Y), ncols, ld);
@@ -516,19 +518,75 @@ std::vector<double> Operator::autotune(int ncols, int pb) {
             TDG_HIP(hipEventSynchronize(e1));
             float t = 0.0f;
             TDG_HIP(hipEventElapsedTime(&t, e0, e1));
-            ms[std::size_t(v)] = double(t) / reps;
+            ms[std::size_t(c)] = double(t) / reps;
+            if (best < 0.0 || ms[std::size_t(c)] < best) best = ms[std::size_t(c)];
         }
     } catch (...) {
-        variant_ = saved;
         cleanup();
         throw;
     }
     cleanup();
+    return best;
+}
+
+std::vector<double> Operator::autotune_shapes(int ncols, int pb) {
+    if (!ready_) throw std::logic_error("set_material() must be called first");
+    if (ncols < 1 || (pb != 8 && pb != 4)) throw std::invalid_argument("bad autotune arguments");
+    const int p = pb == 8 ? 0 : 1;
+    std::vector<double> ms;
+    if (!(kind_ == Kind_Elasticity && mfma_supported(D_, N_))) return std::vector<double>(2 * kSplitShapes, -1.0);
+    const int saved_variant = variant_, saved_shape = split_shape_[p], saved_xcd = split_xcd_[p];
+    try {
+        variant_ = Variant_Auto;
+        time_configs(ncols, pb, 2 * kSplitShapes, [&](int c) {
+            split_shape_[p] = c / 2;
+            split_xcd_[p] = c % 2;
+            return true;
+        }, ms);
+    } catch (...) {
+        variant_ = saved_variant;
+        split_shape_[p] = saved_shape;
+        split_xcd_[p] = saved_xcd;
+        throw;
+    }
+    variant_ = saved_variant;
+    int best = 0;
+    for (int c = 1; c < 2 * kSplitShapes; ++c)
+        if (ms[std::size_t(c)] >= 0.0 && ms[std::size_t(c)] < ms[std::size_t(best)]) best = c;
+    split_shape_[p] = best / 2;
+    split_xcd_[p] = best % 2;
+    return ms;
+}
+
+std::vector<double> Operator::autotune(int ncols, int pb) {
+    if (!ready_) throw std::logic_error("set_material() must be called first");
+    if (ncols < 1 || (pb != 8 && pb != 4)) throw std::invalid_argument("bad autotune arguments");
+    const bool mfma = kind_ == Kind_Elasticity && mfma_supported(D_, N_);
+    if (mfma) autotune_shapes(ncols, pb); // the two-kernel apply runs its fastest launch shape
+    const int saved = variant_;
+    std::vector<double> ms;
+    try {
+        time_configs(ncols, pb, kNumVariants, [&](int v) {
+            if (v != Variant_Points && !mfma) return false;
+            variant_ = v;
+            return true;
+        }, ms);
+    } catch (...) {
+        variant_ = saved;
+        throw;
+    }
     int best = -1;
     for (int v = 0; v < kNumVariants; ++v)
         if (ms[std::size_t(v)] >= 0.0 && (best < 0 || ms[std::size_t(v)] < ms[std::size_t(best)])) best = v;
     variant_ = best;
     return ms;
+}
+
+void Operator::set_split_shape(int pb, int shape, int xcd) {
+    if ((pb != 8 && pb != 4) || shape < 0 || shape >= kSplitShapes || (xcd != 0 && xcd != 1))
+        throw std::invalid_argument("bad launch shape");
+    split_shape_[pb == 8 ? 0 : 1] = shape;
+    split_xcd_[pb == 8 ? 0 : 1] = xcd;
 }
 
 bool Operator::uses_mfma() const {
@@ -1304,6 +1362,7 @@ template <class T> SplitArgs<T> Operator::split_args(DeviceMfma<T> const& m, Dev
     b.num_facets = d.nfacets;
     b.num_fgroups = (d.nfacets + 7) / 8;
     b.num_two = d.ntwo;
+    b.xcd_remap = 0;
     return b;
 }
 
@@ -1332,7 +1391,10 @@ template <class T> void Operator::dispatch(T const* x, T* y, int ncols, int64_t 
         }();
         reserve_split(ds, ncols);
         const int64_t ldL = int64_t(E_) * (D_ + 1) * int64_t(block_size());
-        TDG_HIP(launch_elasticity_split<T>(D_, N_, split_args(dm, ds), x, y, ds.L.ptr, ncols, ncols > 1 ? ld : 0,
+        const int p = sizeof(T) == 8 ? 0 : 1;
+        SplitArgs<T> sa = split_args(dm, ds);
+        sa.xcd_remap = split_xcd_[p];
+        TDG_HIP(launch_elasticity_split<T>(D_, N_, split_shape_[p], sa, x, y, ds.L.ptr, ncols, ncols > 1 ? ld : 0,
                                            ldL, stream_));
     } else if (uses_mfma()) {
         if constexpr (std::is_same<T, double>::value) launch_mfma(*m64_, x, y, ncols, ld);

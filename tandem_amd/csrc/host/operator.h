@@ -10,6 +10,7 @@
 
 #include <array>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -136,6 +137,12 @@ public:
     // precision (8 or 4 bytes), keeps the fastest and returns the milliseconds per apply indexed by
     // variant (negative where a variant does not apply).
     std::vector<double> autotune(int ncols, int precision_bytes);
+    // Launch shapes of the two-kernel apply: times shape s with workgroup order x (0 linear, 1 one
+    // contiguous range per XCD) at index 2 s + x, keeps the fastest for that precision.
+    std::vector<double> autotune_shapes(int ncols, int precision_bytes);
+    void set_split_shape(int precision_bytes, int shape, int xcd);
+    int split_shape(int precision_bytes) const { return split_shape_[precision_bytes == 8 ? 0 : 1]; }
+    int split_xcd(int precision_bytes) const { return split_xcd_[precision_bytes == 8 ? 0 : 1]; }
     bool uses_mfma() const;
     bool uses_split() const;
     // Allocates the facet-slot buffers of both precisions for ncols columns (done by the apply when
@@ -255,6 +262,11 @@ private:
     template <class T> void grad_dispatch(T const* u, T const* psi, T* gl, T* gm);
     RhsArgs rhs_args();
     int variant_ = Variant_Auto;
+    // launch shape of the two-kernel apply, FP64 / FP32 (defaults: the fastest measured on the
+    // 24^3-cube P4 workload; the autotuner re-chooses per operator)
+    int split_shape_[2] = {0, 1};
+    int split_xcd_[2] = {1, 1};
+    double time_configs(int ncols, int pb, int count, std::function<bool(int)> const& select, std::vector<double>& ms);
 };
 
 } // namespace tdg

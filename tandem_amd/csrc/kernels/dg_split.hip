@@ -1,0 +1,44 @@
+// Dispatch of the two-kernel matrix-core elasticity apply over (dimension, degree).
+#include "kernels.h"
+#include "mfma_layout.h"
+
+namespace tdg {
+
+#define TDG_SPLIT_DECL(D, N)                                                                                    \
+    template <class T>                                                                                         \
+    hipError_t launch_split_d##D##p##N(int shape, SplitArgs<T> const& a, T const* x, T* y, T* L, int ncols,  \
+                                       int64_t ld, int64_t ldL, hipStream_t s);
+TDG_SPLIT_DECL(3, 2)
+TDG_SPLIT_DECL(3, 3)
+TDG_SPLIT_DECL(3, 4)
+TDG_SPLIT_DECL(2, 2)
+TDG_SPLIT_DECL(2, 4)
+TDG_SPLIT_DECL(2, 6)
+#undef TDG_SPLIT_DECL
+
+template <class T>
+hipError_t launch_elasticity_split(int dim, int degree, int shape, SplitArgs<T> const& a, T const* x, T* y, T* L,
+                                   int ncols, int64_t ld, int64_t ldL, hipStream_t s) {
+    if (shape < 0 || shape >= kSplitShapes) return hipErrorInvalidValue;
+    if (dim == 3) {
+        switch (degree) {
+        case 2: return launch_split_d3p2<T>(shape, a, x, y, L, ncols, ld, ldL, s);
+        case 3: return launch_split_d3p3<T>(shape, a, x, y, L, ncols, ld, ldL, s);
+        case 4: return launch_split_d3p4<T>(shape, a, x, y, L, ncols, ld, ldL, s);
+        }
+    } else if (dim == 2) {
+        switch (degree) {
+        case 2: return launch_split_d2p2<T>(shape, a, x, y, L, ncols, ld, ldL, s);
+        case 4: return launch_split_d2p4<T>(shape, a, x, y, L, ncols, ld, ldL, s);
+        case 6: return launch_split_d2p6<T>(shape, a, x, y, L, ncols, ld, ldL, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_elasticity_split<double>(int, int, int, SplitArgs<double> const&, double const*, double*,
+                                                    double*, int, int64_t, int64_t, hipStream_t);
+template hipError_t launch_elasticity_split<float>(int, int, int, SplitArgs<float> const&, float const*, float*, float*,
+                                                   int, int64_t, int64_t, hipStream_t);
+
+} // namespace tdg

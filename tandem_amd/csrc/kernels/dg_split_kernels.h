@@ -28,6 +28,8 @@
 // divergence on the matrix cores), with the accumulators started from the sum of the element's
 // facet slots.  Slots of faces that carry no facet term (natural boundaries) are zero: the
 // operator clears the slot buffer when it allocates it and the facet kernel never writes them.
+#pragma once
+
 #include "kernels.h"
 #include "mfma_layout.h"
 #include "mfma_ops.h"
@@ -36,9 +38,19 @@
 
 namespace tdg {
 
-namespace {
+namespace split {
 
-template <class T> __device__ __forceinline__ T rcp_(T v) { return T(1) / v; }
+// Reciprocal from the hardware estimate refined by one Newton step (relative error near the rounding
+// unit, far below what the result needs); the argument is a nonzero Jacobian determinant.
+template <class T> __device__ __forceinline__ T rcp_(T v);
+template <> __device__ __forceinline__ double rcp_<double>(double v) {
+    const double r = __builtin_amdgcn_rcp(v);
+    return fma(r, fma(-v, r, 1.0), r);
+}
+template <> __device__ __forceinline__ float rcp_<float>(float v) {
+    const float r = __builtin_amdgcn_rcpf(v);
+    return fmaf(r, fmaf(-v, r, 1.0f), r);
+}
 template <class T> __device__ __forceinline__ T sqrt_(T v);
 template <> __device__ __forceinline__ double sqrt_<double>(double v) { return sqrt(v); }
 template <> __device__ __forceinline__ float sqrt_<float>(float v) { return sqrtf(v); }
@@ -59,10 +71,12 @@ __device__ __forceinline__ void wave_sync() {
 // round-robin to the 8 XCDs (w to XCD w % 8); with several columns, w = 8 (ncols r + c) + x takes
 // column c of block 8 r + x, so all columns of a block run on one XCD one after another and the
 // block's data are read from HBM once into that XCD's L2.
-__device__ __forceinline__ void block_of(unsigned w, int ncols, unsigned& col, unsigned& lb) {
+// One column with xcd_remap: each XCD works through one contiguous range of blocks (the grid is a
+// multiple of 8 workgroups).
+__device__ __forceinline__ void block_of(unsigned w, int ncols, int xcd_remap, unsigned& col, unsigned& lb) {
     const unsigned xcd = w % 8u, r = w / 8u;
     col = ncols > 1 ? r % unsigned(ncols) : 0u;
-    lb = ncols > 1 ? (r / unsigned(ncols)) * 8u + xcd : w;
+    lb = ncols > 1 ? (r / unsigned(ncols)) * 8u + xcd : (xcd_remap ? xcd * (gridDim.x / 8u) + r : w);
 }
 
 // =====================================================================================  facets
@@ -81,8 +95,8 @@ __device__ __forceinline__ double mirror16(double v) {
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo));
 }
 
-template <int D, int N, class T, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
+template <int D, int N, class T, int W, int OCC, bool VTAIL_ON>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
 void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __restrict__ Ls, int64_t ld,
                              int64_t ldL, int ncols) {
     using L = MfmaLayout<D, N>;
@@ -93,7 +107,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
     constexpr int NbfP = Strides<T>::pad(Nbf), BSP = Strides<T>::pad(BS);
     constexpr int nT = NF * nq * Nbf;
     constexpr int TAIL = Nbf - 16 * (MT - 1);
-    constexpr bool VTAIL = D == 3 && MT > 1 && TAIL <= 4;
+    constexpr bool VTAIL = VTAIL_ON && D == 3 && MT > 1 && TAIL <= 4;
     constexpr int MM = VTAIL ? MT - 1 : MT;
     constexpr int NT = VTAIL ? TAIL : 1;
     constexpr int NI = (BS + 3) / 4; // staging steps of 4 entries per column
@@ -104,7 +118,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, j = lane & 15, g = lane >> 4;
     unsigned col, lb;
-    block_of(blockIdx.x, ncols, col, lb);
+    block_of(blockIdx.x, ncols, a.xcd_remap, col, lb);
     const int64_t grp = int64_t(lb) * W + wv;
     const bool live = grp < a.num_fgroups; // uniform per wave
     const int64_t gj = live ? grp : 0;
@@ -182,7 +196,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
         const int tq = mfma_group<T>(j);
         const int tc = min(mfma_reg<T>(j), D);
         T const* const U = X + j * BSP;
-        const T sgn = sd == 0 ? T(-1) : T(1); // lift of the traction term: -w tau on side 0, +w tau on side 1
+
 #pragma unroll 1
         for (int t = 0; t < FT; ++t) {
             const int q = 4 * t + g;
@@ -190,7 +204,10 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
             const bool act = ok && q < nq;
             T rn[DD + 2];
             load_rec(min(t + 1, FT - 1), rn); // next tile's record, one tile ahead
-            const T wq = a.w[qc];
+            // quadrature weight, zero on lanes without a facet point (every lane computes finite
+            // values: padding facets carry the identity Jacobian, empty columns a zero vector)
+            const T wq = act ? a.w[qc] : T(0);
+            const T ws = sd == 0 ? -wq : wq; // the traction term lifts with -w on side 0, +w on side 1
             acc C[Q];
 #pragma unroll
             for (int r = 0; r < Q; ++r) C[r] = acc{0, 0, 0, 0};
@@ -310,7 +327,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
             for (int d = 0; d < D; ++d) dn += dl[d] * n[d];
             T Wt[D + 1][Q];
 #pragma unroll
-            for (int u = 0; u < Q; ++u) Wt[0][u] = act ? sgn * wq * tau[u] : T(0);
+            for (int u = 0; u < Q; ++u) Wt[0][u] = ws * tau[u];
 #pragma unroll
             for (int e = 0; e < D; ++e)
 #pragma unroll
@@ -322,7 +339,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
                         if (u == jj) Cuj += lam * dn;
                         v += R[e][jj] * Cuj;
                     }
-                    Wt[1 + e][u] = act ? wq * v : T(0);
+                    Wt[1 + e][u] = wq * v;
                 }
             // ---- lift: Y(k') += sum_(q,c) T0(c, q ; k') W(c, q)
             {
@@ -405,8 +422,8 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
 // block b and the facet slots during the last block, so only the first loads of a wave wait on
 // memory.  Otherwise (two or more waves per SIMD) each block loads its own point data up front and
 // the slots are read at the end; the other waves of the SIMD cover the waits.
-template <int D, int N, class T, int W, bool DEEP>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(DEEP ? 1 : 2)))
+template <int D, int N, class T, int W, int OCC, bool DEEP, bool VTAIL_ON>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
 void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const* __restrict__ Ls,
                                T* __restrict__ ys, int64_t ld, int64_t ldL, int ncols) {
     using L = MfmaLayout<D, N>;
@@ -416,7 +433,7 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
     constexpr int NbfP = Strides<T>::pad(Nbf), BSP = Strides<T>::pad(BS);
     constexpr int nV = D * Nq * Nbf;
     constexpr int TAIL = Nbf - 16 * (MT - 1);
-    constexpr bool VTAIL = D == 3 && MT > 1 && TAIL <= 4;
+    constexpr bool VTAIL = VTAIL_ON && D == 3 && MT > 1 && TAIL <= 4;
     constexpr int MM = VTAIL ? MT - 1 : MT;
     constexpr int NT = VTAIL ? TAIL : 1;
     constexpr int NO = (16 * BS + 63) / 64; // output entries per lane
@@ -426,7 +443,7 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, j = lane & 15, g = lane >> 4;
     unsigned col, lb;
-    block_of(blockIdx.x, ncols, col, lb);
+    block_of(blockIdx.x, ncols, a.xcd_remap, col, lb);
     const int64_t grp = int64_t(lb) * W + wv;
     const bool live = grp < a.num_groups; // uniform per wave
     T const* __restrict__ x = xs + int64_t(col) * ld;
@@ -652,50 +669,54 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
     }
 }
 
-// W waves per facet workgroup; WE per element workgroup, DEEP element kernel (see above)
-template <int D, int N, class T, int W, int WE, bool DEEP>
-hipError_t run_split(SplitArgs<T> const& a, T const* x, T* y, T* Lbuf, int ncols, int64_t ld, int64_t ldL,
+// Launch shapes of the two kernels, searched by the autotuner: waves per workgroup and the
+// occupancy (waves per SIMD) the register allocation targets, for each kernel; DEEP element kernel;
+// the last basis tile of the lifts on the vector ALU (VTAIL) or on the matrix cores.
+template <int FW, int FOCC, int EW, int EOCC, bool DEEP, bool VT> struct Shape {
+    static constexpr int fw = FW, focc = FOCC, ew = EW, eocc = EOCC;
+    static constexpr bool deep = DEEP, vtail = VT;
+};
+
+template <int D, int N, class T, class S>
+hipError_t run_shape(SplitArgs<T> const& a, T const* x, T* y, T* Lbuf, int ncols, int64_t ld, int64_t ldL,
                      hipStream_t s) {
-    const int64_t fb = (a.num_fgroups + W - 1) / W;
+    const int64_t fb = (a.num_fgroups + S::fw - 1) / S::fw;
     const int64_t ft = (fb + 7) / 8 * 8 * int64_t(ncols);
-    const int64_t eb = (a.num_groups + WE - 1) / WE;
+    const int64_t eb = (a.num_groups + S::ew - 1) / S::ew;
     const int64_t et = (eb + 7) / 8 * 8 * int64_t(ncols);
     if (ft > int64_t(0x7fffffff) || et > int64_t(0x7fffffff)) return hipErrorInvalidValue;
     if (ft > 0) {
-        hipLaunchKernelGGL((elasticity_facet_kernel<D, N, T, W>), dim3(unsigned(ft)), dim3(64 * W), 0, s, a, x, Lbuf,
-                           ld, ldL, ncols);
+        hipLaunchKernelGGL((elasticity_facet_kernel<D, N, T, S::fw, S::focc, S::vtail>), dim3(unsigned(ft)),
+                           dim3(64 * S::fw), 0, s, a, x, Lbuf, ld, ldL, ncols);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((elasticity_element_kernel<D, N, T, WE, DEEP>), dim3(unsigned(et)), dim3(64 * WE), 0, s, a, x,
-                       Lbuf, y, ld, ldL, ncols);
+    hipLaunchKernelGGL((elasticity_element_kernel<D, N, T, S::ew, S::eocc, S::deep, S::vtail>), dim3(unsigned(et)),
+                       dim3(64 * S::ew), 0, s, a, x, Lbuf, y, ld, ldL, ncols);
     return hipGetLastError();
 }
 
-} // namespace
-
-template <class T>
-hipError_t launch_elasticity_split(int dim, int degree, SplitArgs<T> const& a, T const* x, T* y, T* Lbuf, int ncols,
-                                   int64_t ld, int64_t ldL, hipStream_t s) {
-    if (dim == 3) {
-        switch (degree) {
-        case 2: return run_split<3, 2, T, 4, 8, false>(a, x, y, Lbuf, ncols, ld, ldL, s);
-        case 3: return run_split<3, 3, T, 4, 8, false>(a, x, y, Lbuf, ncols, ld, ldL, s);
-        case 4: return run_split<3, 4, T, 4, 8, false>(a, x, y, Lbuf, ncols, ld, ldL, s);
+// The shape table (index < kSplitShapes): FP64 kernels hold up to two waves per SIMD, FP32 more.
+template <int D, int N, class T>
+hipError_t run_split(int shape, SplitArgs<T> const& a, T const* x, T* y, T* Lbuf, int ncols, int64_t ld, int64_t ldL,
+                     hipStream_t s) {
+    if constexpr (sizeof(T) == 8) {
+        switch (shape) {
+        case 0: return run_shape<D, N, T, Shape<4, 2, 8, 2, false, true>>(a, x, y, Lbuf, ncols, ld, ldL, s);
+        case 1: return run_shape<D, N, T, Shape<8, 2, 8, 2, false, true>>(a, x, y, Lbuf, ncols, ld, ldL, s);
+        case 2: return run_shape<D, N, T, Shape<4, 2, 4, 1, true, true>>(a, x, y, Lbuf, ncols, ld, ldL, s);
+        case 3: return run_shape<D, N, T, Shape<4, 2, 8, 2, false, false>>(a, x, y, Lbuf, ncols, ld, ldL, s);
         }
-    } else if (dim == 2) {
-        switch (degree) {
-        case 2: return run_split<2, 2, T, 4, 8, false>(a, x, y, Lbuf, ncols, ld, ldL, s);
-        case 4: return run_split<2, 4, T, 4, 8, false>(a, x, y, Lbuf, ncols, ld, ldL, s);
-        case 6: return run_split<2, 6, T, 4, 8, false>(a, x, y, Lbuf, ncols, ld, ldL, s);
+    } else {
+        switch (shape) {
+        case 0: return run_shape<D, N, T, Shape<4, 3, 4, 3, false, false>>(a, x, y, Lbuf, ncols, ld, ldL, s);
+        case 1: return run_shape<D, N, T, Shape<4, 3, 4, 3, false, true>>(a, x, y, Lbuf, ncols, ld, ldL, s);
+        case 2: return run_shape<D, N, T, Shape<4, 4, 4, 3, false, false>>(a, x, y, Lbuf, ncols, ld, ldL, s);
+        case 3: return run_shape<D, N, T, Shape<8, 3, 8, 2, false, false>>(a, x, y, Lbuf, ncols, ld, ldL, s);
         }
     }
     return hipErrorInvalidValue;
 }
 
-template hipError_t launch_elasticity_split<double>(int, int, SplitArgs<double> const&, double const*, double*,
-                                                    double*, int, int64_t, int64_t, hipStream_t);
-template hipError_t launch_elasticity_split<float>(int, int, SplitArgs<float> const&, float const*, float*, float*,
-                                                   int, int64_t, int64_t, hipStream_t);
-
+} // namespace split
 } // namespace tdg

@@ -88,13 +88,18 @@ template <class T> struct SplitArgs {
     T const* FPen;
     int4 const* FInfo;
     int64_t num_elements, num_groups, num_facets, num_fgroups, num_two;
+    int xcd_remap; // one column: each XCD takes one contiguous range of groups
 };
 
+// Number of launch shapes of the two-kernel apply (waves per workgroup, occupancy targets, element
+// prefetch depth, lift tail on the vector ALU or the matrix cores), per precision.
+constexpr int kSplitShapes = 4;
+
 // y = A x for ncols columns (stride ld) through the facet kernel (into the slot buffer L, column
-// stride ldL = E (D+1) Nbf D) and the element kernel.
+// stride ldL = E (D+1) Nbf D) and the element kernel, with launch shape `shape`.
 template <class T>
-hipError_t launch_elasticity_split(int dim, int degree, SplitArgs<T> const& a, T const* x, T* y, T* L, int ncols,
-                                   int64_t ld, int64_t ldL, hipStream_t stream);
+hipError_t launch_elasticity_split(int dim, int degree, int shape, SplitArgs<T> const& a, T const* x, T* y, T* L,
+                                   int ncols, int64_t ld, int64_t ldL, hipStream_t stream);
 
 template <class T> struct GradArgs;
 // Adjoint material gradient on the matrix cores (same dimensions and degrees as the apply):

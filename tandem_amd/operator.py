@@ -145,6 +145,8 @@ def _check(t, what, dtype="f64", numel=None, min_numel=None, cols=None, like=Non
 
 # kernel variants, indexed by TDG_VARIANT_*
 VARIANTS = ["auto", "points", "mfma-xcd", "mfma-element"]
+# launch shapes of the two-kernel apply per precision (TDG_SPLIT_SHAPES)
+SPLIT_SHAPES = 4
 
 
 class DGOperator:
@@ -286,6 +288,19 @@ class DGOperator:
         ms = np.zeros(len(VARIANTS))
         check(lib().tdg_autotune(self._h, ncols, 8 if precision == "fp64" else 4, C.byref(chosen), ms))
         return VARIANTS[chosen.value], {VARIANTS[v]: float(ms[v]) for v in range(len(VARIANTS)) if ms[v] >= 0}
+
+    def autotune_shapes(self, ncols=1, precision="fp64"):
+        """Time the launch shapes of the two-kernel apply (shape s, workgroup order x: 0 linear, 1 per
+        XCD) on ncols columns, keep the fastest for that precision; returns ((s, x), {(s, x): ms})."""
+        shape, xcd = C.c_int(), C.c_int()
+        ms = np.zeros(2 * SPLIT_SHAPES)
+        check(lib().tdg_autotune_shapes(self._h, ncols, 8 if precision == "fp64" else 4, C.byref(shape),
+                                        C.byref(xcd), ms))
+        return (shape.value, xcd.value), {(c // 2, c % 2): float(ms[c]) for c in range(2 * SPLIT_SHAPES) if ms[c] >= 0}
+
+    def set_shape(self, precision, shape, xcd=0):
+        """Launch shape of the two-kernel apply for "fp64" or "fp32"."""
+        check(lib().tdg_set_shape(self._h, 8 if precision == "fp64" else 4, shape, xcd))
 
     def solve_segments(self, b, x, offsets, rtol=1e-10, mode="fp64", max_iter=200000, check_every=10,
                        reliable_delta=0.1, precond=None, graphs=True):
