@@ -8,13 +8,14 @@ sigma 0.3, correlation length 0.2), Dirichlet on all faces.  A step is one matri
 y = A x of a seeded uniform[-1,1] vector in FP64, inputs resident in HBM; each precision is timed
 after 0.3 s of untimed applies (the GPU's clocks ramp up under sustained load) and W warm-up steps.
 
-Prints one JSON line: GDoF/s of the FP64 apply (`value`), the roofline of the apply kernel measured
-with HIP events on its stream, the CPU baseline (the CPU FP64 oracle timed on the host cores on the
-same mesh and vector), the FP32 apply, and the time to a 1e-10 relative residual
-of FP64 CG versus FP32 CG with FP64 residual replacement on the same mesh and body force, without
-and with the element-block Jacobi preconditioner.
+Prints one JSON line: GDoF/s of the FP64 apply (`value`), the roofline of the apply (its two
+kernels, facet then element, back to back on one stream) measured with HIP events on that stream,
+the CPU baseline (the CPU FP64 oracle timed on the host cores on the same mesh and vector), the FP32
+apply, the launch shapes the autotuner chose, and the time to a 1e-10 relative residual of FP64 CG
+versus FP32 CG with FP64 residual replacement on the same mesh and body force, without and with the
+element-block Jacobi preconditioner.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-solve] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-solve] [--no-cpu-baseline] [--no-autotune]
     python bench.py --workload precision|greens|adjoint|multimesh|poisson|gradient   (the other studies; one JSON line each)
 """
 import argparse
@@ -33,7 +34,8 @@ SEED = 20261203
 PEAK_FP64 = 78.6e12   # MI355X dense FP64 (matrix and vector), FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12     # bytes/s
-KERNEL = "elasticity_mfma_kernel<3, 4, double>"
+# the FP64 apply: the facet kernel then the element kernel of the two-kernel matrix-core apply
+KERNELS = ["elasticity_facet_kernel<3, 4, double, ...>", "elasticity_element_kernel<3, 4, double, ...>"]
 
 
 def log(*a):
@@ -153,12 +155,13 @@ def cpu_baseline(mesh, nodes, lam_Q, mu_Q, budget_s=20.0, max_reps=100):
 
 
 def traffic_from_profile():
-    p = os.path.join(ROOT, "profiles", "elasticity_mfma_p4_box24_traffic.json")
+    """HBM bytes per FP64 apply (both kernels) from the committed counter summary, if it covers them."""
+    p = os.path.join(ROOT, "profiles", "elasticity_split_p4_box24_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
         d = json.load(f)
-    return d.get("hbm_bytes_per_launch") if d.get("kernel") == KERNEL else None
+    return d.get("hbm_bytes_per_apply") if d.get("kernels") == KERNELS else None
 
 
 def solve_comparison(op, mesh, nodes, fields, rtol=1e-10):
@@ -709,6 +712,7 @@ def main():
     ap.add_argument("--cells", type=int, default=24)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shapes")
     ap.add_argument("--workload", choices=["apply", "precision", "greens", "adjoint", "multimesh", "poisson", "gradient"],
                     default="apply",
                     help="apply: the headline line (default); precision: FP32-MFMA vs FP64 solve at --cells "
@@ -742,6 +746,16 @@ def main():
     log("setup %.1fs, %d elements, %d DoF" % (time.perf_counter() - t0, op.E, op.n))
     stream = torch.cuda.Stream()
     op.set_stream(stream)
+    # the library's autotuner picks the launch shape of the two-kernel apply for each precision
+    # (--no-autotune keeps the library's defaults, e.g. for counter runs of fixed kernels)
+    shapes = {}
+    for prec in ("fp64", "fp32"):
+        if args.no_autotune:
+            shapes[prec] = "library default"
+            continue
+        best, ms = op.autotune_shapes(1, prec)
+        shapes[prec] = {"shape": best[0], "xcd_order": best[1], "ms": {"%d/%d" % k: v for k, v in ms.items()}}
+        log("autotune %s: shape %d, workgroup order %d" % (prec, best[0], best[1]))
     # torch's own work (vector set-up, conversions, temporaries handed to the library) goes on the
     # operator's stream too: ordered with the library's kernels, and the caching allocator cannot
     # reuse a temporary while a library kernel on another stream still reads it
@@ -763,7 +777,7 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP64,
         "traffic": traffic_from_profile(),
-        "kernel": KERNEL,
+        "kernel": " + ".join(KERNELS),
         "algorithmic_flops_per_launch": F,
         "algorithmic_bytes_per_launch": B64,
         "hbm_frac_algorithmic": B64 / t_ev / PEAK_HBM,
@@ -792,6 +806,7 @@ def main():
         "roofline": roof,
         "fp32_apply": {"value": op.n / t32 * 1e-9, "unit": "GDoF/s", "ms": t32 * 1e3,
                        "frac_fp32_peak": F / t32 / PEAK_FP32},
+        "launch_shapes": shapes,
     }
     if not args.no_solve:
         res["solve"] = solve_comparison(op, mesh, nodes, fields)
