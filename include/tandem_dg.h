@@ -87,6 +87,15 @@ int tdg_upload(tdg_op* op);
 int tdg_mult(tdg_op* op, double const* x, double* y);
 /* The same operator in FP32 arithmetic (FP32 copies of the FP64 precompute). */
 int tdg_mult_f32(tdg_op* op, float const* x, float* y);
+/* The FP64 apply y = A x without any device allocation, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation); no counterpart in the
+ * reference.  The facet slots of the two-kernel apply live in a fixed 64 MiB device array of the
+ * library (each GPU holds it until the process ends): operators whose slots need more
+ * (elements x (dim+1) x block size doubles > 2^23, about 20,000 P4 tetrahedra) are rejected with
+ * TDG_ERR_ARG.  The operator must be on the device already (tdg_upload, which also sets up the array
+ * on the current device, or one tdg_mult), else TDG_ERR_STATE.  All calls on one GPU share the
+ * array: calls that overlap in time on one GPU race; calls ordered on one stream do not. */
+int tdg_mult_noalloc(tdg_op* op, double const* x, double* y);
 /* Y(:, c) = A X(:, c) for ncols columns with leading dimension ld (>= local size). */
 int tdg_mult_batched(tdg_op* op, double const* X, double* Y, int ncols, int64_t ld);
 int tdg_mult_batched_f32(tdg_op* op, float const* X, float* Y, int ncols, int64_t ld);

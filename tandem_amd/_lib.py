@@ -51,7 +51,7 @@ class MeshDesc(C.Structure):
 EXPORTS = [
     "tdg_create", "tdg_destroy", "tdg_last_error", "tdg_sizes", "tdg_block_size", "tdg_local_size",
     "tdg_volume_points", "tdg_facet_points",
-    "tdg_facets", "tdg_set_material", "tdg_upload", "tdg_mult", "tdg_mult_f32", "tdg_mult_batched", "tdg_mult_batched_f32",
+    "tdg_facets", "tdg_set_material", "tdg_upload", "tdg_mult", "tdg_mult_f32", "tdg_mult_noalloc", "tdg_mult_batched", "tdg_mult_batched_f32",
     "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_solve_multi", "tdg_solve_segments", "tdg_autotune", "tdg_autotune_shapes", "tdg_set_shape", "tdg_rhs", "tdg_rhs_device", "tdg_traction", "tdg_block_jacobi_build", "tdg_block_jacobi_apply", "tdg_block_jacobi_apply_batched", "tdg_block_jacobi_inverse", "tdg_material_gradient",
     "tdg_material_gradient_f32", "tdg_fault_setup", "tdg_fault_slip", "tdg_fault_traction", "tdg_inverse_mass", "tdg_wave_rhs", "tdg_interpolation_matrix", "tdg_interpolate",
     "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
@@ -80,6 +80,7 @@ def lib():
     L.tdg_upload.argtypes = [vp]
     L.tdg_mult.argtypes = [vp, vp, vp]
     L.tdg_mult_f32.argtypes = [vp, vp, vp]
+    L.tdg_mult_noalloc.argtypes = [vp, vp, vp]
     L.tdg_mult_batched.argtypes = [vp, vp, vp, C.c_int, C.c_int64]
     L.tdg_mult_batched_f32.argtypes = [vp, vp, vp, C.c_int, C.c_int64]
     L.tdg_set_stream.argtypes = [vp, vp]

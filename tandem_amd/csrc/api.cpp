@@ -1,6 +1,7 @@
 // extern "C" boundary of the MI355X DG operator (declared in include/tandem_dg.h).
 #include "../../include/tandem_dg.h"
 
+#include "abi.h"
 #include "host/mesh.h"
 #include "host/operator.h"
 #include "host/refelem.h"
@@ -17,43 +18,7 @@
 #include <stdexcept>
 #include <string>
 
-struct tdg_op {
-    std::unique_ptr<tdg::Operator> impl;
-};
-
-namespace {
-thread_local std::string g_error;
-
-int fail(int code, std::string const& msg) {
-    g_error = msg;
-    return code;
-}
-
-template <class F> int guarded(F&& f) {
-    try {
-        f();
-        return TDG_OK;
-    } catch (std::invalid_argument const& e) {
-        return fail(TDG_ERR_ARG, e.what());
-    } catch (std::logic_error const& e) {
-        return fail(TDG_ERR_STATE, e.what());
-    } catch (std::exception const& e) {
-        std::string m = e.what();
-        return fail(m.rfind("HIP error", 0) == 0 ? TDG_ERR_HIP : TDG_ERR_INTERNAL, m);
-    } catch (...) {
-        return fail(TDG_ERR_INTERNAL, "unknown exception");
-    }
-}
-
-void require(bool cond, char const* msg) {
-    if (!cond) throw std::invalid_argument(msg);
-}
-tdg::Operator& ready(tdg_op* op) {
-    require(op && op->impl, "null operator");
-    if (!op->impl->ready()) throw std::logic_error("tdg_set_material() has not been called");
-    return *op->impl;
-}
-} // namespace
+using namespace tdg_abi;
 
 extern "C" {
 
@@ -161,7 +126,10 @@ int tdg_set_material(tdg_op* op, double const* c0_Q, double const* c1_Q) {
 }
 
 int tdg_upload(tdg_op* op) {
-    return guarded([&] { ready(op).upload_device(); });
+    return guarded([&] {
+        ready(op).upload_device();
+        prepare_noalloc_scratch();
+    });
 }
 
 int tdg_mult(tdg_op* op, double const* x, double* y) {

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <string>
 #include <stdexcept>
 #include <type_traits>
 
@@ -746,6 +747,11 @@ template <class T> void Operator::upload_split(DeviceSplit<T>& ds) {
     ds.FR.upload(FR);
     ds.FPen.upload(FPen);
     ds.FInfo.upload(FInfo);
+    std::vector<int32_t> idle;
+    for (std::size_t e = 0; e < E_; ++e)
+        for (unsigned f = 0; f < NF; ++f)
+            if ((ef_[(e * NF + f) * 4 + 2] & 3) == Face_Skip) idle.push_back(int32_t(e * NF + f));
+    ds.idle.upload(idle);
     ds.nfacets = int64_t(nact);
     ds.ntwo = int64_t(ntwo);
     ds.cap = 0;
@@ -1405,6 +1411,35 @@ template <class T> void Operator::dispatch(T const* x, T* y, int ncols, int64_t 
 }
 
 void Operator::apply(double const* x, double* y) { dispatch(x, y, 1, 0); }
+
+std::size_t Operator::slot_entries() const {
+    return uses_split() ? E_ * (D_ + 1) * block_size() : 0;
+}
+
+void Operator::apply_noalloc(double const* x, double* y, double* scratch, std::size_t cap) {
+    if (!ready_) throw std::logic_error("set_material() must be called before apply");
+    // everything the selected kernels read must already be on the device: nothing is uploaded here
+    const bool have = d64_ && (!uses_mfma() || (m64_ && (uses_split() ? bool(s64_) : m64_->FD.n > 0)));
+    if (!have)
+        throw std::logic_error("the allocation-free apply needs the operator on the device: call tdg_upload "
+                               "(or one tdg_mult) first");
+    if (!uses_split()) { // the other kernels use no scratch memory
+        dispatch(x, y, 1, 0);
+        return;
+    }
+    const std::size_t need = slot_entries();
+    if (need > cap)
+        throw std::invalid_argument("the allocation-free apply holds at most " + std::to_string(cap) +
+                                    " facet-slot entries; this operator needs " + std::to_string(need) +
+                                    " (use tdg_mult)");
+    int kq = 0, kf = 0;
+    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
+        throw std::runtime_error("no kernel compiled for this dimension/degree");
+    TDG_HIP(launch_zero_slots<double>(scratch, s64_->idle.ptr, int64_t(s64_->idle.n), int(block_size()), stream_));
+    SplitArgs<double> sa = split_args(*m64_, *s64_);
+    sa.xcd_remap = split_xcd_[0];
+    TDG_HIP(launch_elasticity_split<double>(D_, N_, split_shape_[0], sa, x, y, scratch, 1, 0, int64_t(need), stream_));
+}
 void Operator::apply(float const* x, float* y) { dispatch(x, y, 1, 0); }
 void Operator::apply_batched(double const* X, double* Y, int ncols, int64_t ld) { dispatch(X, Y, ncols, ld); }
 void Operator::apply_batched(float const* X, float* Y, int ncols, int64_t ld) { dispatch(X, Y, ncols, ld); }

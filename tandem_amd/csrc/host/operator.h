@@ -60,6 +60,7 @@ template <class T> struct DeviceMfma {
 template <class T> struct DeviceSplit {
     DeviceArray<T> FR, FPen;
     DeviceArray<int32_t> FInfo; // int4 per facet
+    DeviceArray<int32_t> idle;  // slots (element * (D+1) + face) of faces without a facet term
     DeviceArray<T> L;
     int64_t nfacets = 0, ntwo = 0;
     int cap = 0;
@@ -151,6 +152,12 @@ public:
 
     // y = A x on device buffers (x, y of local_size() entries).
     void apply(double const* x, double* y);
+    // The same FP64 apply with the facet slots in caller-provided device memory of cap_doubles
+    // entries; allocates nothing (the operator must have been uploaded before).  Clears the slots of
+    // faces without a facet term, so any previous contents of the memory are harmless.
+    void apply_noalloc(double const* x, double* y, double* scratch, std::size_t cap_doubles);
+    // Entries of the facet-slot buffer of one column (0 when the apply uses no slots).
+    std::size_t slot_entries() const;
     void apply(float const* x, float* y);
     // multi-vector apply: columns of length local_size() with leading dimension ld
     void apply_batched(double const* X, double* Y, int ncols, int64_t ld);

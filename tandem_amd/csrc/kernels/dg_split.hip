@@ -36,6 +36,24 @@ hipError_t launch_elasticity_split(int dim, int degree, int shape, SplitArgs<T> 
     return hipErrorInvalidValue;
 }
 
+namespace {
+template <class T> __global__ void zero_slots_kernel(T* L, int32_t const* slots, int64_t n, int bs) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n * bs) L[int64_t(slots[i / bs]) * bs + i % bs] = T(0);
+}
+} // namespace
+
+template <class T> hipError_t launch_zero_slots(T* L, int32_t const* slots, int64_t nslots, int bs, hipStream_t s) {
+    if (nslots <= 0) return hipSuccess;
+    const int64_t n = nslots * bs;
+    const int64_t blocks = (n + 255) / 256;
+    if (blocks > int64_t(0x7fffffff)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((zero_slots_kernel<T>), dim3(unsigned(blocks)), dim3(256), 0, s, L, slots, nslots, bs);
+    return hipGetLastError();
+}
+template hipError_t launch_zero_slots<double>(double*, int32_t const*, int64_t, int, hipStream_t);
+template hipError_t launch_zero_slots<float>(float*, int32_t const*, int64_t, int, hipStream_t);
+
 template hipError_t launch_elasticity_split<double>(int, int, int, SplitArgs<double> const&, double const*, double*,
                                                     double*, int, int64_t, int64_t, hipStream_t);
 template hipError_t launch_elasticity_split<float>(int, int, int, SplitArgs<float> const&, float const*, float*, float*,

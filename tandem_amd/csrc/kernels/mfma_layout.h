@@ -101,6 +101,9 @@ template <class T>
 hipError_t launch_elasticity_split(int dim, int degree, int shape, SplitArgs<T> const& a, T const* x, T* y, T* L,
                                    int ncols, int64_t ld, int64_t ldL, hipStream_t stream);
 
+// Zeroes the facet slots listed in `slots` (slot s = entries [s bs, (s + 1) bs) of L).
+template <class T> hipError_t launch_zero_slots(T* L, int32_t const* slots, int64_t nslots, int bs, hipStream_t s);
+
 template <class T> struct GradArgs;
 // Adjoint material gradient on the matrix cores (same dimensions and degrees as the apply):
 // g_lam, g_mu (E x Nq) = d/d(lam_Q, mu_Q) of psi^T A u.
