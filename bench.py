@@ -111,7 +111,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(mesh, nodes, lam_Q, mu_Q, budget_s=20.0, max_reps=100):
+def cpu_baseline(mesh, nodes, lam_Q, mu_Q, budget_s=30.0, max_reps=100):
     """The CPU FP64 oracle (element-centric restatement of tandem's apply) on the same mesh,
     geometry, coefficients and input vector as the GPU apply, timed as app/static.cpp:94-121 does:
     one warm-up apply, then up to max_reps applies (as many as fit in budget_s of wall time) on the

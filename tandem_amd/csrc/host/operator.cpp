@@ -503,25 +503,32 @@ double Operator::time_configs(int ncols, int pb, int count, std::function<bool(i
         TDG_HIP(hipEventCreate(&e0));
         TDG_HIP(hipEventCreate(&e1));
         const int64_t ld = int64_t(local_size());
-        for (int c = 0; c < count; ++c) {
-            if (!select(c)) continue;
-            upload_device();
-            auto run = [&] {
-                if (pb == 8) dispatch(static_cast<double const*>(X), static_cast<double*>(Y), ncols, ld);
-                else dispatch(static_cast<float const*>(X), static_cast<float*>(Y), ncols, ld);
-            };
-            run();
-            run();
-            constexpr int reps = 5;
-            TDG_HIP(hipEventRecord(e0, stream_));
-            for (int r = 0; r < reps; ++r) run();
-            TDG_HIP(hipEventRecord(e1, stream_));
-            TDG_HIP(hipEventSynchronize(e1));
-            float t = 0.0f;
-            TDG_HIP(hipEventElapsedTime(&t, e0, e1));
-            ms[std::size_t(c)] = double(t) / reps;
-            if (best < 0.0 || ms[std::size_t(c)] < best) best = ms[std::size_t(c)];
-        }
+        auto run = [&] {
+            if (pb == 8) dispatch(static_cast<double const*>(X), static_cast<double*>(Y), ncols, ld);
+            else dispatch(static_cast<float const*>(X), static_cast<float*>(Y), ncols, ld);
+        };
+        // a warm-up pass over every configuration first (uploads, and the GPU's clocks settle under
+        // sustained load), then two timed rounds; each configuration keeps its faster round
+        for (int round = 0; round < 3; ++round)
+            for (int c = 0; c < count; ++c) {
+                if (!select(c)) continue;
+                upload_device();
+                if (round == 0) {
+                    for (int r = 0; r < 4; ++r) run();
+                    continue;
+                }
+                constexpr int reps = 5;
+                run();
+                TDG_HIP(hipEventRecord(e0, stream_));
+                for (int r = 0; r < reps; ++r) run();
+                TDG_HIP(hipEventRecord(e1, stream_));
+                TDG_HIP(hipEventSynchronize(e1));
+                float t = 0.0f;
+                TDG_HIP(hipEventElapsedTime(&t, e0, e1));
+                const double v = double(t) / reps;
+                if (ms[std::size_t(c)] < 0.0 || v < ms[std::size_t(c)]) ms[std::size_t(c)] = v;
+                if (best < 0.0 || v < best) best = v;
+            }
     } catch (...) {
         cleanup();
         throw;
@@ -1369,6 +1376,7 @@ template <class T> SplitArgs<T> Operator::split_args(DeviceMfma<T> const& m, Dev
     b.num_fgroups = (d.nfacets + 7) / 8;
     b.num_two = d.ntwo;
     b.xcd_remap = 0;
+    b.dot = nullptr;
     return b;
 }
 
@@ -1411,6 +1419,45 @@ template <class T> void Operator::dispatch(T const* x, T* y, int ncols, int64_t 
 }
 
 void Operator::apply(double const* x, double* y) { dispatch(x, y, 1, 0); }
+
+int64_t Operator::dot_partials_capacity() const {
+    if (!(kind_ == Kind_Elasticity && mfma_supported(D_, N_))) return 0;
+    const int64_t ng = int64_t((E_ + 15) / 16);
+    int64_t m = 0;
+    for (int sh = 0; sh < kSplitShapes; ++sh)
+        m = std::max({m, split_element_workgroups<double>(sh, ng), split_element_workgroups<float>(sh, ng)});
+    return m;
+}
+
+template <class T> int64_t Operator::apply_dot_impl(T const* x, T* y, double* partials) {
+    if (!ready_) throw std::logic_error("set_material() must be called before apply");
+    upload_device();
+    if (!uses_split()) {
+        dispatch(x, y, 1, 0);
+        return 0;
+    }
+    int kq = 0, kf = 0;
+    if (kernel_rule_sizes(D_, N_, &kq, &kf) != 0 || kq != int(Nq_) || kf != int(nq_))
+        throw std::runtime_error("no kernel compiled for this dimension/degree");
+    auto& ds = [&]() -> DeviceSplit<T>& {
+        if constexpr (std::is_same<T, double>::value) return *s64_;
+        else return *s32_;
+    }();
+    auto const& dm = [&]() -> DeviceMfma<T> const& {
+        if constexpr (std::is_same<T, double>::value) return *m64_;
+        else return *m32_;
+    }();
+    const int p = sizeof(T) == 8 ? 0 : 1;
+    SplitArgs<T> sa = split_args(dm, ds);
+    sa.xcd_remap = split_xcd_[p];
+    sa.dot = partials;
+    const int64_t ldL = int64_t(E_) * (D_ + 1) * int64_t(block_size());
+    TDG_HIP(launch_elasticity_split<T>(D_, N_, split_shape_[p], sa, x, y, ds.L.ptr, 1, 0, ldL, stream_));
+    return split_element_workgroups<T>(split_shape_[p], int64_t((E_ + 15) / 16));
+}
+
+int64_t Operator::apply_dot(double const* x, double* y, double* partials) { return apply_dot_impl(x, y, partials); }
+int64_t Operator::apply_dot(float const* x, float* y, double* partials) { return apply_dot_impl(x, y, partials); }
 
 std::size_t Operator::slot_entries() const {
     return uses_split() ? E_ * (D_ + 1) * block_size() : 0;

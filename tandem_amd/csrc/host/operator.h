@@ -158,6 +158,12 @@ public:
     void apply_noalloc(double const* x, double* y, double* scratch, std::size_t cap_doubles);
     // Entries of the facet-slot buffer of one column (0 when the apply uses no slots).
     std::size_t slot_entries() const;
+    // y = A x and, where the two-kernel apply runs, per element workgroup the partial sums of x . y
+    // in partials[0 .. return value) (fixed order, deterministic); returns 0 (and only applies) for
+    // the other kernels.  partials needs dot_partials_capacity() entries.
+    int64_t apply_dot(double const* x, double* y, double* partials);
+    int64_t apply_dot(float const* x, float* y, double* partials);
+    int64_t dot_partials_capacity() const;
     void apply(float const* x, float* y);
     // multi-vector apply: columns of length local_size() with leading dimension ld
     void apply_batched(double const* X, double* Y, int ncols, int64_t ld);
@@ -214,6 +220,7 @@ private:
     template <class T> void upload_split(DeviceSplit<T>& d);
     template <class T> SplitArgs<T> split_args(DeviceMfma<T> const& m, DeviceSplit<T> const& s) const;
     template <class T> void reserve_split(DeviceSplit<T>& s, int ncols);
+    template <class T> int64_t apply_dot_impl(T const* x, T* y, double* partials);
     void ensure_mfma_faces();
     template <class T> void launch(DeviceOperator<T> const& d, T const* x, T* y, int ncols, int64_t ld);
     template <class T> MfmaArgs<T> mfma_args(DeviceMfma<T> const& d) const;

@@ -1,6 +1,5 @@
 // Dispatch of the two-kernel matrix-core elasticity apply over (dimension, degree).
-#include "kernels.h"
-#include "mfma_layout.h"
+#include "dg_split_kernels.h"
 
 namespace tdg {
 
@@ -35,6 +34,12 @@ hipError_t launch_elasticity_split(int dim, int degree, int shape, SplitArgs<T> 
     }
     return hipErrorInvalidValue;
 }
+
+template <class T> int64_t split_element_workgroups(int shape, int64_t num_groups) {
+    return split::element_workgroups_of<T>(shape, num_groups);
+}
+template int64_t split_element_workgroups<double>(int, int64_t);
+template int64_t split_element_workgroups<float>(int, int64_t);
 
 namespace {
 template <class T> __global__ void zero_slots_kernel(T* L, int32_t const* slots, int64_t n, int bs) {

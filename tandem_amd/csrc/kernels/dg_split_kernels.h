@@ -655,16 +655,33 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
         if constexpr (!DEEP) load_slots(ls);
     }
     wave_sync();
+    double dot = 0.0; // x . y over this wave's elements (one column only: Krylov solvers)
     if (live) {
         // y = volume + the element's facet slots
         T* dst = y + e0 * BS;
+        T const* src = x + e0 * BS;
 #pragma unroll
         for (int o = 0; o < NO; ++o) {
             const int i = 64 * o + lane;
             if (i < nout) {
                 const int jj = i / BS, rem = i - jj * BS;
-                dst[i] = U[jj * BSP + rem] + ls[o];
+                const T v = U[jj * BSP + rem] + ls[o];
+                dst[i] = v;
+                if (a.dot) dot += double(src[i]) * double(v);
             }
+        }
+    }
+    if (a.dot) {
+        // one partial per workgroup: the wave sums, then the waves' sums in a fixed order
+        __shared__ double red[W];
+        for (int o = 32; o > 0; o >>= 1) dot += __shfl_down(dot, o, 64);
+        if (lane == 0) red[wv] = dot;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) t += red[w];
+            a.dot[blockIdx.x] = t;
         }
     }
 }
@@ -677,13 +694,17 @@ template <int FW, int FOCC, int EW, int EOCC, bool DEEP, bool VT> struct Shape {
     static constexpr bool deep = DEEP, vtail = VT;
 };
 
+template <class S> int64_t element_workgroups(int64_t num_groups) {
+    const int64_t eb = (num_groups + S::ew - 1) / S::ew;
+    return (eb + 7) / 8 * 8;
+}
+
 template <int D, int N, class T, class S>
 hipError_t run_shape(SplitArgs<T> const& a, T const* x, T* y, T* Lbuf, int ncols, int64_t ld, int64_t ldL,
                      hipStream_t s) {
     const int64_t fb = (a.num_fgroups + S::fw - 1) / S::fw;
     const int64_t ft = (fb + 7) / 8 * 8 * int64_t(ncols);
-    const int64_t eb = (a.num_groups + S::ew - 1) / S::ew;
-    const int64_t et = (eb + 7) / 8 * 8 * int64_t(ncols);
+    const int64_t et = element_workgroups<S>(a.num_groups) * int64_t(ncols);
     if (ft > int64_t(0x7fffffff) || et > int64_t(0x7fffffff)) return hipErrorInvalidValue;
     if (ft > 0) {
         hipLaunchKernelGGL((elasticity_facet_kernel<D, N, T, S::fw, S::focc, S::vtail>), dim3(unsigned(ft)),
@@ -716,6 +737,25 @@ hipError_t run_split(int shape, SplitArgs<T> const& a, T const* x, T* y, T* Lbuf
         }
     }
     return hipErrorInvalidValue;
+}
+
+template <class T> int64_t element_workgroups_of(int shape, int64_t num_groups) {
+    if constexpr (sizeof(T) == 8) {
+        switch (shape) {
+        case 0: return element_workgroups<Shape<4, 2, 8, 2, false, true>>(num_groups);
+        case 1: return element_workgroups<Shape<8, 2, 8, 2, false, true>>(num_groups);
+        case 2: return element_workgroups<Shape<4, 2, 4, 1, true, true>>(num_groups);
+        case 3: return element_workgroups<Shape<4, 2, 8, 2, false, false>>(num_groups);
+        }
+    } else {
+        switch (shape) {
+        case 0: return element_workgroups<Shape<4, 3, 4, 3, false, false>>(num_groups);
+        case 1: return element_workgroups<Shape<4, 3, 4, 3, false, true>>(num_groups);
+        case 2: return element_workgroups<Shape<4, 4, 4, 3, false, false>>(num_groups);
+        case 3: return element_workgroups<Shape<8, 3, 8, 2, false, false>>(num_groups);
+        }
+    }
+    return -1;
 }
 
 } // namespace split

@@ -89,6 +89,7 @@ template <class T> struct SplitArgs {
     int4 const* FInfo;
     int64_t num_elements, num_groups, num_facets, num_fgroups, num_two;
     int xcd_remap; // one column: each XCD takes one contiguous range of groups
+    double* dot;   // one column, optional: per element workgroup the partial sum of x . y
 };
 
 // Number of launch shapes of the two-kernel apply (waves per workgroup, occupancy targets, element
@@ -96,10 +97,14 @@ template <class T> struct SplitArgs {
 constexpr int kSplitShapes = 4;
 
 // y = A x for ncols columns (stride ld) through the facet kernel (into the slot buffer L, column
-// stride ldL = E (D+1) Nbf D) and the element kernel, with launch shape `shape`.
+// stride ldL = E (D+1) Nbf D) and the element kernel, with launch shape `shape`.  With a.dot set
+// (one column), the element kernel also writes per workgroup the partial sum of x . y into
+// a.dot[0 .. split_element_workgroups(shape, num_groups)).
 template <class T>
 hipError_t launch_elasticity_split(int dim, int degree, int shape, SplitArgs<T> const& a, T const* x, T* y, T* L,
                                    int ncols, int64_t ld, int64_t ldL, hipStream_t stream);
+// Workgroups of the element kernel of launch shape `shape` for one column (a multiple of 8).
+template <class T> int64_t split_element_workgroups(int shape, int64_t num_groups);
 
 // Zeroes the facet slots listed in `slots` (slot s = entries [s bs, (s + 1) bs) of L).
 template <class T> hipError_t launch_zero_slots(T* L, int32_t const* slots, int64_t nslots, int bs, hipStream_t s);

@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <stdexcept>
@@ -62,6 +63,49 @@ __global__ __launch_bounds__(kBlock) void finalize_sum(double const* __restrict_
     for (int i = threadIdx.x; i < kGrid; i += kBlock) v += partial[i];
     v = block_sum(v);
     if (threadIdx.x == 0) *out = v;
+}
+
+// Sum of partial[0 .. np) in a fixed order (the same in every block), returned to every thread.
+__device__ inline double block_total(double const* __restrict__ partial, int64_t np) {
+    __shared__ double tot;
+    double v = 0.0;
+    for (int64_t i = threadIdx.x; i < np; i += kBlock) v += partial[i];
+    v = block_sum(v);
+    if (threadIdx.x == 0) tot = v;
+    __syncthreads();
+    return tot;
+}
+
+// alpha = s[rho] / sum(pq[0 .. npq));  x += alpha p;  r -= alpha q;  partial sums of r.r.  Every block
+// reduces the p.q partials itself, so no separate reduction launch sits between the apply and this.
+template <class T, class TX>
+__global__ __launch_bounds__(kBlock) void cg_xr_fused(TX* __restrict__ x, T* __restrict__ r, T const* __restrict__ p,
+                                                     T const* __restrict__ q, int64_t n, double const* __restrict__ s,
+                                                     int irho, double const* __restrict__ pq, int64_t npq,
+                                                     double* __restrict__ partial) {
+    const double alpha = s[irho] / block_total(pq, npq);
+    const T a = T(alpha);
+    double v = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBlock) {
+        x[i] += TX(a) * TX(p[i]);
+        T ri = r[i] - a * q[i];
+        r[i] = ri;
+        v += double(ri) * double(ri);
+    }
+    v = block_sum(v);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// rho_new = sum(partial[0 .. np)), written to s[inew] (block 0);  p = zr + (rho_new / s[iold]) p.
+template <class T>
+__global__ __launch_bounds__(kBlock) void cg_p_fused(T* __restrict__ p, T const* __restrict__ zr, int64_t n,
+                                                    double* __restrict__ s, int inew, int iold,
+                                                    double const* __restrict__ partial, int64_t np) {
+    const double rho = block_total(partial, np);
+    if (blockIdx.x == 0 && threadIdx.x == 0) s[inew] = rho;
+    const T beta = T(rho / s[iold]);
+    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBlock)
+        p[i] = zr[i] + beta * p[i];
 }
 
 // alpha = s[rr] / s[pq];  x += alpha p;  r -= alpha q;  partial sums of r.r
@@ -137,7 +181,8 @@ double read_scalar(double const* d, hipStream_t s) {
 // M = null is plain CG (rho = r.r).  w: work vector for M r.
 template <class T, class TX>
 int cg(Operator& op, BlockJacobi const* M, T const* rhs, TX* z, T* r, T* p, T* q, T* w, double* s, double* partial,
-       int64_t n, double target_rr, int max_it, int check_every, bool graphs, hipStream_t st, double* rr_out) {
+       double* pqp, int64_t n, double target_rr, int max_it, int check_every, bool graphs, hipStream_t st,
+       double* rr_out) {
     const dim3 g(kGrid), b(kBlock);
     hipLaunchKernelGGL((fill<TX>), g, b, 0, st, z, TX(0), n);
     hipLaunchKernelGGL((copy_to<T>), g, b, 0, st, r, rhs, n);
@@ -157,18 +202,22 @@ int cg(Operator& op, BlockJacobi const* M, T const* rhs, TX* z, T* r, T* p, T* q
         *rr_out = rr;
         return 0;
     }
+    // one iteration: q = A p with the partials of p.q from the apply's store (or a dot pass for the
+    // kernels without that epilogue); x, r update with the r.r partials; [z = M r, its r.z partials];
+    // p update, which reduces the rho partials itself and records rho in s for the next step
     auto iteration = [&](int c) {
-        op.apply(p, q);
-        hipLaunchKernelGGL((dot_partial<T>), g, b, 0, st, p, q, n, partial);
-        hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + 2);
-        hipLaunchKernelGGL((cg_xr<T, TX>), g, b, 0, st, z, r, p, q, n, s, c, 2, partial);
-        hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (M ? 4 : 1 - c));
+        int64_t npq = op.apply_dot(p, q, pqp);
+        if (npq == 0) {
+            hipLaunchKernelGGL((dot_partial<T>), g, b, 0, st, p, q, n, pqp);
+            npq = kGrid;
+        }
+        hipLaunchKernelGGL((cg_xr_fused<T, TX>), g, b, 0, st, z, r, p, q, n, s, c, pqp, npq, partial);
         if (M) {
+            hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + 4);
             M->apply(r, w, st);
             hipLaunchKernelGGL((dot_partial<T>), g, b, 0, st, r, w, n, partial);
-            hipLaunchKernelGGL(finalize_sum, dim3(1), b, 0, st, partial, s + (1 - c));
         }
-        hipLaunchKernelGGL((cg_p<T>), g, b, 0, st, p, M ? w : r, n, s, 1 - c, c);
+        hipLaunchKernelGGL((cg_p_fused<T>), g, b, 0, st, p, M ? w : r, n, s, 1 - c, c, partial, int64_t(kGrid));
     };
     // check_every iterations between host checks; an even count keeps the rho slots in phase, so
     // the same captured sequence is replayed for every chunk
@@ -214,6 +263,8 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
     if (o.precond == Precond_BlockJacobi) M = &op.block_jacobi();
     else if (o.precond != Precond_None) throw std::invalid_argument("unknown preconditioner");
     partial.alloc(kGrid);
+    Buf<double> pqp; // partial sums of p.q from the apply's store
+    pqp.alloc(std::size_t(std::max<int64_t>(kGrid, op.dot_partials_capacity())));
     SolveResult res{};
     KHIP(hipStreamSynchronize(st));
     auto t0 = std::chrono::steady_clock::now();
@@ -235,8 +286,8 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
         q.alloc(n);
         if (M) w.alloc(n);
         double rr = 0.0;
-        res.iterations = cg<double, double>(op, M, b, x, r.p, p.p, q.p, w.p, s.p, partial.p, n, target, o.max_iter,
-                                            o.check_every, o.graphs != 0, st, &rr);
+        res.iterations = cg<double, double>(op, M, b, x, r.p, p.p, q.p, w.p, s.p, partial.p, pqp.p, n, target,
+                                            o.max_iter, o.check_every, o.graphs != 0, st, &rr);
         // report the true residual ||b - A x|| / ||b||
         op.apply(x, q.p);
         hipLaunchKernelGGL(residual, g, bl, 0, st, b, q.p, r.p, (float*)nullptr, n, partial.p);
@@ -267,7 +318,7 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
             // FP32 CG on A d = r to relative tolerance inner_rtol, then the FP64 update x += d
             double inner_target = o.inner_rtol * o.inner_rtol * rr;
             double rr_in = 0.0;
-            int its = cg<float, float>(op, M, r32.p, d32.p, w32.p, p32.p, q32.p, z32.p, s.p, partial.p, n,
+            int its = cg<float, float>(op, M, r32.p, d32.p, w32.p, p32.p, q32.p, z32.p, s.p, partial.p, pqp.p, n,
                                        inner_target, o.max_iter - inner_total, o.check_every, o.graphs != 0, st,
                                        &rr_in);
             inner_total += its;
@@ -322,11 +373,13 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
         // iteration = part A (apply, step, r.r) + part B (preconditioner, direction); the host
         // check sits between A and B, so a chunk between checks is A B A B ... A
         auto partA = [&](int c) {
-            op.apply(p32.p, q32.p);
-            hipLaunchKernelGGL((dot_partial<float>), g, bl, 0, st, p32.p, q32.p, n, partial.p);
-            hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + 2);
-            hipLaunchKernelGGL((cg_xr<float, float>), g, bl, 0, st, xa32.p, r32.p, p32.p, q32.p, n, s.p, c, 2,
-                               partial.p);
+            int64_t npq = op.apply_dot(p32.p, q32.p, pqp.p);
+            if (npq == 0) {
+                hipLaunchKernelGGL((dot_partial<float>), g, bl, 0, st, p32.p, q32.p, n, pqp.p);
+                npq = kGrid;
+            }
+            hipLaunchKernelGGL((cg_xr_fused<float, float>), g, bl, 0, st, xa32.p, r32.p, p32.p, q32.p, n, s.p, c,
+                               pqp.p, npq, partial.p);
             hipLaunchKernelGGL(finalize_sum, dim3(1), bl, 0, st, partial.p, s.p + (M ? 4 : 1 - c));
         };
         auto partB = [&](int c) {
