@@ -114,6 +114,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
 
     __shared__ T sT[NF * nq * NbfP];
     __shared__ int sP[2 * NF * Nbf];
+    __shared__ T sW[nq];
     __shared__ T sX[W][16 * BSP];
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, j = lane & 15, g = lane >> 4;
@@ -168,6 +169,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
             const int i = i0 + tid;
             if (i < nT) sT[(i / Nbf) * NbfP + i % Nbf] = a.T0[i];
         }
+        if (tid < nq) sW[tid] = a.w[tid];
         __syncthreads(); // sP ready
         int const* const ip = sP + NF * Nbf + lf * Nbf;
 #pragma unroll
@@ -206,7 +208,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
             load_rec(min(t + 1, FT - 1), rn); // next tile's record, one tile ahead
             // quadrature weight, zero on lanes without a facet point (every lane computes finite
             // values: padding facets carry the identity Jacobian, empty columns a zero vector)
-            const T wq = act ? a.w[qc] : T(0);
+            const T wq = act ? sW[qc] : T(0);
             const T ws = sd == 0 ? -wq : wq; // the traction term lifts with -w on side 0, +w on side 1
             acc C[Q];
 #pragma unroll
@@ -453,6 +455,8 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
     const int nel = live ? int(min(int64_t(16), a.num_elements - e0)) : 0;
     const bool valid = j < nel;
     const int64_t gj = live ? grp : 0;
+    T const* const xw = x + (live ? e0 : 0) * BS; // this wave's 16 element vectors (contiguous)
+    const int nout = nel * BS;
 
     // point data of point q = 16 b + mfma_row(g, i) (clamped)
     auto load_pd = [&](int b, T (&pd)[4][NCH]) {
@@ -466,7 +470,6 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
     };
     // the element's facet slots for output entry i0 + lane (16 consecutive elements: contiguous)
     T const* const lsl = Lc + (live ? e0 : 0) * NF * BS;
-    const int nout = nel * BS;
     auto load_slots = [&](T (&ls)[NO]) {
 #pragma unroll
         for (int o = 0; o < NO; ++o) {
@@ -489,12 +492,14 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
     }
     T* const U = sU[wv];
     if (live) {
-        T const* src = x + e0 * BS;
-        const int n = nel * BS;
+        // every load first (clamped addresses), then the LDS stores: no load waits on another
+        T v[NO];
 #pragma unroll
-        for (int i0 = 0; i0 < 16 * BS; i0 += 64) {
-            const int i = i0 + lane;
-            if (i < 16 * BS) U[(i / BS) * BSP + i % BS] = i < n ? src[i] : T(0);
+        for (int o = 0; o < NO; ++o) v[o] = xw[min(64 * o + lane, nout - 1)];
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            const int i = 64 * o + lane;
+            if (i < 16 * BS) U[(i / BS) * BSP + i % BS] = i < nout ? v[o] : T(0);
         }
     }
     __syncthreads();
@@ -654,12 +659,17 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
         }
         if constexpr (!DEEP) load_slots(ls);
     }
+    // x again for x . y (one column only: Krylov solvers), all loads issued before the first use
+    T xv[NO];
+    if (a.dot && live) {
+#pragma unroll
+        for (int o = 0; o < NO; ++o) xv[o] = xw[min(64 * o + lane, nout - 1)];
+    }
     wave_sync();
-    double dot = 0.0; // x . y over this wave's elements (one column only: Krylov solvers)
+    double dot = 0.0;
     if (live) {
         // y = volume + the element's facet slots
         T* dst = y + e0 * BS;
-        T const* src = x + e0 * BS;
 #pragma unroll
         for (int o = 0; o < NO; ++o) {
             const int i = 64 * o + lane;
@@ -667,7 +677,7 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
                 const int jj = i / BS, rem = i - jj * BS;
                 const T v = U[jj * BSP + rem] + ls[o];
                 dst[i] = v;
-                if (a.dot) dot += double(src[i]) * double(v);
+                if (a.dot) dot += double(xv[o]) * double(v);
             }
         }
     }
