@@ -368,6 +368,44 @@ def run_greens(args):
         "single_rhs_iterations_per_second"]
     xs = Xs[0].cpu().numpy()
     out["column0_rel_diff_lockstep_vs_single"] = float(np.linalg.norm(xs - x.cpu().numpy()) / np.linalg.norm(xs))
+    if args.pmg_levels != "none":
+        # the same block of 64 right-hand sides solved to 1e-10 by flexible CG with a p-multigrid
+        # V-cycle (coarse levels on the same mesh and material)
+        degs = [int(v) for v in args.pmg_levels.split(",")]
+        if degs[0] != op.degree:
+            raise ValueError("--pmg-levels must start at the fine degree %d" % op.degree)
+        del Xf, Yf
+        t0 = time.perf_counter()
+        coarse = []
+        for p in degs[1:]:
+            c = on_stream(T.DGOperator(mesh, T.ELASTICITY, p))
+            Xc = c.volume_points()
+            c.set_material(lf(Xc), mf(Xc))
+            coarse.append(c)
+        M = T.PMultigrid([op] + coarse, smooth=args.pmg_smooth)
+        lmax = M.setup()
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        log("greens p-multigrid %s: setup %.1fs, lambda_max %s" % (degs, t_setup, lmax))
+        Xp = torch.zeros_like(B)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rp = M.solve_multi(B, Xp, rtol=1e-10, max_iter=args.pmg_max_iter)
+        torch.cuda.synchronize()
+        t_p = time.perf_counter() - t1
+        log("greens p-multigrid solve %.2fs" % t_p, {k: rp[k] for k in ("iterations", "outer_iterations",
+                                                                          "rel_residual", "converged")})
+        Gp = op.fault_traction(Xp, fdat)
+        torch.cuda.synchronize()
+        out["pmg"] = {
+            "levels": degs, "smooth": M.options["smooth"], "coarse_rtol": M.options["coarse_rtol"],
+            "eig_interval": [M.options["eig_lo"], M.options["eig_hi"]], "lambda_max": [float(v) for v in lmax],
+            "setup_seconds": t_setup, "seconds_to_1e-10": t_p, "iterations": int(rp["iterations"]),
+            "coarse_cg_iterations": int(rp["outer_iterations"]), "converged": int(rp["converged"]),
+            "worst_rel_residual": float(rp["rel_residual"]),
+            "max_column_iterations": int(np.max(rp["column_iterations"])),
+            "green_columns_rel_diff_vs_lockstep": float(torch.linalg.norm(Gp - G) / torch.linalg.norm(Gp)),
+        }
     return out
 
 
@@ -719,6 +757,10 @@ def main():
                          "(40 for the 384,000-element study); greens: batched Green's-function solves; "
                          "adjoint: misfit gradient with finite-difference checks; gradient: the gradient kernels alone")
     ap.add_argument("--blocks", type=int, default=2, help="greens: number of 64-column blocks timed")
+    ap.add_argument("--pmg-levels", default="6,3,1",
+                    help="greens: degrees of the p-multigrid levels for the solve to 1e-10 ('none': skip)")
+    ap.add_argument("--pmg-smooth", type=int, default=3, help="greens: Chebyshev steps per smoothing")
+    ap.add_argument("--pmg-max-iter", type=int, default=300, help="greens: flexible-CG iteration cap")
     ap.add_argument("--meshes", type=int, default=32, help="multimesh: number of independent meshes")
     ap.add_argument("--precond", choices=["block-jacobi", "none"], default="block-jacobi",
                     help="precision: preconditioner of both solves")

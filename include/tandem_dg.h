@@ -231,6 +231,46 @@ int tdg_block_jacobi_apply(tdg_op* op, double const* r, double* z);
 int tdg_block_jacobi_apply_batched(tdg_op* op, double const* R, double* Z, int ncols, int64_t ld);
 int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block);
 
+/* p-multigrid over polynomial degree (tandem: PCMG set up by PetscLinearSolver::setup_mg,
+ * app/common/PetscLinearSolver.cpp:53-114, with the levels of MGConfig::levels,
+ * app/common/MGConfig.cpp:11-40, and the options of examples/options/mg_cheby.cfg).
+ * levels[0] is the fine operator, levels[1..nlevels-1] operators of strictly decreasing degree
+ * built by the caller on the same mesh with the same material (tandem forms the coarse matrices
+ * as P^T A P; here each coarse level is its own matrix-free operator, and with galerkin_penalty it
+ * takes the fine level's facet penalty, which makes it the fine form restricted to the coarse space
+ * up to the quadrature of the material).  The coarse operators are switched to the fine operator's
+ * stream.  Levels are joined by the nodal interpolation (tdg_interpolation_matrix) and its
+ * transpose; each level below the coarsest is smoothed by `smooth` Chebyshev steps on
+ * D^-1 A (D: element-block diagonal) over [eig_lo, eig_hi] x lambda_max (lambda_max from eig_iters
+ * Lanczos steps); the coarsest level is solved by block-Jacobi CG to coarse_rtol (at most
+ * coarse_max_iter iterations; in FP32 with coarse_fp32).  With vcycle_fp32 the V-cycle (smoothing,
+ * transfers) runs in FP32 on the FP32 copies of the operators; the outer CG stays FP64.
+ * tdg_pmg_create keeps the level pointers (the operators must outlive the hierarchy);
+ * tdg_pmg_setup builds the block-Jacobi inverses and the eigenvalue estimates (lambda_max: host
+ * array of nlevels-1, may be NULL); tdg_pmg_vcycle applies one V-cycle z = M^-1 r to ncols device
+ * FP64 columns at stride ld; tdg_solve_multi_pmg solves ncols right-hand sides together by flexible
+ * CG preconditioned with one V-cycle per iteration (FP64; opts->rtol and opts->max_iter are used,
+ * the outputs as for tdg_solve_multi; result->outer_iterations = coarse CG iterations summed over
+ * the V-cycles). */
+typedef struct tdg_pmg tdg_pmg;
+typedef struct {
+    int smooth;
+    double eig_lo, eig_hi;
+    int eig_iters;
+    double coarse_rtol;
+    int coarse_max_iter;
+    int galerkin_penalty;
+    int coarse_fp32;
+    int vcycle_fp32;
+} tdg_pmg_opts;
+void tdg_pmg_default_opts(tdg_pmg_opts* opts);
+int tdg_pmg_create(tdg_op* const* levels, int nlevels, tdg_pmg_opts const* opts, tdg_pmg** out);
+int tdg_pmg_destroy(tdg_pmg* pmg);
+int tdg_pmg_setup(tdg_pmg* pmg, double* lambda_max);
+int tdg_pmg_vcycle(tdg_pmg* pmg, double const* r, double* z, int ncols, int64_t ld);
+int tdg_solve_multi_pmg(tdg_pmg* pmg, double const* B, double* X, int ncols, int64_t ld, tdg_solve_opts const* opts,
+                        tdg_solve_result* result, int* iterations, double* rel_residual);
+
 /* Adjoint gradient kernel (elasticity): g_lam, g_mu (device, elements x Nq) = d/d(lam_Q, mu_Q) of
  * psi^T A(lam, mu) u for device vectors u and psi, through everything the coefficients enter: the
  * L2 projection to nodal coefficients and the volume weights (Elasticity.cpp:104-175,

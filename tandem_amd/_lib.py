@@ -33,6 +33,12 @@ class SolveResult(C.Structure):
                 ("rel_residual", C.c_double), ("seconds", C.c_double), ("converged", C.c_int)]
 
 
+class PMGOpts(C.Structure):
+    _fields_ = [("smooth", C.c_int), ("eig_lo", C.c_double), ("eig_hi", C.c_double), ("eig_iters", C.c_int),
+                ("coarse_rtol", C.c_double), ("coarse_max_iter", C.c_int), ("galerkin_penalty", C.c_int),
+                ("coarse_fp32", C.c_int), ("vcycle_fp32", C.c_int)]
+
+
 class MeshDesc(C.Structure):
     _fields_ = [
         ("dim", C.c_int),
@@ -55,6 +61,8 @@ EXPORTS = [
     "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_solve_multi", "tdg_solve_segments", "tdg_autotune", "tdg_autotune_shapes", "tdg_set_shape", "tdg_rhs", "tdg_rhs_device", "tdg_traction", "tdg_block_jacobi_build", "tdg_block_jacobi_apply", "tdg_block_jacobi_apply_batched", "tdg_block_jacobi_inverse", "tdg_material_gradient",
     "tdg_material_gradient_f32", "tdg_fault_setup", "tdg_fault_slip", "tdg_fault_traction", "tdg_inverse_mass", "tdg_wave_rhs", "tdg_interpolation_matrix", "tdg_interpolate",
     "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
+    "tdg_pmg_default_opts", "tdg_pmg_create", "tdg_pmg_destroy", "tdg_pmg_setup", "tdg_pmg_vcycle",
+    "tdg_solve_multi_pmg",
 ]
 
 
@@ -102,6 +110,14 @@ def lib():
     L.tdg_interpolation_matrix.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
     L.tdg_interpolate.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, vp, vp, vp]
     L.tdg_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]
+    L.tdg_pmg_default_opts.argtypes = [C.POINTER(PMGOpts)]
+    L.tdg_pmg_default_opts.restype = None
+    L.tdg_pmg_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(PMGOpts), C.POINTER(vp)]
+    L.tdg_pmg_destroy.argtypes = [vp]
+    L.tdg_pmg_setup.argtypes = [vp, _dp]
+    L.tdg_pmg_vcycle.argtypes = [vp, vp, vp, C.c_int, C.c_int64]
+    L.tdg_solve_multi_pmg.argtypes = [vp, vp, vp, C.c_int, C.c_int64, C.POINTER(SolveOpts), C.POINTER(SolveResult),
+                                      _ip32, _dp]
     L.tdg_solve_multi.argtypes = [vp, vp, vp, C.c_int, C.c_int64, C.POINTER(SolveOpts), C.POINTER(SolveResult),
                                   _ip32, _dp]
     L.tdg_solve_segments.argtypes = [vp, vp, vp, C.c_int, _ip64, C.POINTER(SolveOpts), C.POINTER(SolveResult),

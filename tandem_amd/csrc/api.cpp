@@ -8,6 +8,7 @@
 #include "kernels/mfma_layout.h"
 #include "solver/krylov.h"
 #include "solver/block_jacobi.h"
+#include "solver/pmg.h"
 
 #include <hip/hip_runtime.h>
 
@@ -258,6 +259,90 @@ int tdg_solve_multi(tdg_op* op, double const* B, double* X, int ncols, int64_t l
         auto r = tdg::solve_multi(o, B, X, ncols, ld, so);
         result->iterations = r.total_iterations;
         result->outer_iterations = r.replacements;
+        result->inner_iterations = r.total_iterations;
+        double worst = 0.0;
+        for (int c = 0; c < ncols; ++c) {
+            worst = std::max(worst, r.rel_residual[c]);
+            if (iterations) iterations[c] = r.iterations[c];
+            if (rel_residual) rel_residual[c] = r.rel_residual[c];
+        }
+        result->rel_residual = worst;
+        result->seconds = r.seconds;
+        result->converged = r.converged;
+    });
+}
+
+struct tdg_pmg {
+    std::unique_ptr<tdg::PMG> impl;
+};
+
+void tdg_pmg_default_opts(tdg_pmg_opts* opts) {
+    if (!opts) return;
+    tdg::PMGOptions d;
+    opts->smooth = d.smooth;
+    opts->eig_lo = d.eig_lo;
+    opts->eig_hi = d.eig_hi;
+    opts->eig_iters = d.eig_iters;
+    opts->coarse_rtol = d.coarse_rtol;
+    opts->coarse_max_iter = d.coarse_max_iter;
+    opts->galerkin_penalty = d.galerkin_penalty;
+    opts->coarse_fp32 = d.coarse_fp32;
+    opts->vcycle_fp32 = d.vcycle_fp32;
+}
+
+int tdg_pmg_create(tdg_op* const* levels, int nlevels, tdg_pmg_opts const* opts, tdg_pmg** out) {
+    return guarded([&] {
+        require(levels && out && nlevels >= 2, "bad p-multigrid arguments");
+        *out = nullptr;
+        std::vector<tdg::Operator*> ops;
+        for (int l = 0; l < nlevels; ++l) ops.push_back(&ready(levels[l]));
+        tdg::PMGOptions o;
+        if (opts) {
+            o.smooth = opts->smooth;
+            o.eig_lo = opts->eig_lo;
+            o.eig_hi = opts->eig_hi;
+            o.eig_iters = opts->eig_iters;
+            o.coarse_rtol = opts->coarse_rtol;
+            o.coarse_max_iter = opts->coarse_max_iter;
+            o.galerkin_penalty = opts->galerkin_penalty;
+            o.coarse_fp32 = opts->coarse_fp32;
+            o.vcycle_fp32 = opts->vcycle_fp32;
+        }
+        auto h = std::make_unique<tdg_pmg>();
+        h->impl = std::make_unique<tdg::PMG>(ops, o);
+        *out = h.release();
+    });
+}
+
+int tdg_pmg_destroy(tdg_pmg* pmg) {
+    return guarded([&] { delete pmg; });
+}
+
+int tdg_pmg_setup(tdg_pmg* pmg, double* lambda_max) {
+    return guarded([&] {
+        require(pmg && pmg->impl, "null p-multigrid hierarchy");
+        pmg->impl->setup();
+        if (lambda_max)
+            for (std::size_t l = 0; l < pmg->impl->lambda_max().size(); ++l) lambda_max[l] = pmg->impl->lambda_max()[l];
+    });
+}
+
+int tdg_pmg_vcycle(tdg_pmg* pmg, double const* r, double* z, int ncols, int64_t ld) {
+    return guarded([&] {
+        require(pmg && pmg->impl && r && z && r != z, "bad V-cycle arguments");
+        pmg->impl->vcycle(r, z, ncols, ld);
+    });
+}
+
+int tdg_solve_multi_pmg(tdg_pmg* pmg, double const* B, double* X, int ncols, int64_t ld, tdg_solve_opts const* opts,
+                        tdg_solve_result* result, int* iterations, double* rel_residual) {
+    return guarded([&] {
+        require(pmg && pmg->impl && B && X && opts && result && B != X, "bad p-multigrid solve arguments");
+        auto so = solve_options(opts);
+        const int64_t before = pmg->impl->coarse_iterations();
+        auto r = tdg::solve_multi_pmg(*pmg->impl, B, X, ncols, ld, so);
+        result->iterations = r.total_iterations;
+        result->outer_iterations = int(pmg->impl->coarse_iterations() - before);
         result->inner_iterations = r.total_iterations;
         double worst = 0.0;
         for (int c = 0; c < ncols; ++c) {

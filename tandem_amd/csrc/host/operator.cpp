@@ -394,6 +394,11 @@ void Operator::set_material(double const* c0_Q, double const* c1_Q) {
             o[3] = 0;
         }
     if (E_ >= (std::size_t(1) << 31) || F >= (std::size_t(1) << 31)) throw std::runtime_error("mesh too large for 32-bit indices");
+    drop_device();
+    ready_ = true;
+}
+
+void Operator::drop_device() {
     d64_.reset();
     d32_.reset();
     m64_.reset();
@@ -404,7 +409,13 @@ void Operator::set_material(double const* c0_Q, double const* c1_Q) {
     bj_.reset();
     g64_.reset();
     g32_.reset();
-    ready_ = true;
+}
+
+void Operator::set_penalty(std::vector<double> const& pen) {
+    if (!ready_) throw std::logic_error("set_material() must be called before set_penalty()");
+    if (pen.size() != penalty_.size()) throw std::invalid_argument("penalty: one value per facet expected");
+    penalty_ = pen;
+    drop_device();
 }
 
 template <class T> static std::vector<T> cast(std::vector<double> const& v);

@@ -126,6 +126,12 @@ public:
     // onto the nodal P_N material space, builds all per-point data and uploads it.
     void set_material(double const* c0_Q, double const* c1_Q);
 
+    // SIPG penalty per facet; set_penalty replaces it (same facet numbering, after set_material) and
+    // drops the device copies (p-multigrid coarse levels take the fine level's penalty).
+    std::vector<double> const& penalty() const { return penalty_; }
+    void set_penalty(std::vector<double> const& pen);
+    unsigned quantities() const { return Q_; }
+
     // Uploads both precisions to the device (done implicitly by the first apply).
     void upload_device();
     template <class T> DeviceOperator<T> const& device();
@@ -212,6 +218,7 @@ public:
     double apply_flops() const;
 
 private:
+    void drop_device();
     void build_tables();
     void build_geometry(std::vector<double> const* geometry_nodes);
     template <class T> void upload(DeviceOperator<T>& d);

@@ -15,6 +15,7 @@
 
 #include "../host/operator.h"
 #include "block_jacobi.h"
+#include "multi_kernels.h"
 #include "progress.h"
 #include "replay.h"
 
@@ -29,158 +30,10 @@
 
 namespace tdg {
 
-namespace {
-
-#define MHIP(call)                                                                                 \
-    do {                                                                                           \
-        hipError_t e_ = (call);                                                                    \
-        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_)); \
-    } while (0)
-
-constexpr int kB = 256;
-constexpr int kG = 256;  // partial sums per column
-constexpr int kS = 8;    // scalars per column: 0,1 rho; 2 p.Ap; 3 b.b; 4 r.r; 5 active; 6 replace
-
-// Where column c lives: c ld with n entries, or the device tables off/len (segments).
-struct Cols {
-    int64_t ld = 0, n = 0;
-    int64_t const* off = nullptr;
-    int64_t const* len = nullptr;
-    __device__ int64_t base(int64_t c) const { return off ? off[c] : c * ld; }
-    __device__ int64_t size(int64_t c) const { return off ? len[c] : n; }
-};
-
-__device__ inline double bsum(double v) {
-    __shared__ double s[kB / 64];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) s[w] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < kB / 64; ++i) r += s[i];
-    return r;
-}
-
-template <class A, class B>
-__global__ __launch_bounds__(kB) void mdot(A const* __restrict__ a, B const* __restrict__ b, Cols L,
-                                           double* __restrict__ partial) {
-    const int64_t c = blockIdx.y, n = L.size(c);
-    A const* ac = a + L.base(c);
-    B const* bc = b + L.base(c);
-    double v = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
-        v += double(ac[i]) * double(bc[i]);
-    v = bsum(v);
-    if (threadIdx.x == 0) partial[c * kG + blockIdx.x] = v;
-}
-
-// S[c][slot] = sum of the column's partials (only where mask slot is set, if msk >= 0)
-__global__ __launch_bounds__(kB) void mfinal(double const* __restrict__ partial, double* __restrict__ S, int slot,
-                                             int msk) {
-    const int c = blockIdx.x;
-    double v = 0.0;
-    for (int i = threadIdx.x; i < kG; i += kB) v += partial[c * kG + i];
-    v = bsum(v);
-    if (threadIdx.x == 0 && (msk < 0 || S[c * kS + msk] != 0.0)) S[c * kS + slot] = v;
-}
-
-// active columns: x += alpha p, r -= alpha q, alpha = rho / pAp; partial sums of r.r
-template <class T, class TX>
-__global__ __launch_bounds__(kB) void mxr(TX* __restrict__ x, T* __restrict__ r, T const* __restrict__ p,
-                                          T const* __restrict__ q, Cols L, double const* __restrict__ S,
-                                          int irho, double* __restrict__ partial) {
-    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
-    double const* s = S + c * kS;
-    const bool act = s[5] != 0.0;
-    const T a = act ? T(s[irho] / s[2]) : T(0);
-    TX* xc = x + o;
-    T* rc = r + o;
-    T const* pc = p + o;
-    T const* qc = q + o;
-    double v = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) {
-        T ri = rc[i];
-        if (act) {
-            xc[i] += TX(a) * TX(pc[i]);
-            ri = ri - a * qc[i];
-            rc[i] = ri;
-        }
-        v += double(ri) * double(ri);
-    }
-    v = bsum(v);
-    if (threadIdx.x == 0) partial[c * kG + blockIdx.x] = v;
-}
-
-// active columns: p = z + (rho_new / rho_old) p
-template <class T>
-__global__ __launch_bounds__(kB) void mp(T* __restrict__ p, T const* __restrict__ z, Cols L,
-                                         double const* __restrict__ S, int inew, int iold) {
-    const int64_t c = blockIdx.y, n = L.size(c);
-    double const* s = S + c * kS;
-    if (s[5] == 0.0) return;
-    const T beta = T(s[inew] / s[iold]);
-    T* pc = p + L.base(c);
-    T const* zc = z + L.base(c);
-    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
-        pc[i] = zc[i] + beta * pc[i];
-}
-
-template <class T>
-__global__ __launch_bounds__(kB) void mcopy(T* __restrict__ d, T const* __restrict__ s, Cols L) {
-    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
-    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
-        d[o + i] = s[o + i];
-}
-template <class T>
-__global__ __launch_bounds__(kB) void mfill(T* __restrict__ d, T v, Cols L) {
-    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
-    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) d[o + i] = v;
-}
-
-// columns flagged for replacement (S[c][6]): x += xa (FP64), xa = 0
-__global__ __launch_bounds__(kB) void mfold(double* __restrict__ x, float* __restrict__ xa, Cols L,
-                                            double const* __restrict__ S) {
-    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
-    if (S[c * kS + 6] == 0.0) return;
-    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) {
-        x[o + i] += double(xa[o + i]);
-        xa[o + i] = 0.0f;
-    }
-}
-
-// flagged columns (or all if all != 0): r = b - Ax in FP64, r32 = float(r); partial sums of r.r
-__global__ __launch_bounds__(kB) void mresidual(double const* __restrict__ b, double const* __restrict__ Ax,
-                                                double* __restrict__ r, float* __restrict__ r32, Cols L,
-                                                double const* __restrict__ S, int all, double* __restrict__ partial) {
-    const int64_t c = blockIdx.y, n = L.size(c), o = L.base(c);
-    const bool on = all || S[c * kS + 6] != 0.0;
-    double v = 0.0;
-    if (on)
-        for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB) {
-            const double ri = b[o + i] - Ax[o + i];
-            r[o + i] = ri;
-            if (r32) r32[o + i] = float(ri);
-            v += ri * ri;
-        }
-    v = bsum(v);
-    if (threadIdx.x == 0) partial[c * kG + blockIdx.x] = v;
-}
-
-template <class T> struct DBuf {
-    T* p = nullptr;
-    ~DBuf() {
-        if (p) (void)hipFree(p);
-    }
-    void alloc(std::size_t n) { MHIP(hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T))); }
-};
-
-} // namespace
-
 // The lockstep recursions over ncols columns laid out by L (tot entries per vector).  Segments:
 // one apply of the whole operator per iteration; columns: one batched apply.
 MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols, Cols L, std::size_t tot,
-                             bool segments, SolveOptions const& o) {
+                             bool segments, SolveOptions const& o, MultiWork* w) {
     if (o.mode != Solve_FP64_CG && o.mode != Solve_FP32_CG_FP64_RELIABLE)
         throw std::invalid_argument("multi-column solves support FP64 CG and FP32 CG with FP64 replacement");
     op.upload_device();
@@ -199,9 +52,12 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         if (segments) M->apply(R, Z, st, 1, 0);
         else M->apply(R, Z, st, ncols, ld);
     };
-    DBuf<double> S, partial;
-    S.alloc(std::size_t(ncols) * kS);
-    partial.alloc(std::size_t(ncols) * kG);
+    MultiWork local;
+    MultiWork& W = w ? *w : local;
+    DBuf<double>& S = W.S;
+    DBuf<double>& partial = W.partial;
+    S.ensure(std::size_t(ncols) * kS);
+    partial.ensure(std::size_t(ncols) * kG);
     std::vector<double> hs(std::size_t(ncols) * kS, 0.0);
     MultiSolveResult res;
     res.rel_residual.assign(ncols, 0.0);
@@ -238,11 +94,11 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
     hipLaunchKernelGGL((mfill<double>), g, bl, 0, st, X, 0.0, L);
 
     if (o.mode == Solve_FP64_CG) {
-        DBuf<double> r, p, q, z;
-        r.alloc(tot);
-        p.alloc(tot);
-        q.alloc(tot);
-        if (M) z.alloc(tot);
+        DBuf<double>&r = W.r, &p = W.p, &q = W.q, &z = W.z;
+        r.ensure(tot);
+        p.ensure(tot);
+        q.ensure(tot);
+        if (M) z.ensure(tot);
         push();
         hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, r.p, B, L);
         hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, L, partial.p);
@@ -309,15 +165,15 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         MHIP(hipGetLastError());
         pull();
     } else {
-        DBuf<double> r, Ax;
-        DBuf<float> r32, xa, p, q, z;
-        r.alloc(tot);
-        Ax.alloc(tot);
-        r32.alloc(tot);
-        xa.alloc(tot);
-        p.alloc(tot);
-        q.alloc(tot);
-        if (M) z.alloc(tot);
+        DBuf<double>&r = W.r, &Ax = W.Ax;
+        DBuf<float>&r32 = W.r32, &xa = W.xa, &p = W.pf, &q = W.qf, &z = W.zf;
+        r.ensure(tot);
+        Ax.ensure(tot);
+        r32.ensure(tot);
+        xa.ensure(tot);
+        p.ensure(tot);
+        q.ensure(tot);
+        if (M) z.ensure(tot);
         push();
         hipLaunchKernelGGL((mfill<float>), g, bl, 0, st, xa.p, 0.0f, L);
         hipLaunchKernelGGL((mfill<double>), g, bl, 0, st, Ax.p, 0.0, L);

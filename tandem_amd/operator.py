@@ -498,6 +498,71 @@ class DGOperator:
             self._h = None
 
 
+class PMultigrid:
+    """p-multigrid hierarchy over DG operators of decreasing degree on one mesh (levels[0] the fine
+    one; the coarse operators are built by the caller with the same material).  Mirrors the PCMG
+    that PetscLinearSolver::setup_mg (app/common/PetscLinearSolver.cpp:53-114) builds: nodal
+    interpolation between levels, Chebyshev / block-Jacobi smoothing, a block-Jacobi CG coarse
+    solve to coarse_rtol.  The operators must outlive the hierarchy (it keeps references)."""
+
+    def __init__(self, levels, smooth=None, eig_lo=None, eig_hi=None, eig_iters=None, coarse_rtol=None,
+                 coarse_max_iter=None, galerkin_penalty=None, coarse_fp32=None, vcycle_fp32=None):
+        o = _lib.PMGOpts()
+        lib().tdg_pmg_default_opts(C.byref(o))
+        for k, v in (("smooth", smooth), ("eig_lo", eig_lo), ("eig_hi", eig_hi), ("eig_iters", eig_iters),
+                     ("coarse_rtol", coarse_rtol), ("coarse_max_iter", coarse_max_iter),
+                     ("galerkin_penalty", galerkin_penalty), ("coarse_fp32", coarse_fp32),
+                     ("vcycle_fp32", vcycle_fp32)):
+            if v is not None:
+                setattr(o, k, type(getattr(o, k))(v))
+        self.options = {k: getattr(o, k) for k, _ in _lib.PMGOpts._fields_}
+        self.levels = list(levels)
+        arr = (C.c_void_p * len(self.levels))(*[op._h for op in self.levels])
+        h = C.c_void_p()
+        check(lib().tdg_pmg_create(arr, len(self.levels), C.byref(o), C.byref(h)))
+        self._h = h
+        self.n = self.levels[0].n
+
+    def setup(self):
+        """Block-Jacobi inverses and eigenvalue estimates; returns lambda_max per smoothed level."""
+        lm = np.zeros(max(len(self.levels) - 1, 1))
+        check(lib().tdg_pmg_setup(self._h, lm))
+        return lm[: len(self.levels) - 1]
+
+    def vcycle(self, R, Z):
+        """Z = M^-1 R for (ncols, ld) float64 GPU tensors (one V-cycle)."""
+        rp = _check(R, "vcycle: R", cols=(None, self.n))
+        zp = _check(Z, "vcycle: Z", cols=(R.shape[0], self.n), like=R)
+        if R.shape != Z.shape:
+            raise ValueError("vcycle: R and Z shapes differ")
+        check(lib().tdg_pmg_vcycle(self._h, rp, zp, R.shape[0], R.shape[1]))
+
+    def solve_multi(self, B, X, rtol=1e-10, max_iter=1000):
+        """Solve A X[c] = B[c] (rows of (ncols, ld) float64 GPU tensors) by flexible CG with one
+        V-cycle per iteration.  Returns the statistics with per-column iterations and relative
+        residuals; outer_iterations = coarse CG iterations over all V-cycles."""
+        bp = _check(B, "solve_multi: B", cols=(None, self.n))
+        xp = _check(X, "solve_multi: X", cols=(B.shape[0], self.n), like=B)
+        if X.shape != B.shape:
+            raise ValueError("solve_multi: B and X shapes differ")
+        nc, ld = B.shape
+        opts = _lib.SolveOpts(rtol, max_iter, 0, 1e-4, 1, 1, 0.1, 0, 0)
+        res = _lib.SolveResult()
+        its = np.zeros(nc, dtype=np.int32)
+        rel = np.zeros(nc)
+        check(lib().tdg_solve_multi_pmg(self._h, bp, xp, nc, ld, C.byref(opts), C.byref(res), its, rel))
+        out = {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
+        out["column_iterations"] = its
+        out["column_rel_residual"] = rel
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.tdg_pmg_destroy(h)
+            self._h = None
+
+
 class Elasticity(DGOperator):
     """Linear elasticity (tandem Elasticity): lam_Q, mu_Q at the volume points, or callables f(x)."""
 
