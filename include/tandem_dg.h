@@ -168,14 +168,16 @@ int tdg_traction(tdg_op* op, double const* x, double const* facet_q, double* t);
  * PetscLinearSolver::solve = KSPSolve with KSPCG on the shell matrix
  * (app/common/PetscLinearSolver.h:42, PetscLinearSolver.cpp:22-32); precond = TDG_PRECOND_BLOCK_JACOBI uses the
  * element-block Jacobi preconditioner (tandem's PetscDGMatrix + PCBJACOBI with one block per
- * element, PetscLinearSolver.cpp:14-15, PetscDGMatrix.cpp:14-77), built on first use.
+ * element, PetscLinearSolver.cpp:14-15, PetscDGMatrix.cpp:14-77), built on first use;
+ * TDG_PRECOND_BLOCK_JACOBI_F32 is the same preconditioner with FP32-stored inverse blocks applied in
+ * FP64 arithmetic by FP64 solves (half the bytes per application; FP32 solves always use them).
  * mode 0: FP64 CG.  mode 1: FP32 CG inner solves (to inner_rtol) inside an FP64 iterative
  * refinement loop whose residual b - A x and update x += d are FP64 (restarted each refinement).
  * mode 2: FP32 CG whose residual is replaced by the FP64 true residual, and whose accumulated FP32
  * update is added to the FP64 x, each time the residual drops by reliable_delta (refinement
  * without restarting the Krylov direction).  rtol applies to the FP64 true residual. */
 enum { TDG_SOLVE_FP64_CG = 0, TDG_SOLVE_FP32_CG_FP64_IR = 1, TDG_SOLVE_FP32_CG_FP64_RELIABLE = 2 };
-enum { TDG_PRECOND_NONE = 0, TDG_PRECOND_BLOCK_JACOBI = 1 };
+enum { TDG_PRECOND_NONE = 0, TDG_PRECOND_BLOCK_JACOBI = 1, TDG_PRECOND_BLOCK_JACOBI_F32 = 2 };
 typedef struct {
     double rtol;
     int max_iter;

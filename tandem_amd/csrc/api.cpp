@@ -223,7 +223,9 @@ static tdg::SolveOptions solve_options(tdg_solve_opts const* opts) {
     so.reliable_delta = opts->reliable_delta;
     so.precond = opts->precond;
     so.graphs = opts->graphs;
-    require(so.precond == TDG_PRECOND_NONE || so.precond == TDG_PRECOND_BLOCK_JACOBI, "unknown preconditioner");
+    require(so.precond == TDG_PRECOND_NONE || so.precond == TDG_PRECOND_BLOCK_JACOBI ||
+                so.precond == TDG_PRECOND_BLOCK_JACOBI_F32,
+            "unknown preconditioner");
     require(so.mode == TDG_SOLVE_FP64_CG || so.mode == TDG_SOLVE_FP32_CG_FP64_IR ||
                 so.mode == TDG_SOLVE_FP32_CG_FP64_RELIABLE,
             "unknown solve mode");

@@ -35,6 +35,9 @@ public:
     // leading dimension ld.
     void apply(double const* r, double* z, hipStream_t s, int ncols = 1, int64_t ld = 0) const;
     void apply(float const* r, float* z, hipStream_t s, int ncols = 1, int64_t ld = 0) const;
+    // FP64 vectors and arithmetic with the FP32-stored blocks (half the bytes of the FP64 blocks;
+    // the same fixed symmetric preconditioner for an FP64 solve as the FP32 one applies)
+    void apply_fp32_blocks(double const* r, double* z, hipStream_t s, int ncols = 1, int64_t ld = 0) const;
 
     // FP64 inverse block of element e (bs x bs, column-major) copied to the host, for checks.
     void inverse_block(int64_t e, double* out) const;
@@ -45,6 +48,20 @@ private:
     int64_t E_ = 0;
     double* inv64_ = nullptr; // [E][bs][bs] column-major blocks
     float* inv32_ = nullptr;
+};
+
+// The preconditioner as a solver applies it: FP32 vectors always use the FP32 blocks; FP64 vectors
+// use the FP64 blocks, or the FP32-stored blocks with FP64 arithmetic (fp32_blocks).
+struct BlockJacobiApply {
+    BlockJacobi const* bj = nullptr;
+    bool fp32_blocks = false;
+    void apply(double const* r, double* z, hipStream_t s, int ncols = 1, int64_t ld = 0) const {
+        if (fp32_blocks) bj->apply_fp32_blocks(r, z, s, ncols, ld);
+        else bj->apply(r, z, s, ncols, ld);
+    }
+    void apply(float const* r, float* z, hipStream_t s, int ncols = 1, int64_t ld = 0) const {
+        bj->apply(r, z, s, ncols, ld);
+    }
 };
 
 // Greedy colouring of the elements over their facet neighbours (colour per element).

@@ -81,9 +81,10 @@ __global__ __launch_bounds__(kThreads) void invert_blocks(double* __restrict__ D
     }
 }
 
-// z_e = inv_e r_e, one workgroup per element (column-major blocks: lanes read consecutive rows)
-template <class T>
-__global__ __launch_bounds__(128) void apply_blocks(T const* __restrict__ inv, T const* __restrict__ rs_, T* __restrict__ zs,
+// z_e = inv_e r_e, one workgroup per element (column-major blocks: lanes read consecutive rows);
+// blocks of precision TB, vectors and arithmetic of precision T
+template <class T, class TB>
+__global__ __launch_bounds__(128) void apply_blocks(TB const* __restrict__ inv, T const* __restrict__ rs_, T* __restrict__ zs,
                                                     int bs, int64_t ld) {
     __shared__ T rs[128];
     const int64_t e = blockIdx.x;
@@ -93,9 +94,9 @@ __global__ __launch_bounds__(128) void apply_blocks(T const* __restrict__ inv, T
     if (i < bs) rs[i] = r[e * bs + i];
     __syncthreads();
     if (i >= bs) return;
-    T const* B = inv + e * int64_t(bs) * bs;
+    TB const* B = inv + e * int64_t(bs) * bs;
     T acc = T(0);
-    for (int j = 0; j < bs; ++j) acc += B[i + j * bs] * rs[j];
+    for (int j = 0; j < bs; ++j) acc += T(B[i + j * bs]) * rs[j];
     z[e * bs + i] = acc;
 }
 
@@ -103,13 +104,13 @@ __global__ __launch_bounds__(128) void apply_blocks(T const* __restrict__ inv, T
 // it to every column (columns at stride ld), so a block is read from HBM once per application
 // instead of once per column.  Each lane computes four columns of one row.
 constexpr int kCols = 16; // columns staged per pass
-template <class T, int BSMAX>
-__global__ __launch_bounds__(256) void apply_blocks_cols(T const* __restrict__ inv, T const* __restrict__ rs,
+template <class T, class TB, int BSMAX>
+__global__ __launch_bounds__(256) void apply_blocks_cols(TB const* __restrict__ inv, T const* __restrict__ rs,
                                                          T* __restrict__ zs, int bs, int ncols, int64_t ld) {
-    __shared__ T B[BSMAX * BSMAX];
+    __shared__ TB B[BSMAX * BSMAX];
     __shared__ T R[kCols * BSMAX];
     const int64_t e = blockIdx.x;
-    T const* Bg = inv + e * int64_t(bs) * bs;
+    TB const* Bg = inv + e * int64_t(bs) * bs;
     const int nn = bs * bs;
     for (int t = threadIdx.x; t < nn; t += 256) B[t] = Bg[t];
     for (int c0 = 0; c0 < ncols; c0 += kCols) {
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(256) void apply_blocks_cols(T const* __restrict__ i
             T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
             T const* r0 = R + (4 * q) * bs;
             for (int jj = 0; jj < bs; ++jj) {
-                const T b = B[i + jj * bs];
+                const T b = T(B[i + jj * bs]);
                 a0 += b * r0[jj];
                 a1 += b * r0[jj + bs];
                 a2 += b * r0[jj + 2 * bs];
@@ -234,14 +235,16 @@ void BlockJacobi::build(Operator& op) {
 }
 
 namespace {
-template <class T> void launch_apply(T const* inv, T const* r, T* z, int64_t E, int bs, int ncols, int64_t ld,
-                                     hipStream_t s) {
+template <class T, class TB>
+void launch_apply(TB const* inv, T const* r, T* z, int64_t E, int bs, int ncols, int64_t ld, hipStream_t s) {
     if (ncols <= 1) {
-        hipLaunchKernelGGL((apply_blocks<T>), dim3(unsigned(E), 1u), dim3(128), 0, s, inv, r, z, bs, ld);
+        hipLaunchKernelGGL((apply_blocks<T, TB>), dim3(unsigned(E), 1u), dim3(128), 0, s, inv, r, z, bs, ld);
     } else if (bs <= 64) {
-        hipLaunchKernelGGL((apply_blocks_cols<T, 64>), dim3(unsigned(E)), dim3(256), 0, s, inv, r, z, bs, ncols, ld);
+        hipLaunchKernelGGL((apply_blocks_cols<T, TB, 64>), dim3(unsigned(E)), dim3(256), 0, s, inv, r, z, bs, ncols,
+                           ld);
     } else {
-        hipLaunchKernelGGL((apply_blocks_cols<T, 112>), dim3(unsigned(E)), dim3(256), 0, s, inv, r, z, bs, ncols, ld);
+        hipLaunchKernelGGL((apply_blocks_cols<T, TB, 112>), dim3(unsigned(E)), dim3(256), 0, s, inv, r, z, bs, ncols,
+                           ld);
     }
     BHIP(hipGetLastError());
 }
@@ -250,6 +253,11 @@ template <class T> void launch_apply(T const* inv, T const* r, T* z, int64_t E, 
 void BlockJacobi::apply(double const* r, double* z, hipStream_t s, int ncols, int64_t ld) const {
     if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
     launch_apply(inv64_, r, z, E_, bs_, ncols, ld, s);
+}
+
+void BlockJacobi::apply_fp32_blocks(double const* r, double* z, hipStream_t s, int ncols, int64_t ld) const {
+    if (!inv32_) throw std::logic_error("block Jacobi preconditioner not built");
+    launch_apply(inv32_, r, z, E_, bs_, ncols, ld, s);
 }
 
 void BlockJacobi::apply(float const* r, float* z, hipStream_t s, int ncols, int64_t ld) const {

@@ -9,7 +9,7 @@ namespace tdg {
 class Operator;
 
 enum SolveMode : int { Solve_FP64_CG = 0, Solve_FP32_CG_FP64_IR = 1, Solve_FP32_CG_FP64_RELIABLE = 2 };
-enum Precond : int { Precond_None = 0, Precond_BlockJacobi = 1 };
+enum Precond : int { Precond_None = 0, Precond_BlockJacobi = 1, Precond_BlockJacobiF32 = 2 };
 
 struct SolveOptions {
     double rtol = 1e-10;      // on the FP64 true residual ||b - A x|| / ||b||
@@ -19,7 +19,8 @@ struct SolveOptions {
     int max_outer = 50;
     int check_every = 10;     // host reads the residual every check_every iterations
     double reliable_delta = 0.1; // residual drop that triggers an FP64 replacement (mode 2)
-    int precond = Precond_None;  // element-block Jacobi (built on first use, FP64 and FP32 inverses)
+    int precond = Precond_None;  // element-block Jacobi (built on first use, FP64 and FP32 inverses);
+                                 // Precond_BlockJacobiF32: FP64 solves also apply the FP32-stored blocks
     int graphs = 1;              // replay the iterations between host checks as a captured HIP graph
                                  // (needs a non-default stream and an even check_every)
 };

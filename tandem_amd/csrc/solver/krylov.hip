@@ -180,7 +180,7 @@ double read_scalar(double const* d, hipStream_t s) {
 // scalars [6]: s[0], s[1] alternate rho = r.(M r), s[2] = p.Ap, s[4] = r.r when preconditioned.
 // M = null is plain CG (rho = r.r).  w: work vector for M r.
 template <class T, class TX>
-int cg(Operator& op, BlockJacobi const* M, T const* rhs, TX* z, T* r, T* p, T* q, T* w, double* s, double* partial,
+int cg(Operator& op, BlockJacobiApply const* M, T const* rhs, TX* z, T* r, T* p, T* q, T* w, double* s, double* partial,
        double* pqp, int64_t n, double target_rr, int max_it, int check_every, bool graphs, hipStream_t st,
        double* rr_out) {
     const dim3 g(kGrid), b(kBlock);
@@ -259,9 +259,15 @@ SolveResult solve(Operator& op, double const* b, double* x, SolveOptions const& 
     const dim3 g(kGrid), bl(kBlock);
     Buf<double> s, partial;
     s.alloc(6);
-    BlockJacobi const* M = nullptr;
-    if (o.precond == Precond_BlockJacobi) M = &op.block_jacobi();
-    else if (o.precond != Precond_None) throw std::invalid_argument("unknown preconditioner");
+    BlockJacobiApply bja;
+    BlockJacobiApply const* M = nullptr;
+    if (o.precond == Precond_BlockJacobi || o.precond == Precond_BlockJacobiF32) {
+        bja.bj = &op.block_jacobi();
+        bja.fp32_blocks = o.precond == Precond_BlockJacobiF32;
+        M = &bja;
+    } else if (o.precond != Precond_None) {
+        throw std::invalid_argument("unknown preconditioner");
+    }
     partial.alloc(kGrid);
     Buf<double> pqp; // partial sums of p.q from the apply's store
     pqp.alloc(std::size_t(std::max<int64_t>(kGrid, op.dot_partials_capacity())));

@@ -38,9 +38,15 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         throw std::invalid_argument("multi-column solves support FP64 CG and FP32 CG with FP64 replacement");
     op.upload_device();
     op.reserve_columns(segments ? 1 : ncols); // workspace of the batched apply, outside any capture
-    BlockJacobi const* M = nullptr;
-    if (o.precond == Precond_BlockJacobi) M = &op.block_jacobi();
-    else if (o.precond != Precond_None) throw std::invalid_argument("unknown preconditioner");
+    BlockJacobiApply bja;
+    BlockJacobiApply const* M = nullptr;
+    if (o.precond == Precond_BlockJacobi || o.precond == Precond_BlockJacobiF32) {
+        bja.bj = &op.block_jacobi();
+        bja.fp32_blocks = o.precond == Precond_BlockJacobiF32;
+        M = &bja;
+    } else if (o.precond != Precond_None) {
+        throw std::invalid_argument("unknown preconditioner");
+    }
     hipStream_t st = op.stream();
     const int64_t ld = L.ld;
     const dim3 g(kG, ncols), bl(kB);

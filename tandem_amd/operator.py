@@ -147,6 +147,9 @@ def _check(t, what, dtype="f64", numel=None, min_numel=None, cols=None, like=Non
 VARIANTS = ["auto", "points", "mfma-xcd", "mfma-element"]
 # launch shapes of the two-kernel apply per precision (TDG_SPLIT_SHAPES)
 SPLIT_SHAPES = 4
+# preconditioners, indexed by TDG_PRECOND_*: block Jacobi with FP64 blocks for FP64 solves, or with
+# the FP32-stored blocks for every solve
+_PRECONDS = {None: 0, "block-jacobi": 1, "block-jacobi-f32": 2}
 
 
 class DGOperator:
@@ -255,7 +258,7 @@ class DGOperator:
         bp = _check(b, "solve: b", numel=self.n)
         xp = _check(x, "solve: x", numel=self.n, like=b)
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ir": 1, "fp32-ru": 2}[mode], inner_rtol, max_outer,
-                              check_every, reliable_delta, {None: 0, "block-jacobi": 1}[precond], int(graphs))
+                              check_every, reliable_delta, _PRECONDS[precond], int(graphs))
         res = _lib.SolveResult()
         check(lib().tdg_solve(self._h, bp, xp, C.byref(opts), C.byref(res)))
         return {k: getattr(res, k) for k, _ in _lib.SolveResult._fields_}
@@ -271,7 +274,7 @@ class DGOperator:
             raise ValueError("solve_multi: B and X shapes differ")
         nc, ld = B.shape
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ru": 2}[mode], 1e-4, 1, check_every, reliable_delta,
-                              {None: 0, "block-jacobi": 1}[precond], int(graphs))
+                              _PRECONDS[precond], int(graphs))
         res = _lib.SolveResult()
         its = np.zeros(nc, dtype=np.int32)
         rel = np.zeros(nc)
@@ -313,7 +316,7 @@ class DGOperator:
         xp = _check(x, "solve_segments: x", numel=self.n, like=b)
         ns = len(off) - 1
         opts = _lib.SolveOpts(rtol, max_iter, {"fp64": 0, "fp32-ru": 2}[mode], 1e-4, 1, check_every, reliable_delta,
-                              {None: 0, "block-jacobi": 1}[precond], int(graphs))
+                              _PRECONDS[precond], int(graphs))
         res = _lib.SolveResult()
         its = np.zeros(ns, dtype=np.int32)
         rel = np.zeros(ns)

@@ -67,7 +67,7 @@ def test_preconditioned_solves(cuda):
     bd = torch.from_numpy(b).to(cuda)
     its = {}
     for mode in ("fp64", "fp32-ru", "fp32-ir"):
-        for pc in (None, "block-jacobi"):
+        for pc in (None, "block-jacobi", "block-jacobi-f32"):
             x = torch.zeros_like(bd)
             r = c.op.solve(bd, x, rtol=1e-10, mode=mode, precond=pc)
             print(mode, pc, r, flush=True)
@@ -75,4 +75,8 @@ def test_preconditioned_solves(cuda):
             assert rel_l2(x.cpu().numpy(), x_ref) <= 1e-7
             its[(mode, pc)] = r["iterations"]
     assert its[("fp64", "block-jacobi")] < its[("fp64", None)]
+    # the FP32-stored blocks are the same preconditioner to FP32 rounding: the FP64 solve's iteration
+    # count barely moves, and FP32 solves (which always apply them) are unchanged
+    assert abs(its[("fp64", "block-jacobi-f32")] - its[("fp64", "block-jacobi")]) <= 0.05 * its[("fp64", "block-jacobi")]
+    assert its[("fp32-ru", "block-jacobi-f32")] == its[("fp32-ru", "block-jacobi")]
     assert its[("fp32-ru", "block-jacobi")] < its[("fp32-ru", None)]

@@ -53,7 +53,8 @@ def test_poisson_unit_square_solve(cuda):
         assert rel_l2(x32, x64) <= 1e-8
 
 
-@pytest.mark.parametrize("mode,precond", [("fp64", None), ("fp32-ru", None), ("fp32-ru", "block-jacobi")])
+@pytest.mark.parametrize("mode,precond", [("fp64", None), ("fp32-ru", None), ("fp32-ru", "block-jacobi"),
+                                         ("fp64", "block-jacobi-f32")])
 def test_multi_column_solve_matches_dense(cuda, mode, precond):
     """ncols right-hand sides in lockstep (one batched apply per iteration): every column reaches the
     FP64 true relative residual 1e-10 and the dense solution; a zero column stays zero."""
