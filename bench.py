@@ -391,7 +391,8 @@ def run_greens(args):
             Xc = c.volume_points()
             c.set_material(lf(Xc), mf(Xc))
             coarse.append(c)
-        M = T.PMultigrid([op] + coarse, smooth=args.pmg_smooth)
+        M = T.PMultigrid([op] + coarse, smooth=args.pmg_smooth, coarse_max_iter=args.pmg_coarse_max_iter,
+                         galerkin_penalty=args.pmg_galerkin)
         lmax = M.setup()
         torch.cuda.synchronize()
         t_setup = time.perf_counter() - t0
@@ -408,6 +409,7 @@ def run_greens(args):
         torch.cuda.synchronize()
         out["pmg"] = {
             "levels": degs, "smooth": M.options["smooth"], "coarse_rtol": M.options["coarse_rtol"],
+            "coarse_max_iter": M.options["coarse_max_iter"], "galerkin_penalty": M.options["galerkin_penalty"],
             "eig_interval": [M.options["eig_lo"], M.options["eig_hi"]], "lambda_max": [float(v) for v in lmax],
             "setup_seconds": t_setup, "seconds_to_1e-10": t_p, "iterations": int(rp["iterations"]),
             "coarse_cg_iterations": int(rp["outer_iterations"]), "converged": int(rp["converged"]),
@@ -768,7 +770,11 @@ def main():
     ap.add_argument("--blocks", type=int, default=2, help="greens: number of 64-column blocks timed")
     ap.add_argument("--pmg-levels", default="6,3,1",
                     help="greens: degrees of the p-multigrid levels for the solve to 1e-10 ('none': skip)")
-    ap.add_argument("--pmg-smooth", type=int, default=3, help="greens: Chebyshev steps per smoothing")
+    ap.add_argument("--pmg-smooth", type=int, default=6, help="greens: Chebyshev steps per smoothing")
+    ap.add_argument("--pmg-coarse-max-iter", type=int, default=500,
+                    help="greens: coarse CG iteration cap per V-cycle")
+    ap.add_argument("--pmg-galerkin", type=int, default=1,
+                    help="greens: 1 = coarse levels take the fine facet penalty, 0 = their own")
     ap.add_argument("--pmg-max-iter", type=int, default=300, help="greens: flexible-CG iteration cap")
     ap.add_argument("--meshes", type=int, default=32, help="multimesh: number of independent meshes")
     ap.add_argument("--precond", choices=["block-jacobi", "none"], default="block-jacobi",

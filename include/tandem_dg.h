@@ -132,14 +132,15 @@ int tdg_autotune(tdg_op* op, int ncols, int precision_bytes, int* chosen, double
 /* Launch shapes of the two-kernel MFMA apply (TDG_VARIANT_AUTO): TDG_SPLIT_SHAPES compiled shapes
  * per precision (waves per workgroup and the occupancy the register allocation targets for the
  * facet and the element kernel, the element kernel's prefetch depth, the last basis tile of the
- * lifts on the vector ALU or on the matrix cores) times two workgroup orders (0: linear, 1: one
- * contiguous range of groups per XCD).  tdg_autotune_shapes times all of them on ncols columns in
- * FP64 (precision_bytes 8) or FP32 (4), keeps the fastest for that precision, reports it in *shape and
- * *xcd and the milliseconds per apply in ms[2 shape + xcd] (2 TDG_SPLIT_SHAPES entries, -1 if the
- * operator has no two-kernel apply).  tdg_set_shape selects one directly.  tdg_autotune runs the
+ * lifts on the vector ALU or on the matrix cores) times TDG_SPLIT_ORDERS workgroup orders (0: linear,
+ * 1: one contiguous range of groups per XCD, 2: the same ranges run by a persistent launch, as many
+ * workgroups as are resident, each looping over its blocks).  tdg_autotune_shapes times all of them
+ * on ncols columns in FP64 (precision_bytes 8) or FP32 (4), keeps the fastest for that precision,
+ * reports it in *shape and *xcd and the milliseconds per apply in ms[TDG_SPLIT_ORDERS shape + xcd]
+ * (TDG_SPLIT_ORDERS x TDG_SPLIT_SHAPES entries, -1 if the operator has no two-kernel apply).  tdg_set_shape selects one directly.  tdg_autotune runs the
  * shape search before it compares the variants.  No counterpart in the reference (its kernels are
  * generated for one CPU target). */
-enum { TDG_SPLIT_SHAPES = 4 };
+enum { TDG_SPLIT_SHAPES = 4, TDG_SPLIT_ORDERS = 3 };
 int tdg_autotune_shapes(tdg_op* op, int ncols, int precision_bytes, int* shape, int* xcd, double* ms);
 int tdg_set_shape(tdg_op* op, int precision_bytes, int shape, int xcd);
 

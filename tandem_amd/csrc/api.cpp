@@ -374,11 +374,12 @@ int tdg_autotune_shapes(tdg_op* op, int ncols, int precision_bytes, int* shape, 
         auto& o = ready(op);
         require(ncols >= 1 && (precision_bytes == 8 || precision_bytes == 4), "bad autotune arguments");
         static_assert(TDG_SPLIT_SHAPES == tdg::kSplitShapes, "shape count of the ABI and the library differ");
+        static_assert(TDG_SPLIT_ORDERS == tdg::kSplitOrders, "order count of the ABI and the library differ");
         auto t = o.autotune_shapes(ncols, precision_bytes);
         if (shape) *shape = o.split_shape(precision_bytes);
         if (xcd) *xcd = o.split_xcd(precision_bytes);
         if (ms)
-            for (int c = 0; c < 2 * TDG_SPLIT_SHAPES; ++c) ms[c] = t[std::size_t(c)];
+            for (int c = 0; c < TDG_SPLIT_ORDERS * TDG_SPLIT_SHAPES; ++c) ms[c] = t[std::size_t(c)];
     });
 }
 
@@ -386,7 +387,7 @@ int tdg_set_shape(tdg_op* op, int precision_bytes, int shape, int xcd) {
     return guarded([&] {
         require(op && op->impl, "null operator");
         require((precision_bytes == 8 || precision_bytes == 4) && shape >= 0 && shape < TDG_SPLIT_SHAPES &&
-                    (xcd == 0 || xcd == 1),
+                    xcd >= 0 && xcd < TDG_SPLIT_ORDERS,
                 "bad launch shape");
         op->impl->set_split_shape(precision_bytes, shape, xcd);
     });

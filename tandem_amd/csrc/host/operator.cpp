@@ -553,13 +553,14 @@ std::vector<double> Operator::autotune_shapes(int ncols, int pb) {
     if (ncols < 1 || (pb != 8 && pb != 4)) throw std::invalid_argument("bad autotune arguments");
     const int p = pb == 8 ? 0 : 1;
     std::vector<double> ms;
-    if (!(kind_ == Kind_Elasticity && mfma_supported(D_, N_))) return std::vector<double>(2 * kSplitShapes, -1.0);
+    if (!(kind_ == Kind_Elasticity && mfma_supported(D_, N_)))
+        return std::vector<double>(kSplitOrders * kSplitShapes, -1.0);
     const int saved_variant = variant_, saved_shape = split_shape_[p], saved_xcd = split_xcd_[p];
     try {
         variant_ = Variant_Auto;
-        time_configs(ncols, pb, 2 * kSplitShapes, [&](int c) {
-            split_shape_[p] = c / 2;
-            split_xcd_[p] = c % 2;
+        time_configs(ncols, pb, kSplitOrders * kSplitShapes, [&](int c) {
+            split_shape_[p] = c / kSplitOrders;
+            split_xcd_[p] = c % kSplitOrders;
             return true;
         }, ms);
     } catch (...) {
@@ -570,10 +571,10 @@ std::vector<double> Operator::autotune_shapes(int ncols, int pb) {
     }
     variant_ = saved_variant;
     int best = 0;
-    for (int c = 1; c < 2 * kSplitShapes; ++c)
+    for (int c = 1; c < kSplitOrders * kSplitShapes; ++c)
         if (ms[std::size_t(c)] >= 0.0 && ms[std::size_t(c)] < ms[std::size_t(best)]) best = c;
-    split_shape_[p] = best / 2;
-    split_xcd_[p] = best % 2;
+    split_shape_[p] = best / kSplitOrders;
+    split_xcd_[p] = best % kSplitOrders;
     return ms;
 }
 
@@ -602,7 +603,7 @@ std::vector<double> Operator::autotune(int ncols, int pb) {
 }
 
 void Operator::set_split_shape(int pb, int shape, int xcd) {
-    if ((pb != 8 && pb != 4) || shape < 0 || shape >= kSplitShapes || (xcd != 0 && xcd != 1))
+    if ((pb != 8 && pb != 4) || shape < 0 || shape >= kSplitShapes || xcd < 0 || xcd >= kSplitOrders)
         throw std::invalid_argument("bad launch shape");
     split_shape_[pb == 8 ? 0 : 1] = shape;
     split_xcd_[pb == 8 ? 0 : 1] = xcd;

@@ -286,7 +286,7 @@ private:
     // launch shape of the two-kernel apply, FP64 / FP32 (defaults: the fastest measured on the
     // 24^3-cube P4 workload; the autotuner re-chooses per operator)
     int split_shape_[2] = {0, 1};
-    int split_xcd_[2] = {1, 1};
+    int split_xcd_[2] = {2, 1};
     double time_configs(int ncols, int pb, int count, std::function<bool(int)> const& select, std::vector<double>& ms);
 };
 

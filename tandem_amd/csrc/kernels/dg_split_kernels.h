@@ -34,6 +34,7 @@
 #include "mfma_layout.h"
 #include "mfma_ops.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace tdg {
@@ -61,6 +62,15 @@ template <class T> struct Strides {
     static constexpr int pad(int n) { return sizeof(T) == 4 ? (n + 1) / 4 * 4 + 2 : n; }
 };
 
+// Lane index within the wave.  In a persistent kernel it is hidden from the optimiser, so that the
+// lane-dependent addresses of the block loop are computed where they are used instead of being
+// hoisted out of the loop and held in registers across it.
+template <bool PERSIST> __device__ __forceinline__ int lane_of(int tid) {
+    int v = tid & 63;
+    if constexpr (PERSIST) __asm__ volatile("" : "+v"(v));
+    return v & 63;
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -71,12 +81,15 @@ __device__ __forceinline__ void wave_sync() {
 // round-robin to the 8 XCDs (w to XCD w % 8); with several columns, w = 8 (ncols r + c) + x takes
 // column c of block 8 r + x, so all columns of a block run on one XCD one after another and the
 // block's data are read from HBM once into that XCD's L2.
-// One column with xcd_remap: each XCD works through one contiguous range of blocks (the grid is a
-// multiple of 8 workgroups).
-__device__ __forceinline__ void block_of(unsigned w, int ncols, int xcd_remap, unsigned& col, unsigned& lb) {
+// One column with xcd_remap: each XCD works through one contiguous range of blocks (nvb, the
+// number of blocks, is a multiple of 8).  Blocks are virtual: a persistent launch (xcd_remap 2) runs
+// fewer workgroups, each taking blocks w, w + gridDim.x, ... (gridDim.x a multiple of 8, so every
+// block stays on the XCD it would have had).
+__device__ __forceinline__ void block_of(unsigned w, unsigned nvb, int ncols, int xcd_remap, unsigned& col,
+                                         unsigned& lb) {
     const unsigned xcd = w % 8u, r = w / 8u;
     col = ncols > 1 ? r % unsigned(ncols) : 0u;
-    lb = ncols > 1 ? (r / unsigned(ncols)) * 8u + xcd : (xcd_remap ? xcd * (gridDim.x / 8u) + r : w);
+    lb = ncols > 1 ? (r / unsigned(ncols)) * 8u + xcd : (xcd_remap ? xcd * (nvb / 8u) + r : w);
 }
 
 // =====================================================================================  facets
@@ -95,10 +108,10 @@ __device__ __forceinline__ double mirror16(double v) {
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo));
 }
 
-template <int D, int N, class T, int W, int OCC, bool VTAIL_ON>
+template <int D, int N, class T, int W, int OCC, bool VTAIL_ON, bool PERSIST>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
 void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __restrict__ Ls, int64_t ld,
-                             int64_t ldL, int ncols) {
+                             int64_t ldL, int ncols, unsigned nvb) {
     using L = MfmaLayout<D, N>;
     using acc = typename Mma<T>::acc;
     constexpr int Nbf = L::Nbf, nq = L::nq, Q = D, BS = Nbf * Q, NF = D + 1, DD = D * D;
@@ -117,48 +130,9 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
     __shared__ T sW[nq];
     __shared__ T sX[W][16 * BSP];
 
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, j = lane & 15, g = lane >> 4;
-    unsigned col, lb;
-    block_of(blockIdx.x, ncols, a.xcd_remap, col, lb);
-    const int64_t grp = int64_t(lb) * W + wv;
-    const bool live = grp < a.num_fgroups; // uniform per wave
-    const int64_t gj = live ? grp : 0;
-    T const* __restrict__ x = xs + int64_t(col) * ld;
-    T* __restrict__ Lc = Ls + int64_t(col) * ldL;
-
-    // this lane's column: facet fl, side sd; everything it needs first
-    const int fl = column_facet(j), sd = j < 8 ? 0 : 1;
-    const int4 info = a.FInfo[gj * 8 + fl];
-    const T pen = a.FPen[gj * 8 + fl];
-    const int kind = live ? (info.z >> 8) & 3 : 0;
-    const bool two = kind == 1;
-    const bool ok = sd == 0 ? kind != 0 : two; // this column carries a facet side
-    const int el = ok ? (sd == 0 ? info.x : info.y) : 0;
-    const int lf = ok ? (sd == 0 ? (info.z & 15) : ((info.z >> 4) & 15)) : 0;
-    // record channels of this side: c_s and (lam_s, mu_s)
-    const int chc = sd == 0 ? (D - 1) * D : DD, chl = DD + D + 2 * sd;
-    T const* const recb = a.FR + gj * nq * NFR * 8 + fl;
-    auto load_rec = [&](int t, T (&rc)[DD + 2]) {
-        const int qc = min(4 * t + g, nq - 1);
-        T const* r = recb + qc * NFR * 8;
-#pragma unroll
-        for (int c = 0; c < (D - 1) * D; ++c) rc[c] = r[c * 8];
-#pragma unroll
-        for (int c = 0; c < D; ++c) rc[(D - 1) * D + c] = r[(chc + c) * 8];
-        rc[DD] = r[chl * 8];
-        rc[DD + 1] = r[(chl + 1) * 8];
-    };
-    T rc[DD + 2];
-    load_rec(0, rc);
-
-    // element vector of this column in LDS, relabelled to face 0: U'(k', r) = x(perm_f(k'), r);
-    // lane (j, g) moves entries 4 i + g
-    T* const X = sX[wv];
+    // canonical tables, once per workgroup
     {
-        T v[NI];
-        T const* xe = x + int64_t(el) * BS + g;
-#pragma unroll
-        for (int i = 0; i < NI; ++i) v[i] = (4 * i + 3 < BS || 4 * i + g < BS) ? xe[4 * i] : T(0);
+        const int tid = threadIdx.x;
 #pragma unroll
         for (int i0 = 0; i0 < 2 * NF * Nbf; i0 += 64 * W) {
             const int i = i0 + tid;
@@ -170,7 +144,52 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
             if (i < nT) sT[(i / Nbf) * NbfP + i % Nbf] = a.T0[i];
         }
         if (tid < nq) sW[tid] = a.w[tid];
-        __syncthreads(); // sP ready
+    }
+    __syncthreads();
+
+    auto block = [&](unsigned vb) {
+        const int tid = int(threadIdx.x), wv = tid >> 6, lane = lane_of<PERSIST>(tid), j = lane & 15, g = lane >> 4;
+        // this lane's column: facet fl, side sd
+        const int fl = column_facet(j), sd = j < 8 ? 0 : 1;
+        T* const X = sX[wv];
+        unsigned col, lb;
+        block_of(vb, nvb, ncols, a.xcd_remap, col, lb);
+        const int64_t grp = int64_t(lb) * W + wv;
+        const bool live = grp < a.num_fgroups; // uniform per wave
+        const int64_t gj = live ? grp : 0;
+        T const* __restrict__ x = xs + int64_t(col) * ld;
+        T* __restrict__ Lc = Ls + int64_t(col) * ldL;
+
+        const int4 info = a.FInfo[gj * 8 + fl];
+        const T pen = a.FPen[gj * 8 + fl];
+        const int kind = live ? (info.z >> 8) & 3 : 0;
+        const bool two = kind == 1;
+        const bool ok = sd == 0 ? kind != 0 : two; // this column carries a facet side
+        const int el = ok ? (sd == 0 ? info.x : info.y) : 0;
+        const int lf = ok ? (sd == 0 ? (info.z & 15) : ((info.z >> 4) & 15)) : 0;
+        // record channels of this side: c_s and (lam_s, mu_s)
+        const int chc = sd == 0 ? (D - 1) * D : DD, chl = DD + D + 2 * sd;
+        T const* const recb = a.FR + gj * nq * NFR * 8 + fl;
+        auto load_rec = [&](int t, T (&rc)[DD + 2]) {
+            const int qc = min(4 * t + g, nq - 1);
+            T const* r = recb + qc * NFR * 8;
+    #pragma unroll
+            for (int c = 0; c < (D - 1) * D; ++c) rc[c] = r[c * 8];
+    #pragma unroll
+            for (int c = 0; c < D; ++c) rc[(D - 1) * D + c] = r[(chc + c) * 8];
+            rc[DD] = r[chl * 8];
+            rc[DD + 1] = r[(chl + 1) * 8];
+    };
+    T rc[DD + 2];
+    load_rec(0, rc);
+
+    // element vector of this column in LDS, relabelled to face 0: U'(k', r) = x(perm_f(k'), r);
+    // lane (j, g) moves entries 4 i + g
+    {
+        T v[NI];
+        T const* xe = x + int64_t(el) * BS + g;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) v[i] = (4 * i + 3 < BS || 4 * i + g < BS) ? xe[4 * i] : T(0);
         int const* const ip = sP + NF * Nbf + lf * Nbf;
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
@@ -181,7 +200,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
             }
         }
     }
-    __syncthreads(); // tables and staged vectors ready
+    wave_sync(); // staged vectors ready (the region is this wave's)
 
     acc Y[MM][Q];
     T Yt[NT][Q];
@@ -417,6 +436,13 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
         for (int i = 0; i < NI; ++i)
             if (4 * i + 3 < BS || 4 * i + g < BS) dst[4 * i] = X[j * BSP + 4 * i + g];
     }
+    wave_sync(); // every read of the staged region is done before the next block restages it
+    };
+    if constexpr (PERSIST) {
+        for (unsigned vb = blockIdx.x; vb < nvb; vb += gridDim.x) block(vb);
+    } else {
+        block(blockIdx.x);
+    }
 }
 
 // =====================================================================================  elements
@@ -424,10 +450,10 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
 // block b and the facet slots during the last block, so only the first loads of a wave wait on
 // memory.  Otherwise (two or more waves per SIMD) each block loads its own point data up front and
 // the slots are read at the end; the other waves of the SIMD cover the waits.
-template <int D, int N, class T, int W, int OCC, bool DEEP, bool VTAIL_ON>
+template <int D, int N, class T, int W, int OCC, bool DEEP, bool VTAIL_ON, bool PERSIST>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
 void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const* __restrict__ Ls,
-                               T* __restrict__ ys, int64_t ld, int64_t ldL, int ncols) {
+                               T* __restrict__ ys, int64_t ld, int64_t ldL, int ncols, unsigned nvb) {
     using L = MfmaLayout<D, N>;
     using acc = typename Mma<T>::acc;
     constexpr int Nbf = L::Nbf, Nq = L::Nq, Q = D, BS = Nbf * Q, DD = D * D, NF = D + 1;
@@ -443,30 +469,40 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
     __shared__ T sV[D * Nq * NbfP];
     __shared__ T sU[W][16 * BSP];
 
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, j = lane & 15, g = lane >> 4;
-    unsigned col, lb;
-    block_of(blockIdx.x, ncols, a.xcd_remap, col, lb);
-    const int64_t grp = int64_t(lb) * W + wv;
-    const bool live = grp < a.num_groups; // uniform per wave
-    T const* __restrict__ x = xs + int64_t(col) * ld;
-    T const* __restrict__ Lc = Ls + int64_t(col) * ldL;
-    T* __restrict__ y = ys + int64_t(col) * ld;
-    const int64_t e0 = grp * 16;
-    const int nel = live ? int(min(int64_t(16), a.num_elements - e0)) : 0;
-    const bool valid = j < nel;
-    const int64_t gj = live ? grp : 0;
-    T const* const xw = x + (live ? e0 : 0) * BS; // this wave's 16 element vectors (contiguous)
-    const int nout = nel * BS;
+    // the volume table, once per workgroup
+#pragma unroll
+    for (int i0 = 0; i0 < nV; i0 += 64 * W) {
+        const int i = i0 + int(threadIdx.x);
+        if (i < nV) sV[(i / Nbf) * NbfP + i % Nbf] = a.V[i];
+    }
+    __syncthreads();
 
-    // point data of point q = 16 b + mfma_row(g, i) (clamped)
-    auto load_pd = [&](int b, T (&pd)[4][NCH]) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int q = min(16 * b + mfma_row<T>(g, i), Nq - 1);
-            T const* vd = a.VD + ((gj * Nq + q) * NCH) * 16 + j;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) pd[i][c] = vd[c * 16];
-        }
+    auto block = [&](unsigned vb) {
+        const int tid = int(threadIdx.x), wv = tid >> 6, lane = lane_of<PERSIST>(tid), j = lane & 15, g = lane >> 4;
+        T* const U = sU[wv];
+        unsigned col, lb;
+        block_of(vb, nvb, ncols, a.xcd_remap, col, lb);
+        const int64_t grp = int64_t(lb) * W + wv;
+        const bool live = grp < a.num_groups; // uniform per wave
+        T const* __restrict__ x = xs + int64_t(col) * ld;
+        T const* __restrict__ Lc = Ls + int64_t(col) * ldL;
+        T* __restrict__ y = ys + int64_t(col) * ld;
+        const int64_t e0 = grp * 16;
+        const int nel = live ? int(min(int64_t(16), a.num_elements - e0)) : 0;
+        const bool valid = j < nel;
+        const int64_t gj = live ? grp : 0;
+        T const* const xw = x + (live ? e0 : 0) * BS; // this wave's 16 element vectors (contiguous)
+        const int nout = nel * BS;
+
+        // point data of point q = 16 b + mfma_row(g, i) (clamped)
+        auto load_pd = [&](int b, T (&pd)[4][NCH]) {
+    #pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int q = min(16 * b + mfma_row<T>(g, i), Nq - 1);
+                T const* vd = a.VD + ((gj * Nq + q) * NCH) * 16 + j;
+    #pragma unroll
+                for (int c = 0; c < NCH; ++c) pd[i][c] = vd[c * 16];
+            }
     };
     // the element's facet slots for output entry i0 + lane (16 consecutive elements: contiguous)
     T const* const lsl = Lc + (live ? e0 : 0) * NF * BS;
@@ -485,12 +521,6 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
 
     T pd[4][NCH];
     if constexpr (DEEP) load_pd(0, pd);
-#pragma unroll
-    for (int i0 = 0; i0 < nV; i0 += 64 * W) {
-        const int i = i0 + tid;
-        if (i < nV) sV[(i / Nbf) * NbfP + i % Nbf] = a.V[i];
-    }
-    T* const U = sU[wv];
     if (live) {
         // every load first (clamped addresses), then the LDS stores: no load waits on another
         T v[NO];
@@ -502,7 +532,7 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
             if (i < 16 * BS) U[(i / BS) * BSP + i % BS] = i < nout ? v[o] : T(0);
         }
     }
-    __syncthreads();
+    wave_sync(); // staged vectors ready (the region is this wave's)
 
     T const* const Uj = U + j * BSP;
     acc Y[MM][Q];
@@ -682,7 +712,7 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
         }
     }
     if (a.dot) {
-        // one partial per workgroup: the wave sums, then the waves' sums in a fixed order
+        // one partial per block: the wave sums, then the waves' sums in a fixed order
         __shared__ double red[W];
         for (int o = 32; o > 0; o >>= 1) dot += __shfl_down(dot, o, 64);
         if (lane == 0) red[wv] = dot;
@@ -691,8 +721,16 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
             double t = 0.0;
 #pragma unroll
             for (int w = 0; w < W; ++w) t += red[w];
-            a.dot[blockIdx.x] = t;
+            a.dot[vb] = t;
         }
+        __syncthreads(); // red is rewritten by the next block
+    }
+    wave_sync(); // every read of the staged region is done before the next block restages it
+    };
+    if constexpr (PERSIST) {
+        for (unsigned vb = blockIdx.x; vb < nvb; vb += gridDim.x) block(vb);
+    } else {
+        block(blockIdx.x);
     }
 }
 
@@ -709,6 +747,18 @@ template <class S> int64_t element_workgroups(int64_t num_groups) {
     return (eb + 7) / 8 * 8;
 }
 
+// Workgroups a persistent launch of kernel k runs: as many as are resident on the device at once
+// (a multiple of 8), at most nvb.
+inline int64_t resident_workgroups(void const* k, int threads, int64_t nvb) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, 0) != hipSuccess || cus <= 0 || per_cu <= 0)
+        return nvb;
+    const int64_t cap = int64_t(cus) * per_cu / 8 * 8;
+    return cap >= 8 ? std::min(nvb, cap) : nvb;
+}
+
 template <int D, int N, class T, class S>
 hipError_t run_shape(SplitArgs<T> const& a, T const* x, T* y, T* Lbuf, int ncols, int64_t ld, int64_t ldL,
                      hipStream_t s) {
@@ -716,14 +766,29 @@ hipError_t run_shape(SplitArgs<T> const& a, T const* x, T* y, T* Lbuf, int ncols
     const int64_t ft = (fb + 7) / 8 * 8 * int64_t(ncols);
     const int64_t et = element_workgroups<S>(a.num_groups) * int64_t(ncols);
     if (ft > int64_t(0x7fffffff) || et > int64_t(0x7fffffff)) return hipErrorInvalidValue;
+    if (a.xcd_remap == 2) {
+        auto fk = elasticity_facet_kernel<D, N, T, S::fw, S::focc, S::vtail, true>;
+        auto ek = elasticity_element_kernel<D, N, T, S::ew, S::eocc, S::deep, S::vtail, true>;
+        if (ft > 0) {
+            const int64_t fg = resident_workgroups(reinterpret_cast<void const*>(fk), 64 * S::fw, ft);
+            hipLaunchKernelGGL(fk, dim3(unsigned(fg)), dim3(64 * S::fw), 0, s, a, x, Lbuf, ld, ldL, ncols,
+                               unsigned(ft));
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        const int64_t eg = resident_workgroups(reinterpret_cast<void const*>(ek), 64 * S::ew, et);
+        hipLaunchKernelGGL(ek, dim3(unsigned(eg)), dim3(64 * S::ew), 0, s, a, x, Lbuf, y, ld, ldL, ncols,
+                           unsigned(et));
+        return hipGetLastError();
+    }
     if (ft > 0) {
-        hipLaunchKernelGGL((elasticity_facet_kernel<D, N, T, S::fw, S::focc, S::vtail>), dim3(unsigned(ft)),
-                           dim3(64 * S::fw), 0, s, a, x, Lbuf, ld, ldL, ncols);
+        hipLaunchKernelGGL((elasticity_facet_kernel<D, N, T, S::fw, S::focc, S::vtail, false>), dim3(unsigned(ft)),
+                           dim3(64 * S::fw), 0, s, a, x, Lbuf, ld, ldL, ncols, unsigned(ft));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((elasticity_element_kernel<D, N, T, S::ew, S::eocc, S::deep, S::vtail>), dim3(unsigned(et)),
-                       dim3(64 * S::ew), 0, s, a, x, Lbuf, y, ld, ldL, ncols);
+    hipLaunchKernelGGL((elasticity_element_kernel<D, N, T, S::ew, S::eocc, S::deep, S::vtail, false>),
+                       dim3(unsigned(et)), dim3(64 * S::ew), 0, s, a, x, Lbuf, y, ld, ldL, ncols, unsigned(et));
     return hipGetLastError();
 }
 

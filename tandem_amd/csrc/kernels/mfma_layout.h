@@ -88,13 +88,16 @@ template <class T> struct SplitArgs {
     T const* FPen;
     int4 const* FInfo;
     int64_t num_elements, num_groups, num_facets, num_fgroups, num_two;
-    int xcd_remap; // one column: each XCD takes one contiguous range of groups
+    int xcd_remap; // workgroup order: 0 linear; 1 one contiguous range of groups per XCD (one column);
+                   // 2 as 1, persistent (the resident workgroups loop over the blocks)
     double* dot;   // one column, optional: per element workgroup the partial sum of x . y
 };
 
 // Number of launch shapes of the two-kernel apply (waves per workgroup, occupancy targets, element
 // prefetch depth, lift tail on the vector ALU or the matrix cores), per precision.
 constexpr int kSplitShapes = 4;
+// Workgroup orders (SplitArgs::xcd_remap) searched with each shape.
+constexpr int kSplitOrders = 3;
 
 // y = A x for ncols columns (stride ld) through the facet kernel (into the slot buffer L, column
 // stride ldL = E (D+1) Nbf D) and the element kernel, with launch shape `shape`.  With a.dot set

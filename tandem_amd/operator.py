@@ -145,8 +145,10 @@ def _check(t, what, dtype="f64", numel=None, min_numel=None, cols=None, like=Non
 
 # kernel variants, indexed by TDG_VARIANT_*
 VARIANTS = ["auto", "points", "mfma-xcd", "mfma-element"]
-# launch shapes of the two-kernel apply per precision (TDG_SPLIT_SHAPES)
+# launch shapes of the two-kernel apply per precision (TDG_SPLIT_SHAPES) and workgroup orders
+# (TDG_SPLIT_ORDERS: linear, per XCD, per XCD persistent)
 SPLIT_SHAPES = 4
+SPLIT_ORDERS = 3
 # preconditioners, indexed by TDG_PRECOND_*: block Jacobi with FP64 blocks for FP64 solves, or with
 # the FP32-stored blocks for every solve
 _PRECONDS = {None: 0, "block-jacobi": 1, "block-jacobi-f32": 2}
@@ -294,12 +296,15 @@ class DGOperator:
 
     def autotune_shapes(self, ncols=1, precision="fp64"):
         """Time the launch shapes of the two-kernel apply (shape s, workgroup order x: 0 linear, 1 per
-        XCD) on ncols columns, keep the fastest for that precision; returns ((s, x), {(s, x): ms})."""
+        XCD, 2 per XCD with a persistent launch) on ncols columns, keep the fastest for that
+        precision; returns ((s, x), {(s, x): ms})."""
         shape, xcd = C.c_int(), C.c_int()
-        ms = np.zeros(2 * SPLIT_SHAPES)
+        n = SPLIT_ORDERS * SPLIT_SHAPES
+        ms = np.zeros(n)
         check(lib().tdg_autotune_shapes(self._h, ncols, 8 if precision == "fp64" else 4, C.byref(shape),
                                         C.byref(xcd), ms))
-        return (shape.value, xcd.value), {(c // 2, c % 2): float(ms[c]) for c in range(2 * SPLIT_SHAPES) if ms[c] >= 0}
+        return (shape.value, xcd.value), {(c // SPLIT_ORDERS, c % SPLIT_ORDERS): float(ms[c]) for c in range(n)
+                                          if ms[c] >= 0}
 
     def set_shape(self, precision, shape, xcd=0):
         """Launch shape of the two-kernel apply for "fp64" or "fp32"."""

@@ -63,6 +63,20 @@ def test_full_size_apply_matches_oracle(cuda, box24):
     print("full-size rel L2 vs oracle: fp64 %.3e  fp32 %.3e" % (e64, e32), flush=True)
     assert e64 <= 1e-12
     assert e32 <= 1e-5
+    # every launch shape and workgroup order of the two-kernel apply; at this size the persistent
+    # order (2) runs fewer workgroups than blocks, so each workgroup loops over several blocks
+    saved = {p: op.autotune_shapes(1, p)[0] for p in ("fp64", "fp32")}
+    for prec, dt, bar in (("fp64", torch.float64, 1e-12), ("fp32", torch.float32, 1e-5)):
+        xs = xd.to(dt)
+        for shape in range(T.operator.SPLIT_SHAPES):
+            for order in range(T.operator.SPLIT_ORDERS):
+                op.set_shape(prec, shape, order)
+                yy = torch.full_like(xs, float("nan"))
+                op.apply(xs, yy)
+                err = _rel(yy.double().cpu().numpy(), ref)
+                print("  %s shape %d order %d: rel L2 %.3e" % (prec, shape, order, err), flush=True)
+                assert err <= bar
+        op.set_shape(prec, *saved[prec])
 
 
 def test_full_size_symmetry_linearity_and_batching(cuda, box24):
