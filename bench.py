@@ -773,7 +773,7 @@ def main():
     ap.add_argument("--pmg-smooth", type=int, default=6, help="greens: Chebyshev steps per smoothing")
     ap.add_argument("--pmg-coarse-max-iter", type=int, default=500,
                     help="greens: coarse CG iteration cap per V-cycle")
-    ap.add_argument("--pmg-galerkin", type=int, default=1,
+    ap.add_argument("--pmg-galerkin", type=int, default=0,
                     help="greens: 1 = coarse levels take the fine facet penalty, 0 = their own")
     ap.add_argument("--pmg-max-iter", type=int, default=300, help="greens: flexible-CG iteration cap")
     ap.add_argument("--meshes", type=int, default=32, help="multimesh: number of independent meshes")

@@ -173,270 +173,270 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
         auto load_rec = [&](int t, T (&rc)[DD + 2]) {
             const int qc = min(4 * t + g, nq - 1);
             T const* r = recb + qc * NFR * 8;
-    #pragma unroll
+#pragma unroll
             for (int c = 0; c < (D - 1) * D; ++c) rc[c] = r[c * 8];
-    #pragma unroll
+#pragma unroll
             for (int c = 0; c < D; ++c) rc[(D - 1) * D + c] = r[(chc + c) * 8];
             rc[DD] = r[chl * 8];
             rc[DD + 1] = r[(chl + 1) * 8];
-    };
-    T rc[DD + 2];
-    load_rec(0, rc);
+        };
+        T rc[DD + 2];
+        load_rec(0, rc);
 
-    // element vector of this column in LDS, relabelled to face 0: U'(k', r) = x(perm_f(k'), r);
-    // lane (j, g) moves entries 4 i + g
-    {
-        T v[NI];
-        T const* xe = x + int64_t(el) * BS + g;
+        // element vector of this column in LDS, relabelled to face 0: U'(k', r) = x(perm_f(k'), r);
+        // lane (j, g) moves entries 4 i + g
+        {
+            T v[NI];
+            T const* xe = x + int64_t(el) * BS + g;
 #pragma unroll
-        for (int i = 0; i < NI; ++i) v[i] = (4 * i + 3 < BS || 4 * i + g < BS) ? xe[4 * i] : T(0);
-        int const* const ip = sP + NF * Nbf + lf * Nbf;
+            for (int i = 0; i < NI; ++i) v[i] = (4 * i + 3 < BS || 4 * i + g < BS) ? xe[4 * i] : T(0);
+            int const* const ip = sP + NF * Nbf + lf * Nbf;
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            const int rem = 4 * i + g;
-            if (4 * i + 3 < BS || rem < BS) {
-                const int r = rem / Nbf, k = rem - r * Nbf;
-                X[j * BSP + ip[k] + r * Nbf] = ok ? v[i] : T(0);
+            for (int i = 0; i < NI; ++i) {
+                const int rem = 4 * i + g;
+                if (4 * i + 3 < BS || rem < BS) {
+                    const int r = rem / Nbf, k = rem - r * Nbf;
+                    X[j * BSP + ip[k] + r * Nbf] = ok ? v[i] : T(0);
+                }
             }
         }
-    }
-    wave_sync(); // staged vectors ready (the region is this wave's)
+        wave_sync(); // staged vectors ready (the region is this wave's)
 
-    acc Y[MM][Q];
-    T Yt[NT][Q];
-#pragma unroll
-    for (int m = 0; m < MM; ++m)
-#pragma unroll
-        for (int u = 0; u < Q; ++u) Y[m][u] = acc{0, 0, 0, 0};
-#pragma unroll
-    for (int kt = 0; kt < NT; ++kt)
-#pragma unroll
-        for (int u = 0; u < Q; ++u) Yt[kt][u] = T(0);
-
-    if (live) {
-        const int tq = mfma_group<T>(j);
-        const int tc = min(mfma_reg<T>(j), D);
-        T const* const U = X + j * BSP;
-
-#pragma unroll 1
-        for (int t = 0; t < FT; ++t) {
-            const int q = 4 * t + g;
-            const int qc = min(q, nq - 1);
-            const bool act = ok && q < nq;
-            T rn[DD + 2];
-            load_rec(min(t + 1, FT - 1), rn); // next tile's record, one tile ahead
-            // quadrature weight, zero on lanes without a facet point (every lane computes finite
-            // values: padding facets carry the identity Jacobian, empty columns a zero vector)
-            const T wq = act ? sW[qc] : T(0);
-            const T ws = sd == 0 ? -wq : wq; // the traction term lifts with -w on side 0, +w on side 1
-            acc C[Q];
-#pragma unroll
-            for (int r = 0; r < Q; ++r) C[r] = acc{0, 0, 0, 0};
-            const int ab = (tc * nq + min(4 * t + tq, nq - 1)) * NbfP;
-            {
-                T av = sT[ab + min(g, Nbf - 1)], b[Q];
-#pragma unroll
-                for (int r = 0; r < Q; ++r) b[r] = U[min(g, Nbf - 1) + r * Nbf];
-#pragma unroll
-                for (int s = 0; s < KS; ++s) {
-                    const int k = 4 * s + g;
-                    const bool kin = 4 * s + 3 < Nbf || k < Nbf;
-                    T avn = av, bn[Q];
-#pragma unroll
-                    for (int r = 0; r < Q; ++r) bn[r] = b[r];
-                    if (s + 1 < KS) {
-                        const int kn = min(k + 4, Nbf - 1);
-                        avn = sT[ab + kn];
-#pragma unroll
-                        for (int r = 0; r < Q; ++r) bn[r] = U[kn + r * Nbf];
-                    }
-#pragma unroll
-                    for (int r = 0; r < Q; ++r) C[r] = Mma<T>::mma(av, kin ? b[r] : T(0), C[r]);
-                    av = avn;
-#pragma unroll
-                    for (int r = 0; r < Q; ++r) b[r] = bn[r];
-                }
-            }
-            // ---- lane (j, g): side sd of facet fl at point q; register c = value (0) / d/dxi_e (1 + e)
-            // rows of G'_s = M_s^-1 with M_s = [t_1 .. t_(D-1), c_s], raw normal t_1 x t_2 (3D)
-            T R[D][D], nr[D], dt;
-            T const* cv = rc + (D - 1) * D;
-            if constexpr (D == 3) {
-                T const* ta = rc;
-                T const* tb = rc + 3;
-                nr[0] = ta[1] * tb[2] - ta[2] * tb[1];
-                nr[1] = ta[2] * tb[0] - ta[0] * tb[2];
-                nr[2] = ta[0] * tb[1] - ta[1] * tb[0];
-                R[0][0] = tb[1] * cv[2] - tb[2] * cv[1];
-                R[0][1] = tb[2] * cv[0] - tb[0] * cv[2];
-                R[0][2] = tb[0] * cv[1] - tb[1] * cv[0];
-                R[1][0] = cv[1] * ta[2] - cv[2] * ta[1];
-                R[1][1] = cv[2] * ta[0] - cv[0] * ta[2];
-                R[1][2] = cv[0] * ta[1] - cv[1] * ta[0];
-                dt = cv[0] * nr[0] + cv[1] * nr[1] + cv[2] * nr[2];
-                const T id = rcp_(dt);
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    R[0][i] *= id;
-                    R[1][i] *= id;
-                    R[2][i] = nr[i] * id;
-                }
-            } else {
-                T const* ta = rc;
-                nr[0] = ta[1];
-                nr[1] = -ta[0];
-                dt = ta[0] * cv[1] - ta[1] * cv[0];
-                const T id = rcp_(dt);
-                R[0][0] = cv[1] * id;
-                R[0][1] = -cv[0] * id;
-                R[1][0] = -ta[1] * id;
-                R[1][1] = ta[0] * id;
-            }
-            // n outward from side 0: c_0 points into element 0, c_1 into element 1
-            // (3D: nr . c_s = det_s; 2D: nr . c_s = -det_s)
-            const bool flip = (D == 3 ? dt > T(0) : dt < T(0)) != (sd == 1);
-            T n[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) n[d] = flip ? -nr[d] : nr[d];
-            T nn = T(0);
-#pragma unroll
-            for (int d = 0; d < D; ++d) nn += n[d] * n[d];
-            const T nl = sqrt_(nn);
-            const T lam = rc[DD], mu = rc[DD + 1];
-            // own side: u, physical gradient, sigma n
-            T us[Q], sn[Q];
-            {
-                T Jg[Q][D];
-#pragma unroll
-                for (int r = 0; r < Q; ++r) {
-                    us[r] = C[r][0];
-#pragma unroll
-                    for (int i = 0; i < D; ++i) {
-                        T v = T(0);
-#pragma unroll
-                        for (int e = 0; e < D; ++e) v += R[e][i] * C[r][1 + e];
-                        Jg[r][i] = v;
-                    }
-                }
-                T tr = T(0);
-#pragma unroll
-                for (int r = 0; r < D; ++r) tr += Jg[r][r];
-#pragma unroll
-                for (int i = 0; i < Q; ++i) {
-                    T v = T(0);
-#pragma unroll
-                    for (int jj = 0; jj < D; ++jj) v += (Jg[i][jj] + Jg[jj][i]) * n[jj];
-                    sn[i] = mu * v + lam * tr * n[i];
-                }
-            }
-            // the other side's u and sigma n from the mirrored lane
-            T tau[Q], dl[Q];
-#pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                const T uo = mirror16(us[i]), so = mirror16(sn[i]);
-                const T jump = sd == 0 ? us[i] - uo : uo - us[i]; // u_0 - u_1
-                if (two) {
-                    tau[i] = T(0.5) * (sn[i] + so) - pen * nl * jump;
-                    dl[i] = T(-0.5) * jump;
-                } else {
-                    tau[i] = sn[i] - pen * nl * us[i];
-                    dl[i] = -us[i];
-                }
-            }
-            T dn = T(0);
-#pragma unroll
-            for (int d = 0; d < D; ++d) dn += dl[d] * n[d];
-            T Wt[D + 1][Q];
-#pragma unroll
-            for (int u = 0; u < Q; ++u) Wt[0][u] = ws * tau[u];
-#pragma unroll
-            for (int e = 0; e < D; ++e)
-#pragma unroll
-                for (int u = 0; u < Q; ++u) {
-                    T v = T(0);
-#pragma unroll
-                    for (int jj = 0; jj < D; ++jj) {
-                        T Cuj = mu * (dl[u] * n[jj] + dl[jj] * n[u]);
-                        if (u == jj) Cuj += lam * dn;
-                        v += R[e][jj] * Cuj;
-                    }
-                    Wt[1 + e][u] = wq * v;
-                }
-            // ---- lift: Y(k') += sum_(q,c) T0(c, q ; k') W(c, q)
-            {
-                T av = sT[qc * NbfP + min(j, Nbf - 1)];
-#pragma unroll
-                for (int c = 0; c <= D; ++c) {
-#pragma unroll
-                    for (int m = 0; m < MM; ++m) {
-                        const int nx = c * MM + m + 1;
-                        const T avn = nx < (D + 1) * MM
-                                          ? sT[((nx / MM) * nq + qc) * NbfP + min(16 * (nx % MM) + j, Nbf - 1)]
-                                          : av;
-#pragma unroll
-                        for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, Wt[c][u], Y[m][u]);
-                        av = avn;
-                    }
-                    if constexpr (VTAIL) {
-#pragma unroll
-                        for (int kt = 0; kt < NT; ++kt) {
-                            const T tv = sT[(c * nq + qc) * NbfP + 16 * MM + kt];
-#pragma unroll
-                            for (int u = 0; u < Q; ++u) Yt[kt][u] += tv * Wt[c][u];
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < DD + 2; ++c) rc[c] = rn[c];
-        }
-    }
-
-    // ---- lifted rows back to element labels through LDS, then one contiguous slot per column
-    if constexpr (VTAIL) {
-#pragma unroll
-        for (int kt = 0; kt < NT; ++kt)
-#pragma unroll
-            for (int u = 0; u < Q; ++u) {
-                T v = Yt[kt][u];
-                v += __shfl_xor(v, 16, 64);
-                v += __shfl_xor(v, 32, 64);
-                Yt[kt][u] = v;
-            }
-    }
-    wave_sync(); // every read of the staged vectors is complete
-    {
-        int const* const pf = sP + lf * Nbf;
+        acc Y[MM][Q];
+        T Yt[NT][Q];
 #pragma unroll
         for (int m = 0; m < MM; ++m)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int k = 16 * m + mfma_row<T>(g, i);
-                if (k < Nbf) {
-                    const int p = pf[k];
+            for (int u = 0; u < Q; ++u) Y[m][u] = acc{0, 0, 0, 0};
 #pragma unroll
-                    for (int u = 0; u < Q; ++u) X[j * BSP + p + u * Nbf] = Y[m][u][i];
+        for (int kt = 0; kt < NT; ++kt)
+#pragma unroll
+            for (int u = 0; u < Q; ++u) Yt[kt][u] = T(0);
+
+        if (live) {
+            const int tq = mfma_group<T>(j);
+            const int tc = min(mfma_reg<T>(j), D);
+            T const* const U = X + j * BSP;
+
+#pragma unroll 1
+            for (int t = 0; t < FT; ++t) {
+                const int q = 4 * t + g;
+                const int qc = min(q, nq - 1);
+                const bool act = ok && q < nq;
+                T rn[DD + 2];
+                load_rec(min(t + 1, FT - 1), rn); // next tile's record, one tile ahead
+                // quadrature weight, zero on lanes without a facet point (every lane computes finite
+                // values: padding facets carry the identity Jacobian, empty columns a zero vector)
+                const T wq = act ? sW[qc] : T(0);
+                const T ws = sd == 0 ? -wq : wq; // the traction term lifts with -w on side 0, +w on side 1
+                acc C[Q];
+#pragma unroll
+                for (int r = 0; r < Q; ++r) C[r] = acc{0, 0, 0, 0};
+                const int ab = (tc * nq + min(4 * t + tq, nq - 1)) * NbfP;
+                {
+                    T av = sT[ab + min(g, Nbf - 1)], b[Q];
+#pragma unroll
+                    for (int r = 0; r < Q; ++r) b[r] = U[min(g, Nbf - 1) + r * Nbf];
+#pragma unroll
+                    for (int s = 0; s < KS; ++s) {
+                        const int k = 4 * s + g;
+                        const bool kin = 4 * s + 3 < Nbf || k < Nbf;
+                        T avn = av, bn[Q];
+#pragma unroll
+                        for (int r = 0; r < Q; ++r) bn[r] = b[r];
+                        if (s + 1 < KS) {
+                            const int kn = min(k + 4, Nbf - 1);
+                            avn = sT[ab + kn];
+#pragma unroll
+                            for (int r = 0; r < Q; ++r) bn[r] = U[kn + r * Nbf];
+                        }
+#pragma unroll
+                        for (int r = 0; r < Q; ++r) C[r] = Mma<T>::mma(av, kin ? b[r] : T(0), C[r]);
+                        av = avn;
+#pragma unroll
+                        for (int r = 0; r < Q; ++r) b[r] = bn[r];
+                    }
                 }
+                // ---- lane (j, g): side sd of facet fl at point q; register c = value (0) / d/dxi_e (1 + e)
+                // rows of G'_s = M_s^-1 with M_s = [t_1 .. t_(D-1), c_s], raw normal t_1 x t_2 (3D)
+                T R[D][D], nr[D], dt;
+                T const* cv = rc + (D - 1) * D;
+                if constexpr (D == 3) {
+                    T const* ta = rc;
+                    T const* tb = rc + 3;
+                    nr[0] = ta[1] * tb[2] - ta[2] * tb[1];
+                    nr[1] = ta[2] * tb[0] - ta[0] * tb[2];
+                    nr[2] = ta[0] * tb[1] - ta[1] * tb[0];
+                    R[0][0] = tb[1] * cv[2] - tb[2] * cv[1];
+                    R[0][1] = tb[2] * cv[0] - tb[0] * cv[2];
+                    R[0][2] = tb[0] * cv[1] - tb[1] * cv[0];
+                    R[1][0] = cv[1] * ta[2] - cv[2] * ta[1];
+                    R[1][1] = cv[2] * ta[0] - cv[0] * ta[2];
+                    R[1][2] = cv[0] * ta[1] - cv[1] * ta[0];
+                    dt = cv[0] * nr[0] + cv[1] * nr[1] + cv[2] * nr[2];
+                    const T id = rcp_(dt);
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        R[0][i] *= id;
+                        R[1][i] *= id;
+                        R[2][i] = nr[i] * id;
+                    }
+                } else {
+                    T const* ta = rc;
+                    nr[0] = ta[1];
+                    nr[1] = -ta[0];
+                    dt = ta[0] * cv[1] - ta[1] * cv[0];
+                    const T id = rcp_(dt);
+                    R[0][0] = cv[1] * id;
+                    R[0][1] = -cv[0] * id;
+                    R[1][0] = -ta[1] * id;
+                    R[1][1] = ta[0] * id;
+                }
+                // n outward from side 0: c_0 points into element 0, c_1 into element 1
+                // (3D: nr . c_s = det_s; 2D: nr . c_s = -det_s)
+                const bool flip = (D == 3 ? dt > T(0) : dt < T(0)) != (sd == 1);
+                T n[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) n[d] = flip ? -nr[d] : nr[d];
+                T nn = T(0);
+#pragma unroll
+                for (int d = 0; d < D; ++d) nn += n[d] * n[d];
+                const T nl = sqrt_(nn);
+                const T lam = rc[DD], mu = rc[DD + 1];
+                // own side: u, physical gradient, sigma n
+                T us[Q], sn[Q];
+                {
+                    T Jg[Q][D];
+#pragma unroll
+                    for (int r = 0; r < Q; ++r) {
+                        us[r] = C[r][0];
+#pragma unroll
+                        for (int i = 0; i < D; ++i) {
+                            T v = T(0);
+#pragma unroll
+                            for (int e = 0; e < D; ++e) v += R[e][i] * C[r][1 + e];
+                            Jg[r][i] = v;
+                        }
+                    }
+                    T tr = T(0);
+#pragma unroll
+                    for (int r = 0; r < D; ++r) tr += Jg[r][r];
+#pragma unroll
+                    for (int i = 0; i < Q; ++i) {
+                        T v = T(0);
+#pragma unroll
+                        for (int jj = 0; jj < D; ++jj) v += (Jg[i][jj] + Jg[jj][i]) * n[jj];
+                        sn[i] = mu * v + lam * tr * n[i];
+                    }
+                }
+                // the other side's u and sigma n from the mirrored lane
+                T tau[Q], dl[Q];
+#pragma unroll
+                for (int i = 0; i < Q; ++i) {
+                    const T uo = mirror16(us[i]), so = mirror16(sn[i]);
+                    const T jump = sd == 0 ? us[i] - uo : uo - us[i]; // u_0 - u_1
+                    if (two) {
+                        tau[i] = T(0.5) * (sn[i] + so) - pen * nl * jump;
+                        dl[i] = T(-0.5) * jump;
+                    } else {
+                        tau[i] = sn[i] - pen * nl * us[i];
+                        dl[i] = -us[i];
+                    }
+                }
+                T dn = T(0);
+#pragma unroll
+                for (int d = 0; d < D; ++d) dn += dl[d] * n[d];
+                T Wt[D + 1][Q];
+#pragma unroll
+                for (int u = 0; u < Q; ++u) Wt[0][u] = ws * tau[u];
+#pragma unroll
+                for (int e = 0; e < D; ++e)
+#pragma unroll
+                    for (int u = 0; u < Q; ++u) {
+                        T v = T(0);
+#pragma unroll
+                        for (int jj = 0; jj < D; ++jj) {
+                            T Cuj = mu * (dl[u] * n[jj] + dl[jj] * n[u]);
+                            if (u == jj) Cuj += lam * dn;
+                            v += R[e][jj] * Cuj;
+                        }
+                        Wt[1 + e][u] = wq * v;
+                    }
+                // ---- lift: Y(k') += sum_(q,c) T0(c, q ; k') W(c, q)
+                {
+                    T av = sT[qc * NbfP + min(j, Nbf - 1)];
+#pragma unroll
+                    for (int c = 0; c <= D; ++c) {
+#pragma unroll
+                        for (int m = 0; m < MM; ++m) {
+                            const int nx = c * MM + m + 1;
+                            const T avn = nx < (D + 1) * MM
+                                              ? sT[((nx / MM) * nq + qc) * NbfP + min(16 * (nx % MM) + j, Nbf - 1)]
+                                              : av;
+#pragma unroll
+                            for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, Wt[c][u], Y[m][u]);
+                            av = avn;
+                        }
+                        if constexpr (VTAIL) {
+#pragma unroll
+                            for (int kt = 0; kt < NT; ++kt) {
+                                const T tv = sT[(c * nq + qc) * NbfP + 16 * MM + kt];
+#pragma unroll
+                                for (int u = 0; u < Q; ++u) Yt[kt][u] += tv * Wt[c][u];
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < DD + 2; ++c) rc[c] = rn[c];
             }
+        }
+
+        // ---- lifted rows back to element labels through LDS, then one contiguous slot per column
         if constexpr (VTAIL) {
-            if (g == 0) {
 #pragma unroll
-                for (int kt = 0; kt < NT; ++kt) {
-                    const int p = pf[16 * MM + kt];
+            for (int kt = 0; kt < NT; ++kt)
 #pragma unroll
-                    for (int u = 0; u < Q; ++u) X[j * BSP + p + u * Nbf] = Yt[kt][u];
+                for (int u = 0; u < Q; ++u) {
+                    T v = Yt[kt][u];
+                    v += __shfl_xor(v, 16, 64);
+                    v += __shfl_xor(v, 32, 64);
+                    Yt[kt][u] = v;
+                }
+        }
+        wave_sync(); // every read of the staged vectors is complete
+        {
+            int const* const pf = sP + lf * Nbf;
+#pragma unroll
+            for (int m = 0; m < MM; ++m)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int k = 16 * m + mfma_row<T>(g, i);
+                    if (k < Nbf) {
+                        const int p = pf[k];
+#pragma unroll
+                        for (int u = 0; u < Q; ++u) X[j * BSP + p + u * Nbf] = Y[m][u][i];
+                    }
+                }
+            if constexpr (VTAIL) {
+                if (g == 0) {
+#pragma unroll
+                    for (int kt = 0; kt < NT; ++kt) {
+                        const int p = pf[16 * MM + kt];
+#pragma unroll
+                        for (int u = 0; u < Q; ++u) X[j * BSP + p + u * Nbf] = Yt[kt][u];
+                    }
                 }
             }
         }
-    }
-    wave_sync();
-    if (ok) {
-        T* const dst = Lc + (int64_t(el) * NF + lf) * BS + g;
+        wave_sync();
+        if (ok) {
+            T* const dst = Lc + (int64_t(el) * NF + lf) * BS + g;
 #pragma unroll
-        for (int i = 0; i < NI; ++i)
-            if (4 * i + 3 < BS || 4 * i + g < BS) dst[4 * i] = X[j * BSP + 4 * i + g];
-    }
-    wave_sync(); // every read of the staged region is done before the next block restages it
+            for (int i = 0; i < NI; ++i)
+                if (4 * i + 3 < BS || 4 * i + g < BS) dst[4 * i] = X[j * BSP + 4 * i + g];
+        }
+        wave_sync(); // every read of the staged region is done before the next block restages it
     };
     if constexpr (PERSIST) {
         for (unsigned vb = blockIdx.x; vb < nvb; vb += gridDim.x) block(vb);
@@ -496,236 +496,236 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
 
         // point data of point q = 16 b + mfma_row(g, i) (clamped)
         auto load_pd = [&](int b, T (&pd)[4][NCH]) {
-    #pragma unroll
+#pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int q = min(16 * b + mfma_row<T>(g, i), Nq - 1);
                 T const* vd = a.VD + ((gj * Nq + q) * NCH) * 16 + j;
-    #pragma unroll
+#pragma unroll
                 for (int c = 0; c < NCH; ++c) pd[i][c] = vd[c * 16];
             }
-    };
-    // the element's facet slots for output entry i0 + lane (16 consecutive elements: contiguous)
-    T const* const lsl = Lc + (live ? e0 : 0) * NF * BS;
-    auto load_slots = [&](T (&ls)[NO]) {
+        };
+        // the element's facet slots for output entry i0 + lane (16 consecutive elements: contiguous)
+        T const* const lsl = Lc + (live ? e0 : 0) * NF * BS;
+        auto load_slots = [&](T (&ls)[NO]) {
 #pragma unroll
-        for (int o = 0; o < NO; ++o) {
-            const int i = 64 * o + lane;
-            const int ic = min(i, max(nout - 1, 0));
-            const int jj = ic / BS, rem = ic - jj * BS;
-            T v = T(0);
+            for (int o = 0; o < NO; ++o) {
+                const int i = 64 * o + lane;
+                const int ic = min(i, max(nout - 1, 0));
+                const int jj = ic / BS, rem = ic - jj * BS;
+                T v = T(0);
 #pragma unroll
-            for (int f = 0; f < NF; ++f) v += lsl[(jj * NF + f) * BS + rem];
-            ls[o] = v;
-        }
-    };
-
-    T pd[4][NCH];
-    if constexpr (DEEP) load_pd(0, pd);
-    if (live) {
-        // every load first (clamped addresses), then the LDS stores: no load waits on another
-        T v[NO];
-#pragma unroll
-        for (int o = 0; o < NO; ++o) v[o] = xw[min(64 * o + lane, nout - 1)];
-#pragma unroll
-        for (int o = 0; o < NO; ++o) {
-            const int i = 64 * o + lane;
-            if (i < 16 * BS) U[(i / BS) * BSP + i % BS] = i < nout ? v[o] : T(0);
-        }
-    }
-    wave_sync(); // staged vectors ready (the region is this wave's)
-
-    T const* const Uj = U + j * BSP;
-    acc Y[MM][Q];
-    T Yt[NT][Q];
-#pragma unroll
-    for (int m = 0; m < MM; ++m)
-#pragma unroll
-        for (int u = 0; u < Q; ++u) Y[m][u] = acc{0, 0, 0, 0};
-#pragma unroll
-    for (int kt = 0; kt < NT; ++kt)
-#pragma unroll
-        for (int u = 0; u < Q; ++u) Yt[kt][u] = T(0);
-    T ls[NO];
-
-    auto volume_block = [&](int b, T const (&P)[4][NCH]) {
-        acc C[D][Q];
-#pragma unroll
-        for (int e = 0; e < D; ++e)
-#pragma unroll
-            for (int r = 0; r < Q; ++r) C[e][r] = acc{0, 0, 0, 0};
-        const int qa = min(16 * b + j, Nq - 1) * NbfP;
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int k = 4 * s + g;
-            const int kc = min(k, Nbf - 1);
-            T bu[Q];
-#pragma unroll
-            for (int r = 0; r < Q; ++r) {
-                const T v = Uj[kc + r * Nbf];
-                bu[r] = (4 * s + 3 < Nbf || k < Nbf) ? v : T(0);
+                for (int f = 0; f < NF; ++f) v += lsl[(jj * NF + f) * BS + rem];
+                ls[o] = v;
             }
+        };
+
+        T pd[4][NCH];
+        if constexpr (DEEP) load_pd(0, pd);
+        if (live) {
+            // every load first (clamped addresses), then the LDS stores: no load waits on another
+            T v[NO];
 #pragma unroll
-            for (int e = 0; e < D; ++e) {
-                const T av = sV[e * Nq * NbfP + qa + kc];
+            for (int o = 0; o < NO; ++o) v[o] = xw[min(64 * o + lane, nout - 1)];
 #pragma unroll
-                for (int r = 0; r < Q; ++r) C[e][r] = Mma<T>::mma(av, bu[r], C[e][r]);
+            for (int o = 0; o < NO; ++o) {
+                const int i = 64 * o + lane;
+                if (i < 16 * BS) U[(i / BS) * BSP + i % BS] = i < nout ? v[o] : T(0);
             }
         }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (16 * b + mfma_row<T>(0, i) >= Nq) continue; // all four lane groups past the last point
-            const bool act = valid && 16 * b + mfma_row<T>(g, i) < Nq;
-            T la[D][MM];
-            {
-                const int qa2 = min(16 * b + mfma_row<T>(g, i), Nq - 1) * NbfP;
-#pragma unroll
-                for (int e = 0; e < D; ++e)
-#pragma unroll
-                    for (int m = 0; m < MM; ++m) la[e][m] = sV[e * Nq * NbfP + qa2 + min(16 * m + j, Nbf - 1)];
-            }
-            T const* G = P[i];
-            const T lw = P[i][DD], mw = P[i][DD + 1];
-            T Ju[D][Q];
-#pragma unroll
-            for (int s2 = 0; s2 < D; ++s2)
-#pragma unroll
-                for (int r = 0; r < Q; ++r) {
-                    T v = T(0);
-#pragma unroll
-                    for (int e = 0; e < D; ++e) v += G[e + s2 * D] * C[e][r][i];
-                    Ju[s2][r] = v;
-                }
-            T tr = T(0);
-#pragma unroll
-            for (int r = 0; r < D; ++r) tr += Ju[r][r];
-#pragma unroll
-            for (int e = 0; e < D; ++e)
-#pragma unroll
-                for (int u = 0; u < Q; ++u) {
-                    T v = T(0);
-#pragma unroll
-                    for (int jj = 0; jj < D; ++jj) {
-                        T Pv = mw * (Ju[u][jj] + Ju[jj][u]);
-                        if (u == jj) Pv += lw * tr;
-                        v += G[e + jj * D] * Pv;
-                    }
-                    C[e][u][i] = act ? v : T(0);
-                }
-            const int qr = min(16 * b + mfma_row<T>(g, i), Nq - 1) * NbfP;
-#pragma unroll
-            for (int e = 0; e < D; ++e) {
-#pragma unroll
-                for (int m = 0; m < MM; ++m) {
-                    const T av = la[e][m];
-#pragma unroll
-                    for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, C[e][u][i], Y[m][u]);
-                }
-                if constexpr (VTAIL) {
-#pragma unroll
-                    for (int kt = 0; kt < NT; ++kt) {
-                        const T tv = sV[e * Nq * NbfP + qr + 16 * MM + kt];
-#pragma unroll
-                        for (int u = 0; u < Q; ++u) Yt[kt][u] += tv * C[e][u][i];
-                    }
-                }
-            }
-        }
-    };
+        wave_sync(); // staged vectors ready (the region is this wave's)
 
-    if (live) {
-        if constexpr (DEEP) {
-#pragma unroll
-            for (int b = 0; b < QB; ++b) {
-                T pn[4][NCH];
-                if (b + 1 < QB) load_pd(b + 1, pn);
-                else load_slots(ls);
-                volume_block(b, pd);
-                if (b + 1 < QB) {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-#pragma unroll
-                        for (int c = 0; c < NCH; ++c) pd[i][c] = pn[i][c];
-                }
-            }
-        } else {
-#pragma unroll 1
-            for (int b = 0; b < QB; ++b) {
-                load_pd(b, pd);
-                volume_block(b, pd);
-            }
-        }
-    }
-
-    if constexpr (VTAIL) {
-#pragma unroll
-        for (int kt = 0; kt < NT; ++kt)
-#pragma unroll
-            for (int u = 0; u < Q; ++u) {
-                T v = Yt[kt][u];
-                v += __shfl_xor(v, 16, 64);
-                v += __shfl_xor(v, 32, 64);
-                Yt[kt][u] = v;
-            }
-    }
-    wave_sync(); // every read of this wave's U is complete (sU is per wave)
-    if (live) {
+        T const* const Uj = U + j * BSP;
+        acc Y[MM][Q];
+        T Yt[NT][Q];
 #pragma unroll
         for (int m = 0; m < MM; ++m)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int k = 16 * m + mfma_row<T>(g, i);
-                if (k < Nbf) {
+            for (int u = 0; u < Q; ++u) Y[m][u] = acc{0, 0, 0, 0};
 #pragma unroll
-                    for (int u = 0; u < Q; ++u) U[j * BSP + k + u * Nbf] = Y[m][u][i];
+        for (int kt = 0; kt < NT; ++kt)
+#pragma unroll
+            for (int u = 0; u < Q; ++u) Yt[kt][u] = T(0);
+        T ls[NO];
+
+        auto volume_block = [&](int b, T const (&P)[4][NCH]) {
+            acc C[D][Q];
+#pragma unroll
+            for (int e = 0; e < D; ++e)
+#pragma unroll
+                for (int r = 0; r < Q; ++r) C[e][r] = acc{0, 0, 0, 0};
+            const int qa = min(16 * b + j, Nq - 1) * NbfP;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int k = 4 * s + g;
+                const int kc = min(k, Nbf - 1);
+                T bu[Q];
+#pragma unroll
+                for (int r = 0; r < Q; ++r) {
+                    const T v = Uj[kc + r * Nbf];
+                    bu[r] = (4 * s + 3 < Nbf || k < Nbf) ? v : T(0);
+                }
+#pragma unroll
+                for (int e = 0; e < D; ++e) {
+                    const T av = sV[e * Nq * NbfP + qa + kc];
+#pragma unroll
+                    for (int r = 0; r < Q; ++r) C[e][r] = Mma<T>::mma(av, bu[r], C[e][r]);
                 }
             }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (16 * b + mfma_row<T>(0, i) >= Nq) continue; // all four lane groups past the last point
+                const bool act = valid && 16 * b + mfma_row<T>(g, i) < Nq;
+                T la[D][MM];
+                {
+                    const int qa2 = min(16 * b + mfma_row<T>(g, i), Nq - 1) * NbfP;
+#pragma unroll
+                    for (int e = 0; e < D; ++e)
+#pragma unroll
+                        for (int m = 0; m < MM; ++m) la[e][m] = sV[e * Nq * NbfP + qa2 + min(16 * m + j, Nbf - 1)];
+                }
+                T const* G = P[i];
+                const T lw = P[i][DD], mw = P[i][DD + 1];
+                T Ju[D][Q];
+#pragma unroll
+                for (int s2 = 0; s2 < D; ++s2)
+#pragma unroll
+                    for (int r = 0; r < Q; ++r) {
+                        T v = T(0);
+#pragma unroll
+                        for (int e = 0; e < D; ++e) v += G[e + s2 * D] * C[e][r][i];
+                        Ju[s2][r] = v;
+                    }
+                T tr = T(0);
+#pragma unroll
+                for (int r = 0; r < D; ++r) tr += Ju[r][r];
+#pragma unroll
+                for (int e = 0; e < D; ++e)
+#pragma unroll
+                    for (int u = 0; u < Q; ++u) {
+                        T v = T(0);
+#pragma unroll
+                        for (int jj = 0; jj < D; ++jj) {
+                            T Pv = mw * (Ju[u][jj] + Ju[jj][u]);
+                            if (u == jj) Pv += lw * tr;
+                            v += G[e + jj * D] * Pv;
+                        }
+                        C[e][u][i] = act ? v : T(0);
+                    }
+                const int qr = min(16 * b + mfma_row<T>(g, i), Nq - 1) * NbfP;
+#pragma unroll
+                for (int e = 0; e < D; ++e) {
+#pragma unroll
+                    for (int m = 0; m < MM; ++m) {
+                        const T av = la[e][m];
+#pragma unroll
+                        for (int u = 0; u < Q; ++u) Y[m][u] = Mma<T>::mma(av, C[e][u][i], Y[m][u]);
+                    }
+                    if constexpr (VTAIL) {
+#pragma unroll
+                        for (int kt = 0; kt < NT; ++kt) {
+                            const T tv = sV[e * Nq * NbfP + qr + 16 * MM + kt];
+#pragma unroll
+                            for (int u = 0; u < Q; ++u) Yt[kt][u] += tv * C[e][u][i];
+                        }
+                    }
+                }
+            }
+        };
+
+        if (live) {
+            if constexpr (DEEP) {
+#pragma unroll
+                for (int b = 0; b < QB; ++b) {
+                    T pn[4][NCH];
+                    if (b + 1 < QB) load_pd(b + 1, pn);
+                    else load_slots(ls);
+                    volume_block(b, pd);
+                    if (b + 1 < QB) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int c = 0; c < NCH; ++c) pd[i][c] = pn[i][c];
+                    }
+                }
+            } else {
+#pragma unroll 1
+                for (int b = 0; b < QB; ++b) {
+                    load_pd(b, pd);
+                    volume_block(b, pd);
+                }
+            }
+        }
+
         if constexpr (VTAIL) {
-            if (g == 0) {
 #pragma unroll
-                for (int kt = 0; kt < NT; ++kt)
+            for (int kt = 0; kt < NT; ++kt)
 #pragma unroll
-                    for (int u = 0; u < Q; ++u) U[j * BSP + 16 * MM + kt + u * Nbf] = Yt[kt][u];
+                for (int u = 0; u < Q; ++u) {
+                    T v = Yt[kt][u];
+                    v += __shfl_xor(v, 16, 64);
+                    v += __shfl_xor(v, 32, 64);
+                    Yt[kt][u] = v;
+                }
+        }
+        wave_sync(); // every read of this wave's U is complete (sU is per wave)
+        if (live) {
+#pragma unroll
+            for (int m = 0; m < MM; ++m)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int k = 16 * m + mfma_row<T>(g, i);
+                    if (k < Nbf) {
+#pragma unroll
+                        for (int u = 0; u < Q; ++u) U[j * BSP + k + u * Nbf] = Y[m][u][i];
+                    }
+                }
+            if constexpr (VTAIL) {
+                if (g == 0) {
+#pragma unroll
+                    for (int kt = 0; kt < NT; ++kt)
+#pragma unroll
+                        for (int u = 0; u < Q; ++u) U[j * BSP + 16 * MM + kt + u * Nbf] = Yt[kt][u];
+                }
+            }
+            if constexpr (!DEEP) load_slots(ls);
+        }
+        // x again for x . y (one column only: Krylov solvers), all loads issued before the first use
+        T xv[NO];
+        if (a.dot && live) {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) xv[o] = xw[min(64 * o + lane, nout - 1)];
+        }
+        wave_sync();
+        double dot = 0.0;
+        if (live) {
+            // y = volume + the element's facet slots
+            T* dst = y + e0 * BS;
+#pragma unroll
+            for (int o = 0; o < NO; ++o) {
+                const int i = 64 * o + lane;
+                if (i < nout) {
+                    const int jj = i / BS, rem = i - jj * BS;
+                    const T v = U[jj * BSP + rem] + ls[o];
+                    dst[i] = v;
+                    if (a.dot) dot += double(xv[o]) * double(v);
+                }
             }
         }
-        if constexpr (!DEEP) load_slots(ls);
-    }
-    // x again for x . y (one column only: Krylov solvers), all loads issued before the first use
-    T xv[NO];
-    if (a.dot && live) {
+        if (a.dot) {
+            // one partial per block: the wave sums, then the waves' sums in a fixed order
+            __shared__ double red[W];
+            for (int o = 32; o > 0; o >>= 1) dot += __shfl_down(dot, o, 64);
+            if (lane == 0) red[wv] = dot;
+            __syncthreads();
+            if (tid == 0) {
+                double t = 0.0;
 #pragma unroll
-        for (int o = 0; o < NO; ++o) xv[o] = xw[min(64 * o + lane, nout - 1)];
-    }
-    wave_sync();
-    double dot = 0.0;
-    if (live) {
-        // y = volume + the element's facet slots
-        T* dst = y + e0 * BS;
-#pragma unroll
-        for (int o = 0; o < NO; ++o) {
-            const int i = 64 * o + lane;
-            if (i < nout) {
-                const int jj = i / BS, rem = i - jj * BS;
-                const T v = U[jj * BSP + rem] + ls[o];
-                dst[i] = v;
-                if (a.dot) dot += double(xv[o]) * double(v);
+                for (int w = 0; w < W; ++w) t += red[w];
+                a.dot[vb] = t;
             }
+            __syncthreads(); // red is rewritten by the next block
         }
-    }
-    if (a.dot) {
-        // one partial per block: the wave sums, then the waves' sums in a fixed order
-        __shared__ double red[W];
-        for (int o = 32; o > 0; o >>= 1) dot += __shfl_down(dot, o, 64);
-        if (lane == 0) red[wv] = dot;
-        __syncthreads();
-        if (tid == 0) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < W; ++w) t += red[w];
-            a.dot[vb] = t;
-        }
-        __syncthreads(); // red is rewritten by the next block
-    }
-    wave_sync(); // every read of the staged region is done before the next block restages it
+        wave_sync(); // every read of the staged region is done before the next block restages it
     };
     if constexpr (PERSIST) {
         for (unsigned vb = blockIdx.x; vb < nvb; vb += gridDim.x) block(vb);
