@@ -252,14 +252,19 @@ def run_precision(args):
         torch.cuda.synchronize()
         out["block_jacobi_setup_s"] = time.perf_counter() - t1
     xs = {}
-    for mode in ("fp64", "fp32-ru"):
+    # with block Jacobi the FP64 solve also runs with the FP32-stored blocks the FP32 solve applies
+    runs = [("fp64", pc), ("fp32-ru", pc)] + ([("fp64", "block-jacobi-f32")] if pc else [])
+    for mode, p in runs:
+        key = mode if p == pc else mode + "/fp32-blocks"
         x = torch.zeros_like(b)
         torch.cuda.synchronize()
-        r = op.solve(b, x, rtol=1e-10, mode=mode, precond=pc, max_iter=100000)
-        log("precision solve", mode, r)
-        out[mode] = {k: r[k] for k in ("seconds", "iterations", "outer_iterations", "rel_residual", "converged")}
-        xs[mode] = x
+        r = op.solve(b, x, rtol=1e-10, mode=mode, precond=p, max_iter=100000)
+        log("precision solve", key, r)
+        out[key] = {k: r[k] for k in ("seconds", "iterations", "outer_iterations", "rel_residual", "converged")}
+        xs[key] = x
     out["speedup_fp32ru_vs_fp64"] = out["fp64"]["seconds"] / out["fp32-ru"]["seconds"]
+    if pc:
+        out["speedup_fp32ru_vs_fp64_fp32blocks"] = out["fp64/fp32-blocks"]["seconds"] / out["fp32-ru"]["seconds"]
     x64, x32 = xs["fp64"], xs["fp32-ru"]
     out["solution_rel_l2_fp32ru_vs_fp64"] = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
     # traction on the boundary faces

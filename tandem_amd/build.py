@@ -22,6 +22,14 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TDG_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-variable", "-Wno-sign-compare",
           "-I" + os.path.join(ROOT, "include")]
+# PETSc (optional): with PETSC_DIR set, the MATSHELL adapter (csrc/petsc) is compiled against its
+# headers and the library links libpetsc; without it that translation unit is empty.
+PETSC_DIR = os.environ.get("PETSC_DIR")
+if PETSC_DIR:
+    COMMON += ["-I" + os.path.join(PETSC_DIR, "include")]
+    _arch = os.environ.get("PETSC_ARCH")
+    if _arch:
+        COMMON += ["-I" + os.path.join(PETSC_DIR, _arch, "include")]
 # Packed FP32 instructions are disabled for all device code on this platform.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
@@ -94,6 +102,9 @@ def build(verbose=True, jobs=None):
         objs = list(ex.map(compile_one, srcs))
     cmd = [HIPCC, "-shared", "-fopenmp", "--offload-arch=" + ARCH, "-o", LIB + ".tmp"] + objs + [
         "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib/llvm/lib"]
+    if PETSC_DIR:
+        libdir = os.path.join(PETSC_DIR, os.environ.get("PETSC_ARCH", ""), "lib")
+        cmd += ["-L" + libdir, "-Wl,-rpath," + libdir, "-lpetsc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s%s" % (r.stdout, r.stderr))
