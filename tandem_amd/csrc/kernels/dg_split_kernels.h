@@ -233,6 +233,9 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
 #pragma unroll
                 for (int r = 0; r < Q; ++r) C[r] = acc{0, 0, 0, 0};
                 const int ab = (tc * nq + min(4 * t + tq, nq - 1)) * NbfP;
+                // matrix-core phases run at raised wave priority, so that the SIMD's other wave's
+                // point arithmetic fills the matrix core's busy cycles rather than delaying the MFMAs
+                __builtin_amdgcn_s_setprio(1);
                 {
                     T av = sT[ab + min(g, Nbf - 1)], b[Q];
 #pragma unroll
@@ -257,6 +260,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
                         for (int r = 0; r < Q; ++r) b[r] = bn[r];
                     }
                 }
+                __builtin_amdgcn_s_setprio(0);
                 // ---- lane (j, g): side sd of facet fl at point q; register c = value (0) / d/dxi_e (1 + e)
                 // rows of G'_s = M_s^-1 with M_s = [t_1 .. t_(D-1), c_s], raw normal t_1 x t_2 (3D)
                 T R[D][D], nr[D], dt;
@@ -362,6 +366,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
                         }
                         Wt[1 + e][u] = wq * v;
                     }
+                __builtin_amdgcn_s_setprio(1);
                 // ---- lift: Y(k') += sum_(q,c) T0(c, q ; k') W(c, q)
                 {
                     T av = sT[qc * NbfP + min(j, Nbf - 1)];
@@ -390,6 +395,7 @@ void elasticity_facet_kernel(SplitArgs<T> a, T const* __restrict__ xs, T* __rest
 #pragma unroll
                 for (int c = 0; c < DD + 2; ++c) rc[c] = rn[c];
             }
+            __builtin_amdgcn_s_setprio(0);
         }
 
         // ---- lifted rows back to element labels through LDS, then one contiguous slot per column
@@ -494,10 +500,10 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
         T const* const xw = x + (live ? e0 : 0) * BS; // this wave's 16 element vectors (contiguous)
         const int nout = nel * BS;
 
-        // point data of point q = 16 b + mfma_row(g, i) (clamped)
-        auto load_pd = [&](int b, T (&pd)[4][NCH]) {
+        // point data of point q = 16 b + mfma_row(g, i) (clamped), for registers i0 <= i < i1
+        auto load_pd = [&](int b, T (&pd)[4][NCH], int i0 = 0, int i1 = 4) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = i0; i < i1; ++i) {
                 const int q = min(16 * b + mfma_row<T>(g, i), Nq - 1);
                 T const* vd = a.VD + ((gj * Nq + q) * NCH) * 16 + j;
 #pragma unroll
@@ -547,7 +553,9 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
             for (int u = 0; u < Q; ++u) Yt[kt][u] = T(0);
         T ls[NO];
 
-        auto volume_block = [&](int b, T const (&P)[4][NCH]) {
+        // late: the point data of registers 2 and 3 are loaded here, once the block's reference
+        // gradients are on the matrix cores
+        auto volume_block = [&](int b, T (&P)[4][NCH], bool late) {
             acc C[D][Q];
 #pragma unroll
             for (int e = 0; e < D; ++e)
@@ -571,6 +579,7 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
                     for (int r = 0; r < Q; ++r) C[e][r] = Mma<T>::mma(av, bu[r], C[e][r]);
                 }
             }
+            if (late) load_pd(b, P, 2, 4);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 if (16 * b + mfma_row<T>(0, i) >= Nq) continue; // all four lane groups past the last point
@@ -639,7 +648,7 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
                     T pn[4][NCH];
                     if (b + 1 < QB) load_pd(b + 1, pn);
                     else load_slots(ls);
-                    volume_block(b, pd);
+                    volume_block(b, pd, false);
                     if (b + 1 < QB) {
 #pragma unroll
                         for (int i = 0; i < 4; ++i)
@@ -650,8 +659,13 @@ void elasticity_element_kernel(SplitArgs<T> a, T const* __restrict__ xs, T const
             } else {
 #pragma unroll 1
                 for (int b = 0; b < QB; ++b) {
-                    load_pd(b, pd);
-                    volume_block(b, pd);
+                    // the point data are loaded before the matrix-core work that does not need them:
+                    // FP32 all four registers; FP64 (fewer registers to spare) two, and the other two
+                    // once the block's reference-gradient MFMAs are issued
+                    constexpr bool half = sizeof(T) == 8;
+                    load_pd(b, pd, 0, half ? 2 : 4);
+                    __builtin_amdgcn_sched_barrier(0);
+                    volume_block(b, pd, half);
                 }
             }
         }
