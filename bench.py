@@ -459,7 +459,7 @@ def run_multimesh(args):
     lockstep, one apply launch over every mesh's element groups per iteration (FP32 MFMA + FP64
     residual replacement, block Jacobi, towards 1e-10 within args.iters iterations, each segment
     stopping on its own); compared with the same 32 solves one mesh at a time with an operator per
-    mesh and the same budget."""
+    mesh and the same budget.  Every operator runs the FP32 kernel variant its autotuner picks."""
     import torch
     import tandem_amd as T
     from tandem_amd import scenario as S
@@ -476,6 +476,7 @@ def run_multimesh(args):
     op.set_material(lf(X), mf(X))
     op.upload()
     op.block_jacobi_build()
+    variant, _ = op.autotune(ncols=1, precision="fp32")   # the union's fastest FP32 kernel
     bs = op.bs
     # right-hand side: each mesh's uniform slip, placed in its segment
     parts = []
@@ -501,6 +502,7 @@ def run_multimesh(args):
                        "one unit-slip right-hand side each" % (len(frs), counts.min(), counts.max(), op.E, op.n),
            "iterations_budget": args.iters,
            "element_counts": [int(m.num_elements) for m in meshes],
+           "union_kernel_variant": variant,
            "lockstep_seconds": t_lock, "lockstep_iterations": int(r["iterations"]),
            "lockstep_seconds_per_iteration": t_lock / max(1, int(r["iterations"])),
            "segments_converged": int(np.sum(r["segment_rel_residual"] <= 1e-10)),
@@ -514,6 +516,7 @@ def run_multimesh(args):
         sub.set_material(lf(Xi), mf(Xi))
         sub.upload()
         sub.block_jacobi_build()
+        sub.autotune(ncols=1, precision="fp32")            # each mesh's fastest FP32 kernel too
         bi = torch.empty(sub.n, dtype=torch.float64, device="cuda")
         sub.rhs_device(bi, None, torch.from_numpy(parts[i].reshape(-1)).cuda())
         xi = torch.zeros_like(bi)
