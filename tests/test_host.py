@@ -260,3 +260,34 @@ def test_tensor_arguments_are_checked_before_the_library_is_called():
     for i, (exc, call) in enumerate(calls):
         with pytest.raises(exc):
             call()
+
+
+def test_pmultigrid_rejects_inconsistent_levels():
+    """The p-multigrid hierarchy checks its levels on the host before any device work: at least two,
+    one problem on one mesh, strictly decreasing degree, sane options; a replaced penalty needs one
+    value per facet."""
+    m = T.Mesh.box(2, 3, 0, T.BC_DIRICHLET)
+    m2 = T.Mesh.box(2, 4, 0, T.BC_DIRICHLET)
+
+    def make(mesh, p, kind=T.ELASTICITY):
+        op = T.DGOperator(mesh, kind, p)
+        op.set_material(np.full((op.E, op.Nq), 2.0), None if kind == T.POISSON else np.full((op.E, op.Nq), 1.0))
+        return op
+
+    p4, p2, p2b = make(m, 4), make(m, 2), make(m2, 2)
+    cases = [
+        ([p4], {}, "bad p-multigrid arguments"),
+        ([p2, p4], {}, "decreasing degree"),
+        ([p4, p4], {}, "decreasing degree"),
+        ([p4, p2b], {}, "one mesh"),
+        ([p4, make(m, 2, T.POISSON)], {}, "one mesh"),
+        ([p4, p2], {"smooth": 0}, "options"),
+        ([p4, p2], {"eig_lo": 1.2}, "options"),
+        ([p4, p2], {"coarse_rtol": 0.0}, "options"),
+    ]
+    for levels, kw, msg in cases:
+        with pytest.raises(L.TdgError, match=msg):
+            T.PMultigrid(levels, **kw)
+    unset = T.DGOperator(m, T.ELASTICITY, 2)
+    with pytest.raises(L.TdgError, match="set_material"):
+        T.PMultigrid([p4, unset])
