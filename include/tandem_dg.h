@@ -93,7 +93,8 @@ int tdg_mult_f32(tdg_op* op, float const* x, float* y);
  * library (each GPU holds it until the process ends): operators whose slots need more
  * (elements x (dim+1) x block size doubles > 2^23, about 20,000 P4 tetrahedra) are rejected with
  * TDG_ERR_ARG.  The operator must be on the device already (tdg_upload, which also sets up the array
- * on the current device, or one tdg_mult), else TDG_ERR_STATE.  All calls on one GPU share the
+ * on the current device, or one tdg_mult), else TDG_ERR_STATE; with TDG_VARIANT_ASSEMBLED it must
+ * have been assembled (tdg_assemble or one tdg_mult), and the apply then uses no scratch memory.  All calls on one GPU share the
  * array: calls that overlap in time on one GPU race; calls ordered on one stream do not. */
 int tdg_mult_noalloc(tdg_op* op, double const* x, double* y);
 /* Y(:, c) = A X(:, c) for ncols columns with leading dimension ld (>= local size). */
@@ -110,14 +111,19 @@ int tdg_set_stream(tdg_op* op, void* stream);
  * TDG_VARIANT_POINTS always runs the thread-per-point kernel.  TDG_VARIANT_MFMA_ELEMENT runs the
  * element-centric MFMA kernel (each element evaluates both sides of its faces, as the reference's
  * element loop does), TDG_VARIANT_MFMA_XCD the same kernel with its workgroups renumbered so that
- * each XCD takes one contiguous range of elements.  All compute the same operator; they differ in
- * summation order only. */
+ * each XCD takes one contiguous range of elements.  TDG_VARIANT_ASSEMBLED multiplies by the
+ * assembled block rows (tdg_assemble; assembled by the first apply if needed): a register kernel
+ * that reads each block row once per range of columns for block sizes 3, 6, 12 (2D) and 4, 12 (3D),
+ * i.e. the P1 / P2 levels of p-multigrid, and a generic kernel for every other order (a cross-check
+ * of the matrix-free apply; the autotuner only considers the register kernel).  All compute the
+ * same operator; they differ in summation order only. */
 enum {
     TDG_VARIANT_AUTO = 0,
     TDG_VARIANT_POINTS = 1,
     TDG_VARIANT_MFMA_XCD = 2,
     TDG_VARIANT_MFMA_ELEMENT = 3,
-    TDG_NUM_VARIANTS = 4
+    TDG_VARIANT_ASSEMBLED = 4,
+    TDG_NUM_VARIANTS = 5
 };
 int tdg_set_variant(tdg_op* op, int variant);
 
@@ -234,6 +240,20 @@ int tdg_block_jacobi_apply(tdg_op* op, double const* r, double* z);
 int tdg_block_jacobi_apply_batched(tdg_op* op, double const* R, double* Z, int ncols, int64_t ld);
 int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block);
 
+/* Assembled operator: the block matrix tandem's PetscDGMatrix holds (BAIJ, one block row per
+ * element with its own block and one block per face neighbour, app/common/PetscDGMatrix.cpp:14-77)
+ * as DGOperator::assemble fills it (src/form/DGOperator.h:121-170, assemble_volume / skeleton /
+ * boundary, Elasticity.cpp:317-519, Poisson.cpp:225-381).  Here the blocks are probed from the FP64
+ * matrix-free apply on a distance-2 element colouring (no two elements within two face-steps share
+ * a colour) and kept on the device in FP64 and FP32 until the material or penalty changes:
+ * (faces + 2) x bs^2 values per element in each precision.  tdg_assemble builds it (once) and
+ * returns the number of colours; tdg_assembled_block copies block `slot` of element `element`'s
+ * block row (slot 0: the element's own block, 1 + f: the block of its face-f neighbour, zero
+ * without one) to the host, bs x bs row-major.  Fails for meshes where an element meets one
+ * neighbour through two faces. */
+int tdg_assemble(tdg_op* op, int* colours);
+int tdg_assembled_block(tdg_op* op, int64_t element, int slot, double* block);
+
 /* p-multigrid over polynomial degree (tandem: PCMG set up by PetscLinearSolver::setup_mg,
  * app/common/PetscLinearSolver.cpp:53-114, with the levels of MGConfig::levels,
  * app/common/MGConfig.cpp:11-40, and the options of examples/options/mg_cheby.cfg).
@@ -247,7 +267,10 @@ int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block);
  * D^-1 A (D: element-block diagonal) over [eig_lo, eig_hi] x lambda_max (lambda_max from eig_iters
  * Lanczos steps); the coarsest level is solved by block-Jacobi CG to coarse_rtol (at most
  * coarse_max_iter iterations; in FP32 with coarse_fp32).  With vcycle_fp32 the V-cycle (smoothing,
- * transfers) runs in FP32 on the FP32 copies of the operators; the outer CG stays FP64.
+ * transfers) runs in FP32 on the FP32 copies of the operators; the outer CG stays FP64.  With
+ * coarse_assembled the coarsest operator is switched to TDG_VARIANT_ASSEMBLED at set-up where its
+ * block size has the register kernel (P1 and P2 levels): the coarse CG then reads the assembled
+ * block rows once per range of columns instead of the per-point geometry once per column.
  * tdg_pmg_create keeps the level pointers (the operators must outlive the hierarchy);
  * tdg_pmg_setup builds the block-Jacobi inverses and the eigenvalue estimates (lambda_max: host
  * array of nlevels-1, may be NULL); tdg_pmg_vcycle applies one V-cycle z = M^-1 r to ncols device
@@ -265,6 +288,7 @@ typedef struct {
     int galerkin_penalty;
     int coarse_fp32;
     int vcycle_fp32;
+    int coarse_assembled;
 } tdg_pmg_opts;
 void tdg_pmg_default_opts(tdg_pmg_opts* opts);
 int tdg_pmg_create(tdg_op* const* levels, int nlevels, tdg_pmg_opts const* opts, tdg_pmg** out);

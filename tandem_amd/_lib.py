@@ -36,7 +36,7 @@ class SolveResult(C.Structure):
 class PMGOpts(C.Structure):
     _fields_ = [("smooth", C.c_int), ("eig_lo", C.c_double), ("eig_hi", C.c_double), ("eig_iters", C.c_int),
                 ("coarse_rtol", C.c_double), ("coarse_max_iter", C.c_int), ("galerkin_penalty", C.c_int),
-                ("coarse_fp32", C.c_int), ("vcycle_fp32", C.c_int)]
+                ("coarse_fp32", C.c_int), ("vcycle_fp32", C.c_int), ("coarse_assembled", C.c_int)]
 
 
 class MeshDesc(C.Structure):
@@ -58,7 +58,7 @@ EXPORTS = [
     "tdg_create", "tdg_destroy", "tdg_last_error", "tdg_sizes", "tdg_block_size", "tdg_local_size",
     "tdg_volume_points", "tdg_facet_points",
     "tdg_facets", "tdg_set_material", "tdg_upload", "tdg_mult", "tdg_mult_f32", "tdg_mult_noalloc", "tdg_mult_batched", "tdg_mult_batched_f32",
-    "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_solve_multi", "tdg_solve_segments", "tdg_autotune", "tdg_autotune_shapes", "tdg_set_shape", "tdg_rhs", "tdg_rhs_device", "tdg_traction", "tdg_block_jacobi_build", "tdg_block_jacobi_apply", "tdg_block_jacobi_apply_batched", "tdg_block_jacobi_inverse", "tdg_material_gradient",
+    "tdg_set_stream", "tdg_set_variant", "tdg_solve", "tdg_solve_multi", "tdg_solve_segments", "tdg_autotune", "tdg_autotune_shapes", "tdg_set_shape", "tdg_rhs", "tdg_rhs_device", "tdg_traction", "tdg_block_jacobi_build", "tdg_block_jacobi_apply", "tdg_block_jacobi_apply_batched", "tdg_block_jacobi_inverse", "tdg_assemble", "tdg_assembled_block", "tdg_material_gradient",
     "tdg_material_gradient_f32", "tdg_fault_setup", "tdg_fault_slip", "tdg_fault_traction", "tdg_inverse_mass", "tdg_wave_rhs", "tdg_interpolation_matrix", "tdg_interpolate",
     "tdg_apply_counts", "tdg_host_array", "tdg_box_mesh", "tdg_reference_nodes",
     "tdg_pmg_default_opts", "tdg_pmg_create", "tdg_pmg_destroy", "tdg_pmg_setup", "tdg_pmg_vcycle",
@@ -100,6 +100,8 @@ def lib():
     L.tdg_block_jacobi_apply.argtypes = [vp, vp, vp]
     L.tdg_block_jacobi_apply_batched.argtypes = [vp, vp, vp, C.c_int, C.c_int64]
     L.tdg_block_jacobi_inverse.argtypes = [vp, C.c_int64, _dp]
+    L.tdg_assemble.argtypes = [vp, C.POINTER(C.c_int)]
+    L.tdg_assembled_block.argtypes = [vp, C.c_int64, C.c_int, _dp]
     L.tdg_material_gradient.argtypes = [vp, vp, vp, vp, vp]
     L.tdg_material_gradient_f32.argtypes = [vp, vp, vp, vp, vp]
     L.tdg_fault_setup.argtypes = [vp, _dp, _dp, _ip64]

@@ -290,6 +290,7 @@ void tdg_pmg_default_opts(tdg_pmg_opts* opts) {
     opts->galerkin_penalty = d.galerkin_penalty;
     opts->coarse_fp32 = d.coarse_fp32;
     opts->vcycle_fp32 = d.vcycle_fp32;
+    opts->coarse_assembled = d.coarse_assembled;
 }
 
 int tdg_pmg_create(tdg_op* const* levels, int nlevels, tdg_pmg_opts const* opts, tdg_pmg** out) {
@@ -309,6 +310,7 @@ int tdg_pmg_create(tdg_op* const* levels, int nlevels, tdg_pmg_opts const* opts,
             o.galerkin_penalty = opts->galerkin_penalty;
             o.coarse_fp32 = opts->coarse_fp32;
             o.vcycle_fp32 = opts->vcycle_fp32;
+            o.coarse_assembled = opts->coarse_assembled;
         }
         auto h = std::make_unique<tdg_pmg>();
         h->impl = std::make_unique<tdg::PMG>(ops, o);
@@ -444,6 +446,24 @@ int tdg_block_jacobi_inverse(tdg_op* op, int64_t element, double* block) {
         auto& o = ready(op);
         require(block, "null argument");
         o.block_jacobi().inverse_block(element, block);
+    });
+}
+
+int tdg_assemble(tdg_op* op, int* colours) {
+    return guarded([&] {
+        const int c = ready(op).assemble();
+        if (colours) *colours = c;
+    });
+}
+
+int tdg_assembled_block(tdg_op* op, int64_t element, int slot, double* block) {
+    return guarded([&] {
+        auto& o = ready(op);
+        require(block, "null argument");
+        require(element >= 0 && element < int64_t(o.num_elements()) && slot >= 0 && slot <= int(o.dim()) + 1,
+                "no such block");
+        o.assemble();
+        o.assembled_block(element, slot, block);
     });
 }
 

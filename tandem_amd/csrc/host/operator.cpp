@@ -407,6 +407,7 @@ void Operator::drop_device() {
     s32_.reset();
     rhs64_.reset();
     bj_.reset();
+    asm_.reset();
     g64_.reset();
     g32_.reset();
 }
@@ -587,6 +588,11 @@ std::vector<double> Operator::autotune(int ncols, int pb) {
     std::vector<double> ms;
     try {
         time_configs(ncols, pb, kNumVariants, [&](int v) {
+            if (v == Variant_Assembled) {
+                if (!assembled_fast()) return false; // the generic kernel is a cross-check only
+                variant_ = v;
+                return true;
+            }
             if (v != Variant_Points && !mfma) return false;
             variant_ = v;
             return true;
@@ -610,7 +616,8 @@ void Operator::set_split_shape(int pb, int shape, int xcd) {
 }
 
 bool Operator::uses_mfma() const {
-    return variant_ != Variant_Points && kind_ == Kind_Elasticity && mfma_supported(D_, N_);
+    return variant_ != Variant_Points && variant_ != Variant_Assembled && kind_ == Kind_Elasticity &&
+           mfma_supported(D_, N_);
 }
 
 bool Operator::uses_split() const { return variant_ == Variant_Auto && uses_mfma(); }
@@ -1261,6 +1268,104 @@ BlockJacobi& Operator::block_jacobi() {
     return *bj_;
 }
 
+// Assembled block rows by probing.  Colouring: no two elements within two face-steps of each other
+// share a colour, so with x = e_j on every element of colour c, y on an element of that colour is
+// column j of its own block, and y on any other element meets at most one element of colour c
+// among its face neighbours: column j of the block coupling it to that neighbour.
+int Operator::assemble() {
+    if (!ready_) throw std::logic_error("set_material() must be called before assembling");
+    if (asm_) return asm_->colours;
+    const unsigned NF = D_ + 1;
+    const int bs = int(block_size());
+    const int64_t E = int64_t(E_), n = int64_t(local_size());
+    std::vector<int32_t> nb(std::size_t(NF) * E_);
+    for (std::size_t e = 0; e < E_; ++e)
+        for (unsigned f = 0; f < NF; ++f) {
+            const int64_t o = topo_.sides[e * NF + f].nb;
+            nb[std::size_t(f) * E_ + e] = int32_t(o);
+            for (unsigned g = 0; g < f; ++g)
+                if (o != int64_t(e) && nb[std::size_t(g) * E_ + e] == int32_t(o))
+                    throw std::runtime_error("assembly: element " + std::to_string(e) +
+                                             " meets one neighbour through two faces");
+        }
+    std::vector<int32_t> col(E_, -1);
+    int nc = 0;
+    std::vector<char> used;
+    for (std::size_t e = 0; e < E_; ++e) {
+        used.assign(std::size_t(nc) + 1, 0);
+        for (unsigned f = 0; f < NF; ++f) {
+            const int64_t o = nb[std::size_t(f) * E_ + e];
+            if (o == int64_t(e)) continue;
+            if (col[std::size_t(o)] >= 0) used[std::size_t(col[std::size_t(o)])] = 1;
+            for (unsigned g = 0; g < NF; ++g) {
+                const int64_t p = nb[std::size_t(g) * E_ + std::size_t(o)];
+                if (p != int64_t(e) && col[std::size_t(p)] >= 0) used[std::size_t(col[std::size_t(p)])] = 1;
+            }
+        }
+        int c = 0;
+        while (used[std::size_t(c)]) ++c;
+        col[e] = c;
+        nc = std::max(nc, c + 1);
+    }
+    auto a = std::make_unique<DeviceAssembled>();
+    a->colours = nc;
+    a->nb.upload(nb);
+    DeviceArray<int32_t> dcol;
+    dcol.upload(col);
+    const std::size_t entries = std::size_t(NF + 1) * std::size_t(bs) * std::size_t(bs) * E_;
+    a->A64.alloc(entries);
+    TDG_HIP(hipMemsetAsync(a->A64.ptr, 0, entries * sizeof(double), stream_));
+    // probe columns per batched apply: at most 1 GiB for each of X and Y
+    const int chunk = int(std::max<int64_t>(1, std::min<int64_t>(bs, (int64_t(1) << 30) / (n * 8))));
+    DeviceArray<double> X, Y;
+    X.alloc(std::size_t(chunk) * std::size_t(n));
+    Y.alloc(std::size_t(chunk) * std::size_t(n));
+    const int saved = variant_;
+    if (variant_ == Variant_Assembled) variant_ = Variant_Auto; // probe the matrix-free apply
+    try {
+        for (int c = 0; c < nc; ++c)
+            for (int c0 = 0; c0 < bs; c0 += chunk) {
+                const int m = std::min(chunk, bs - c0);
+                TDG_HIP(launch_assembly_probe(X.ptr, dcol.ptr, c, c0, m, E, bs, stream_));
+                dispatch(static_cast<double const*>(X.ptr), Y.ptr, m, n);
+                TDG_HIP(launch_assembly_gather(Y.ptr, dcol.ptr, a->nb.ptr, a->A64.ptr, c, c0, m, E, bs, int(NF),
+                                               stream_));
+            }
+    } catch (...) {
+        variant_ = saved;
+        throw;
+    }
+    variant_ = saved;
+    a->A32.alloc(entries);
+    TDG_HIP(launch_narrow(a->A64.ptr, a->A32.ptr, int64_t(entries), stream_));
+    TDG_HIP(hipStreamSynchronize(stream_));
+    asm_ = std::move(a);
+    return nc;
+}
+
+void Operator::assembled_block(int64_t e, int slot, double* out) const {
+    if (!asm_) throw std::logic_error("the operator has not been assembled");
+    if (e < 0 || e >= int64_t(E_) || slot < 0 || slot > int(D_ + 1)) throw std::out_of_range("no such block");
+    const std::size_t bs = block_size();
+    TDG_HIP(hipStreamSynchronize(stream_));
+    // entries (slot, i, j) of element e lie E apart
+    TDG_HIP(hipMemcpy2D(out, sizeof(double), asm_->A64.ptr + (std::size_t(slot) * bs * bs) * E_ + std::size_t(e),
+                        E_ * sizeof(double), sizeof(double), bs * bs, hipMemcpyDeviceToHost));
+}
+
+bool Operator::assembled_fast() const { return assembled_register_kernel(int(block_size()), int(D_ + 2)); }
+
+template <class T> AssembledArgs<T> Operator::assembled_args() const {
+    AssembledArgs<T> a{};
+    if constexpr (std::is_same<T, double>::value) a.A = asm_->A64.ptr;
+    else a.A = asm_->A32.ptr;
+    a.nb = asm_->nb.ptr;
+    a.num_elements = int64_t(E_);
+    a.block_size = int(block_size());
+    a.slots = int(D_ + 2);
+    return a;
+}
+
 void Operator::traction_device(double const* x, double const* facet_q, double* t) {
     if (!ready_) throw std::logic_error("set_material() must be called first");
     RhsArgs a = rhs_args();
@@ -1402,6 +1507,11 @@ void Operator::launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, 
 
 template <class T> void Operator::dispatch(T const* x, T* y, int ncols, int64_t ld) {
     if (!ready_) throw std::logic_error("set_material() must be called before apply");
+    if (variant_ == Variant_Assembled) {
+        if (!asm_) assemble();
+        TDG_HIP(launch_assembled<T>(assembled_args<T>(), x, y, ncols, ld, stream_));
+        return;
+    }
     upload_device();
     if (uses_split()) {
         int kq = 0, kf = 0;
@@ -1477,6 +1587,12 @@ std::size_t Operator::slot_entries() const {
 
 void Operator::apply_noalloc(double const* x, double* y, double* scratch, std::size_t cap) {
     if (!ready_) throw std::logic_error("set_material() must be called before apply");
+    if (variant_ == Variant_Assembled) { // no scratch memory; the blocks must already be assembled
+        if (!asm_) throw std::logic_error("the allocation-free apply needs the assembled operator: call tdg_assemble "
+                                          "(or one tdg_mult) first");
+        TDG_HIP(launch_assembled<double>(assembled_args<double>(), x, y, 1, 0, stream_));
+        return;
+    }
     // everything the selected kernels read must already be on the device: nothing is uploaded here
     const bool have = d64_ && (!uses_mfma() || (m64_ && (uses_split() ? bool(s64_) : m64_->FD.n > 0)));
     if (!have)

@@ -92,12 +92,28 @@ struct DeviceFault {
     int nbf = 0;
 };
 
+// Assembled block rows of the operator (kernels/dg_assembled.hip): FP64 blocks probed from the
+// matrix-free apply, an FP32 copy, the face neighbours and the number of probe colours.
+struct DeviceAssembled {
+    DeviceArray<double> A64;
+    DeviceArray<float> A32;
+    DeviceArray<int32_t> nb;
+    int colours = 0;
+};
+
 // Variant_Auto uses the two-kernel (facet + element) MFMA apply where one is compiled and the
 // thread-per-point kernel otherwise; Variant_Points forces the thread-per-point kernel;
 // Variant_MfmaXcd / Variant_MfmaElement run the element-centric MFMA kernel (every element
-// evaluates both sides of its faces) with workgroups renumbered per XCD / in linear order.
-enum Variant : int { Variant_Auto = 0, Variant_Points = 1, Variant_MfmaXcd = 2, Variant_MfmaElement = 3 };
-constexpr int kNumVariants = 4;
+// evaluates both sides of its faces) with workgroups renumbered per XCD / in linear order;
+// Variant_Assembled multiplies by the assembled block rows (assembled on first use).
+enum Variant : int {
+    Variant_Auto = 0,
+    Variant_Points = 1,
+    Variant_MfmaXcd = 2,
+    Variant_MfmaElement = 3,
+    Variant_Assembled = 4
+};
+constexpr int kNumVariants = 5;
 
 class BlockJacobi;
 
@@ -188,6 +204,17 @@ public:
     // Element-block Jacobi preconditioner (built on first use; rebuilt after set_material).
     BlockJacobi& block_jacobi();
 
+    // Assembled form (tandem's PetscDGMatrix filled by DGOperator::assemble): every block row
+    // (element) holds its own block and one block per face neighbour, probed from the matrix-free
+    // FP64 apply on a distance-2 colouring of the elements; kept on the device in FP64 and FP32 until
+    // set_material / set_penalty.  Returns the number of colours (built once).
+    int assemble();
+    bool assembled() const { return bool(asm_); }
+    // Block `slot` (0: own, 1 + f: face-f neighbour) of element e's block row, row-major, to host.
+    void assembled_block(int64_t e, int slot, double* out) const;
+    // Whether the assembled apply has its register kernel for this block size (P1 / P2 levels).
+    bool assembled_fast() const;
+
     // g_lam, g_mu (E x Nq, device) = d/d(lam_Q, mu_Q) of psi^T A u for device vectors u, psi
     // (elasticity): the adjoint gradient kernel.
     void material_gradient(double const* u, double const* psi, double* g_lam, double* g_mu);
@@ -233,6 +260,7 @@ private:
     template <class T> MfmaArgs<T> mfma_args(DeviceMfma<T> const& d) const;
     template <class T> void launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld);
     template <class T> void dispatch(T const* x, T* y, int ncols, int64_t ld);
+    template <class T> AssembledArgs<T> assembled_args() const;
 
     int kind_;
     unsigned D_, N_, Q_, Nbf_, Nq_, nq_;
@@ -273,6 +301,7 @@ private:
     std::unique_ptr<DeviceSplit<float>> s32_;
     std::unique_ptr<DeviceRhs> rhs64_;
     std::unique_ptr<BlockJacobi> bj_;
+    std::unique_ptr<DeviceAssembled> asm_;
     std::unique_ptr<DeviceGrad<double>> g64_;
     std::unique_ptr<DeviceGrad<float>> g32_;
     std::unique_ptr<DeviceWave> wave_;

@@ -68,6 +68,28 @@ hipError_t launch_interpolate(double const* I, int n_to, int n_from, int nquant,
                               double const* x, double* y, hipStream_t stream);
 hipError_t launch_axpy(double* y, double alpha, double const* x, int64_t n, hipStream_t stream);
 
+// Assembled block rows (one per element): A[slot][row][col][element] with slot 0 the element's own
+// block and slot 1 + f the block of its face-f neighbour nb[f][element] (the element itself and a zero
+// block where face f has none); block_size x block_size blocks, slots = faces + 1.
+template <class T> struct AssembledArgs {
+    T const* A;
+    int32_t const* nb;
+    int64_t num_elements;
+    int block_size, slots;
+};
+// y_c = A x_c for ncols columns at stride ld (the register kernel for the block sizes of
+// assembled_register_kernel, the generic kernel otherwise).
+template <class T>
+hipError_t launch_assembled(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t stream);
+bool assembled_register_kernel(int block_size, int slots);
+// Assembly by probing (FP64): X (m columns of E bs) = unit vectors c0 .. c0 + m - 1 on the elements of
+// colour c; gather stores the isolated block columns of Y = A X into the assembled layout.
+hipError_t launch_assembly_probe(double* X, int32_t const* colour, int c, int c0, int m, int64_t E, int bs,
+                                 hipStream_t stream);
+hipError_t launch_assembly_gather(double const* Y, int32_t const* colour, int32_t const* nb, double* A, int c, int c0,
+                                  int m, int64_t E, int bs, int faces, hipStream_t stream);
+hipError_t launch_narrow(double const* a, float* b, int64_t n, hipStream_t stream);
+
 // Volume / facet rule sizes the kernels are compiled for (2N+1 rules of tandem's autorule).
 int kernel_rule_sizes(int dim, int degree, int* Nq, int* nq);
 

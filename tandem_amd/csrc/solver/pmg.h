@@ -27,6 +27,8 @@ struct PMGOptions {
     int galerkin_penalty = 1;   // coarse levels take the fine level's facet penalty
     int coarse_fp32 = 1;        // coarse CG in FP32 (FP64 right-hand side and solution)
     int vcycle_fp32 = 1;        // smoothing and transfers in FP32 (the outer flexible CG stays FP64)
+    int coarse_assembled = 1;   // coarsest level on its assembled block rows where it has the register
+                                //   kernel (P1 / P2)
 };
 
 struct PMGLevel;

@@ -288,6 +288,11 @@ double PMG::estimate(int l) {
 
 void PMG::setup() {
     if (ready_) return;
+    Operator& coarse = *ops_.back();
+    if (o_.coarse_assembled && coarse.assembled_fast()) {
+        coarse.assemble();
+        coarse.set_variant(Variant_Assembled);
+    }
     for (auto* a : ops_) {
         a->upload_device();
         (void)a->block_jacobi();

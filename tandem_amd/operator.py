@@ -144,7 +144,7 @@ def _check(t, what, dtype="f64", numel=None, min_numel=None, cols=None, like=Non
 
 
 # kernel variants, indexed by TDG_VARIANT_*
-VARIANTS = ["auto", "points", "mfma-xcd", "mfma-element"]
+VARIANTS = ["auto", "points", "mfma-xcd", "mfma-element", "assembled"]
 # launch shapes of the two-kernel apply per precision (TDG_SPLIT_SHAPES) and workgroup orders
 # (TDG_SPLIT_ORDERS: linear, per XCD, per XCD persistent)
 SPLIT_SHAPES = 4
@@ -215,7 +215,8 @@ class DGOperator:
     def set_variant(self, variant):
         """"auto" (two-kernel matrix-core apply where compiled: facet kernel + element kernel),
         "points" (thread-per-point kernel), "mfma-element" (element-centric matrix-core kernel) or
-        "mfma-xcd" (the same, workgroups renumbered so each XCD takes one element range)."""
+        "mfma-xcd" (the same, workgroups renumbered so each XCD takes one element range) or
+        "assembled" (the assembled block rows, assembled on first use)."""
         check(lib().tdg_set_variant(self._h, VARIANTS.index(variant)))
 
     def set_stream(self, stream):
@@ -399,6 +400,20 @@ class DGOperator:
         check(lib().tdg_block_jacobi_inverse(self._h, element, out))
         return out.reshape(self.bs, self.bs, order="F")
 
+    def assemble(self):
+        """Assembled block rows on the device (tandem's PetscDGMatrix, DGOperator::assemble), probed
+        from the FP64 matrix-free apply; returns the number of probe colours.  The "assembled"
+        variant multiplies by them."""
+        c = C.c_int()
+        check(lib().tdg_assemble(self._h, C.byref(c)))
+        return c.value
+
+    def assembled_block(self, element, slot):
+        """Block `slot` (0: own, 1 + f: face-f neighbour) of an element's assembled block row."""
+        out = np.zeros(self.bs * self.bs)
+        check(lib().tdg_assembled_block(self._h, element, slot, out))
+        return out.reshape(self.bs, self.bs)
+
     def material_gradient(self, u, psi, g_lam=None, g_mu=None):
         """d/d(lam_Q, mu_Q) of psi^T A u for GPU vectors u, psi (float64 or float32); returns two
         (E, Nq) tensors."""
@@ -514,13 +529,14 @@ class PMultigrid:
     solve to coarse_rtol.  The operators must outlive the hierarchy (it keeps references)."""
 
     def __init__(self, levels, smooth=None, eig_lo=None, eig_hi=None, eig_iters=None, coarse_rtol=None,
-                 coarse_max_iter=None, galerkin_penalty=None, coarse_fp32=None, vcycle_fp32=None):
+                 coarse_max_iter=None, galerkin_penalty=None, coarse_fp32=None, vcycle_fp32=None,
+                 coarse_assembled=None):
         o = _lib.PMGOpts()
         lib().tdg_pmg_default_opts(C.byref(o))
         for k, v in (("smooth", smooth), ("eig_lo", eig_lo), ("eig_hi", eig_hi), ("eig_iters", eig_iters),
                      ("coarse_rtol", coarse_rtol), ("coarse_max_iter", coarse_max_iter),
                      ("galerkin_penalty", galerkin_penalty), ("coarse_fp32", coarse_fp32),
-                     ("vcycle_fp32", vcycle_fp32)):
+                     ("vcycle_fp32", vcycle_fp32), ("coarse_assembled", coarse_assembled)):
             if v is not None:
                 setattr(o, k, type(getattr(o, k))(v))
         self.options = {k: getattr(o, k) for k, _ in _lib.PMGOpts._fields_}

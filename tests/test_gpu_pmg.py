@@ -86,6 +86,17 @@ def test_vcycle_symmetric_positive_and_eigen_estimate(cuda):
     d32 = (torch.linalg.norm(Z32 - Z) / torch.linalg.norm(Z)).item()
     print("FP32 V-cycle vs FP64: %.3e" % d32, flush=True)
     assert d32 <= 1e-4
+    # the coarsest level ran on its assembled block rows (the default where the block size has the
+    # register kernel); the matrix-free coarse operator gives the same V-cycle to the coarse tolerance
+    ops[-1].set_variant("auto")
+    M0 = T.PMultigrid(ops, coarse_rtol=1e-14, coarse_max_iter=5000, coarse_fp32=0, vcycle_fp32=0,
+                      coarse_assembled=0)
+    Z0 = torch.zeros_like(X)
+    M0.vcycle(X, Z0)
+    torch.cuda.synchronize()
+    d0 = (torch.linalg.norm(Z0 - Z) / torch.linalg.norm(Z)).item()
+    print("matrix-free coarse level vs assembled: %.3e" % d0, flush=True)
+    assert d0 <= 1e-10
     # power iteration on D^-1 A of the fine level: the Lanczos estimate is a lower bound of the
     # largest eigenvalue, as is every power iterate
     op = ops[0]
