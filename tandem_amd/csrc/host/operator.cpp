@@ -452,6 +452,7 @@ void Operator::upload_device() {
     } else if (uses_mfma()) {
         ensure_mfma_faces();
     }
+    if (variant_ == Variant_Assembled && !asm_) assemble(); // before any graph capture of the apply
 }
 
 // The element-face records of the element-centric MFMA kernel (also read by the matrix-core

@@ -18,6 +18,8 @@ namespace tdg {
 
 class Operator;
 
+constexpr int kDotParts = 256; // partial sums per column of the fused block application and dot
+
 class BlockJacobi {
 public:
     BlockJacobi() = default;
@@ -38,6 +40,12 @@ public:
     // FP64 vectors and arithmetic with the FP32-stored blocks (half the bytes of the FP64 blocks;
     // the same fixed symmetric preconditioner for an FP64 solve as the FP32 one applies)
     void apply_fp32_blocks(double const* r, double* z, hipStream_t s, int ncols = 1, int64_t ld = 0) const;
+
+    // Small blocks only (returns false otherwise, having done nothing): z = D^-1 r and the
+    // per-column partial sums of r . z in partial[c kDotParts + i], i < kDotParts.
+    bool apply_dot(float const* r, float* z, hipStream_t s, int ncols, int64_t ld, double* partial) const;
+    bool apply_dot(double const* r, double* z, hipStream_t s, int ncols, int64_t ld, double* partial,
+                   bool fp32_blocks) const;
 
     // FP64 inverse block of element e (bs x bs, column-major) copied to the host, for checks.
     void inverse_block(int64_t e, double* out) const;
@@ -61,6 +69,13 @@ struct BlockJacobiApply {
     }
     void apply(float const* r, float* z, hipStream_t s, int ncols = 1, int64_t ld = 0) const {
         bj->apply(r, z, s, ncols, ld);
+    }
+    // z = D^-1 r with the partial sums of r . z fused in (small blocks; false: nothing done)
+    bool apply_dot(double const* r, double* z, hipStream_t s, int ncols, int64_t ld, double* partial) const {
+        return bj->apply_dot(r, z, s, ncols, ld, partial, fp32_blocks);
+    }
+    bool apply_dot(float const* r, float* z, hipStream_t s, int ncols, int64_t ld, double* partial) const {
+        return bj->apply_dot(r, z, s, ncols, ld, partial);
     }
 };
 

@@ -144,6 +144,69 @@ __global__ __launch_bounds__(256) void apply_blocks_cols(TB const* __restrict__ 
     }
 }
 
+// Small blocks (P1 / P2 levels): one lane per element keeps the element's inverse block in registers
+// and applies it to a range of up to kSmallCols columns; waves stride over groups of 64 elements
+// with kDotParts workgroups per column range.  Optionally the per-workgroup partial sums of r . z of
+// every column (partial[c kDotParts + workgroup], the layout of the solvers' two-stage reductions).
+constexpr int kSmallCols = 16;
+template <class T, class TB, int BS>
+__global__ __launch_bounds__(256) void apply_blocks_small(TB const* __restrict__ inv, T const* __restrict__ rs,
+                                                          T* __restrict__ zs, int64_t E, int ncols, int64_t ld,
+                                                          double* __restrict__ partial) {
+    __shared__ double red[4][kSmallCols];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = blockIdx.y * kSmallCols;
+    const int nc = min(kSmallCols, ncols - c0);
+    double dacc[kSmallCols];
+#pragma unroll
+    for (int k = 0; k < kSmallCols; ++k) dacc[k] = 0.0;
+    const int64_t groups = (E + 63) / 64;
+    for (int64_t g = int64_t(blockIdx.x) * 4 + wave; g < groups; g += int64_t(gridDim.x) * 4) {
+        const int64_t e = g * 64 + lane;
+        const bool act = e < E;
+        const int64_t ec = act ? e : 0;
+        TB b[BS * BS];
+#pragma unroll
+        for (int t = 0; t < BS * BS; ++t) b[t] = inv[ec * (BS * BS) + t]; // column-major block
+#pragma unroll
+        for (int k = 0; k < kSmallCols; ++k) {
+            if (k >= nc) break;
+            T const* r = rs + int64_t(c0 + k) * ld + ec * BS;
+            T rv[BS];
+#pragma unroll
+            for (int j = 0; j < BS; ++j) rv[j] = r[j];
+            T z[BS];
+#pragma unroll
+            for (int i = 0; i < BS; ++i) z[i] = T(0);
+#pragma unroll
+            for (int j = 0; j < BS; ++j)
+#pragma unroll
+                for (int i = 0; i < BS; ++i) z[i] += T(b[i + j * BS]) * rv[j];
+            if (act) {
+                T* zo = zs + int64_t(c0 + k) * ld + ec * BS;
+                double d = 0.0;
+#pragma unroll
+                for (int i = 0; i < BS; ++i) {
+                    zo[i] = z[i];
+                    d += double(rv[i]) * double(z[i]);
+                }
+                dacc[k] += d;
+            }
+        }
+    }
+    if (!partial) return;
+#pragma unroll
+    for (int k = 0; k < kSmallCols; ++k) {
+        double v = dacc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nc)
+        partial[int64_t(c0 + threadIdx.x) * kDotParts + blockIdx.x] =
+            ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+
 unsigned grid_for(int64_t items) { return unsigned(std::min<int64_t>((items + kThreads - 1) / kThreads, 65536)); }
 
 } // namespace
@@ -235,8 +298,41 @@ void BlockJacobi::build(Operator& op) {
 }
 
 namespace {
+template <class T, class TB, int BS>
+void launch_small(TB const* inv, T const* r, T* z, int64_t E, int ncols, int64_t ld, double* partial, hipStream_t s) {
+    hipLaunchKernelGGL((apply_blocks_small<T, TB, BS>), dim3(kDotParts, unsigned((ncols + kSmallCols - 1) / kSmallCols)),
+                       dim3(256), 0, s, inv, r, z, E, ncols, ld, partial);
+    BHIP(hipGetLastError());
+}
+
+// The register kernel's block sizes: elasticity P1 (2D 6, 3D 12) and P2 (2D 12), Poisson P1 / P2
+// (3, 4, 6, 10); 10 and 12 in FP32 only.
+template <class T, class TB>
+bool apply_small(TB const* inv, T const* r, T* z, int64_t E, int bs, int ncols, int64_t ld, double* partial,
+                 hipStream_t s) {
+    if (ncols < 1 || (ncols > 1 && ld < E * bs)) return false;
+    switch (bs) {
+    case 3: launch_small<T, TB, 3>(inv, r, z, E, ncols, ld, partial, s); return true;
+    case 4: launch_small<T, TB, 4>(inv, r, z, E, ncols, ld, partial, s); return true;
+    case 6: launch_small<T, TB, 6>(inv, r, z, E, ncols, ld, partial, s); return true;
+    default: break;
+    }
+    if constexpr (sizeof(T) == 4 && sizeof(TB) == 4) { // FP64 registers would not hold these blocks
+        if (bs == 10) {
+            launch_small<T, TB, 10>(inv, r, z, E, ncols, ld, partial, s);
+            return true;
+        }
+        if (bs == 12) {
+            launch_small<T, TB, 12>(inv, r, z, E, ncols, ld, partial, s);
+            return true;
+        }
+    }
+    return false;
+}
+
 template <class T, class TB>
 void launch_apply(TB const* inv, T const* r, T* z, int64_t E, int bs, int ncols, int64_t ld, hipStream_t s) {
+    if (apply_small(inv, r, z, E, bs, ncols, ld, static_cast<double*>(nullptr), s)) return;
     if (ncols <= 1) {
         hipLaunchKernelGGL((apply_blocks<T, TB>), dim3(unsigned(E), 1u), dim3(128), 0, s, inv, r, z, bs, ld);
     } else if (bs <= 64) {
@@ -263,6 +359,18 @@ void BlockJacobi::apply_fp32_blocks(double const* r, double* z, hipStream_t s, i
 void BlockJacobi::apply(float const* r, float* z, hipStream_t s, int ncols, int64_t ld) const {
     if (!inv32_) throw std::logic_error("block Jacobi preconditioner not built");
     launch_apply(inv32_, r, z, E_, bs_, ncols, ld, s);
+}
+
+bool BlockJacobi::apply_dot(float const* r, float* z, hipStream_t s, int ncols, int64_t ld, double* partial) const {
+    if (!inv32_) throw std::logic_error("block Jacobi preconditioner not built");
+    return apply_small(inv32_, r, z, E_, bs_, ncols, ld, partial, s);
+}
+
+bool BlockJacobi::apply_dot(double const* r, double* z, hipStream_t s, int ncols, int64_t ld, double* partial,
+                            bool fp32_blocks) const {
+    if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
+    return fp32_blocks ? apply_small(inv32_, r, z, E_, bs_, ncols, ld, partial, s)
+                       : apply_small(inv64_, r, z, E_, bs_, ncols, ld, partial, s);
 }
 
 void BlockJacobi::inverse_block(int64_t e, double* out) const {

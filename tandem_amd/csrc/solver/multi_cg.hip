@@ -26,6 +26,7 @@
 #include <cmath>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace tdg {
@@ -62,6 +63,16 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
     MultiWork& W = w ? *w : local;
     DBuf<double>& S = W.S;
     DBuf<double>& partial = W.partial;
+    static_assert(kDotParts == kG, "the fused block application writes the solver's partial-sum layout");
+    // z = M^-1 r and the partial sums of r . z: one fused kernel for small blocks
+    auto applyM_dot = [&](auto const* R, auto* Z) {
+        if (segments || !M->apply_dot(R, Z, st, ncols, ld, partial.p)) {
+            applyM(R, Z);
+            hipLaunchKernelGGL((mdot<std::remove_const_t<std::remove_pointer_t<decltype(R)>>,
+                                     std::remove_pointer_t<decltype(Z)>>),
+                               dim3(kG, ncols), dim3(kB), 0, st, R, Z, L, partial.p);
+        }
+    };
     S.ensure(std::size_t(ncols) * kS);
     partial.ensure(std::size_t(ncols) * kG);
     std::vector<double> hs(std::size_t(ncols) * kS, 0.0);
@@ -111,8 +122,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
         auto prec = [&](int slot) {
             if (M) {
-                applyM(r.p, z.p);
-                hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, z.p, L, partial.p);
+                applyM_dot(r.p, z.p);
             } else {
                 hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, L, partial.p);
             }
@@ -188,8 +198,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         // rho into slot `slot` of the columns flagged in slot msk (all if msk < 0)
         auto prec = [&](int slot, int msk) {
             if (M) {
-                applyM(r32.p, z.p);
-                hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, z.p, L, partial.p);
+                applyM_dot(r32.p, z.p);
             } else {
                 hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, r32.p, L, partial.p);
             }

@@ -14,7 +14,9 @@ from cases import Case, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(3, 2, 1, 4, 0), (3, 2, 0, 2, 0), (2, 3, 0, 6, 0), (2, 4, 0, 2, 1), (3, 2, 1, 2, 1)]
+# block sizes 105, 30, 56 (one workgroup per element) and 6, 10, 6, 12 (the small-block kernel)
+CASES = [(3, 2, 1, 4, 0), (3, 2, 0, 2, 0), (2, 3, 0, 6, 0), (2, 4, 0, 2, 1), (3, 2, 1, 2, 1), (2, 4, 0, 1, 0),
+         (3, 2, 1, 1, 0)]
 IDS = ["D%d-n%d-s%d-P%d-%s" % (c[:4] + ("elast" if c[4] == 0 else "poisson",)) for c in CASES]
 
 

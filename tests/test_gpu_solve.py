@@ -80,6 +80,34 @@ def test_multi_column_solve_matches_dense(cuda, mode, precond):
         assert rel_l2(Xh[j, :c.n], np.linalg.solve(A, Bh[j, :c.n])) <= 1e-7
 
 
+@pytest.mark.parametrize("variant", ["auto", "assembled"])
+def test_multi_column_solve_small_blocks(cuda, variant):
+    """P1 operators (the p-multigrid coarse level): the block-Jacobi application with its fused r . z
+    partial sums (small-block kernel) inside lockstep CG, on the matrix-free and the assembled apply,
+    FP64 and FP32 with FP64 replacement, 18 columns (a ragged second range of the 16-column kernel),
+    replayed from graphs and launched: every column reaches 1e-10 and the dense solution."""
+    import torch
+
+    c = Case(2, 6, 0, 1, 0, seed=12)
+    c.op.set_variant(variant)
+    A = c.oracle.assemble()
+    nc, ld = 18, c.n + 2
+    Bh = np.random.default_rng(13).uniform(-1, 1, (nc, ld))
+    B = torch.from_numpy(Bh).to(cuda)
+    for mode in ("fp64", "fp32-ru"):
+        its = {}
+        for graphs in (False, True):
+            X = torch.zeros_like(B)
+            r = c.op.solve_multi(B, X, rtol=1e-10, mode=mode, precond="block-jacobi", graphs=graphs)
+            print(variant, mode, graphs, r["iterations"], max(r["column_rel_residual"]), flush=True)
+            assert r["converged"]
+            Xh = X.cpu().numpy()
+            for j in (0, 9, 17):
+                assert rel_l2(Xh[j, :c.n], np.linalg.solve(A, Bh[j, :c.n])) <= 1e-7
+            its[graphs] = (r["iterations"], Xh)
+        assert its[True][0] == its[False][0] and np.array_equal(its[True][1], its[False][1])
+
+
 @pytest.mark.parametrize("mode,precond", [("fp64", None), ("fp32-ru", "block-jacobi"), ("fp32-ir", None)])
 def test_graph_replayed_solve_equals_launched(cuda, mode, precond):
     """The iterations between residual checks replayed from a captured HIP graph (operator on a
