@@ -105,7 +105,7 @@ int tdg_mult_batched_f32(tdg_op* op, float const* X, float* Y, int ncols, int64_
 int tdg_set_stream(tdg_op* op, void* stream);
 
 /* Kernel choice: TDG_VARIANT_AUTO (default) runs the two-kernel matrix-core (MFMA) apply where one
- * is compiled for the operator (elasticity, 3D P2-P4, 2D P2/P4/P6): a facet kernel evaluates each
+ * is compiled for the operator (elasticity, 3D P2-P4, 2D P2-P4 and P6): a facet kernel evaluates each
  * facet once (both sides' traces, the flux and both lifts) into per-element-face slots, an element
  * kernel applies the volume term and adds the slots; the thread-per-point kernel otherwise.
  * TDG_VARIANT_POINTS always runs the thread-per-point kernel.  TDG_VARIANT_MFMA_ELEMENT runs the

@@ -1620,6 +1620,22 @@ void Operator::apply(float const* x, float* y) { dispatch(x, y, 1, 0); }
 void Operator::apply_batched(double const* X, double* Y, int ncols, int64_t ld) { dispatch(X, Y, ncols, ld); }
 void Operator::apply_batched(float const* X, float* Y, int ncols, int64_t ld) { dispatch(X, Y, ncols, ld); }
 
+template <class T> bool Operator::apply_batched_dot_impl(T const* X, T* Y, int ncols, int64_t ld, double* partial) {
+    if (!ready_) throw std::logic_error("set_material() must be called before apply");
+    if (variant_ != Variant_Assembled || !assembled_fast()) return false;
+    if (!asm_) assemble();
+    const hipError_t e = launch_assembled<T>(assembled_args<T>(), X, Y, ncols, ld, stream_, partial);
+    if (e == hipErrorNotSupported) return false; // no fused kernel for this block size and precision
+    TDG_HIP(e);
+    return true;
+}
+bool Operator::apply_batched_dot(double const* X, double* Y, int ncols, int64_t ld, double* partial) {
+    return apply_batched_dot_impl(X, Y, ncols, ld, partial);
+}
+bool Operator::apply_batched_dot(float const* X, float* Y, int ncols, int64_t ld, double* partial) {
+    return apply_batched_dot_impl(X, Y, ncols, ld, partial);
+}
+
 void Operator::material_gradient(double const* u, double const* psi, double* g_lam, double* g_mu) {
     grad_dispatch(u, psi, g_lam, g_mu);
 }

@@ -190,6 +190,10 @@ public:
     // multi-vector apply: columns of length local_size() with leading dimension ld
     void apply_batched(double const* X, double* Y, int ncols, int64_t ld);
     void apply_batched(float const* X, float* Y, int ncols, int64_t ld);
+    // The same with the per-column partial sums of x . y (partial[c kDotParts + i], i < kDotParts)
+    // from the same kernel, where the assembled register kernel runs; false (nothing done) otherwise.
+    bool apply_batched_dot(double const* X, double* Y, int ncols, int64_t ld, double* partial);
+    bool apply_batched_dot(float const* X, float* Y, int ncols, int64_t ld, double* partial);
 
     // Right-hand side, FP64, from the force at the volume points (E x Q x Nq) and facet data at the
     // facet points (F x Q x nq): Dirichlet values on Dirichlet facets, slip on Fault facets (half
@@ -261,6 +265,7 @@ private:
     template <class T> void launch_mfma(DeviceMfma<T> const& d, T const* x, T* y, int ncols, int64_t ld);
     template <class T> void dispatch(T const* x, T* y, int ncols, int64_t ld);
     template <class T> AssembledArgs<T> assembled_args() const;
+    template <class T> bool apply_batched_dot_impl(T const* X, T* Y, int ncols, int64_t ld, double* partial);
 
     int kind_;
     unsigned D_, N_, Q_, Nbf_, Nq_, nq_;

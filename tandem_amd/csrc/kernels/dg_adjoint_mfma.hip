@@ -459,6 +459,7 @@ hipError_t launch_material_gradient_mfma(int dim, int degree, MfmaArgs<T> const&
     } else if (dim == 2) {
         switch (degree) {
         case 2: return run<2, 2, T>(a, g, u, psi, g_lam, g_mu, s);
+        case 3: return run<2, 3, T>(a, g, u, psi, g_lam, g_mu, s);
         case 4: return run<2, 4, T>(a, g, u, psi, g_lam, g_mu, s);
         case 6: return run<2, 6, T>(a, g, u, psi, g_lam, g_mu, s);
         }

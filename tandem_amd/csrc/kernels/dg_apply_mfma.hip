@@ -527,6 +527,7 @@ hipError_t launch_elasticity_mfma(int dim, int degree, MfmaArgs<T> const& a, T c
     } else if (dim == 2) {
         switch (degree) {
         case 2: return run<2, 2, T>(a, x, y, ncols, ld, s);
+        case 3: return run<2, 3, T>(a, x, y, ncols, ld, s);
         case 4: return run<2, 4, T>(a, x, y, ncols, ld, s);
         case 6: return run<2, 6, T>(a, x, y, ncols, ld, s);
         }
@@ -535,7 +536,7 @@ hipError_t launch_elasticity_mfma(int dim, int degree, MfmaArgs<T> const& a, T c
 }
 
 bool mfma_supported(int dim, int degree) {
-    return (dim == 3 && degree >= 2 && degree <= 4) || (dim == 2 && (degree == 2 || degree == 4 || degree == 6));
+    return (dim == 3 && degree >= 2 && degree <= 4) || (dim == 2 && (degree == 2 || degree == 3 || degree == 4 || degree == 6));
 }
 
 template hipError_t launch_elasticity_mfma<double>(int, int, MfmaArgs<double> const&, double const*, double*, int,

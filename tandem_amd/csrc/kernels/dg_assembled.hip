@@ -28,74 +28,103 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / 64;
 
-template <class T, int BS, int NB, int R>
+// DOT: the workgroups stride over the element blocks (gridDim.x = kDotParts, column ranges of at
+// most kDotCols) and write the per-workgroup partial sums of x . y of every column to
+// partial[c kDotParts + workgroup] (the solvers' two-stage reduction layout).
+constexpr int kDotCols = 16;
+template <class T, int BS, int NB, int R, bool DOT>
 __global__ __launch_bounds__(kThreads) void assembled_apply_kernel(AssembledArgs<T> a, T const* __restrict__ xs,
                                                                    T* __restrict__ ys, int64_t ld, int ncols,
-                                                                   int cchunk) {
+                                                                   int cchunk, double* __restrict__ partial) {
     constexpr int LPE = BS / R;   // lanes per element
     constexpr int EPW = 64 / LPE; // elements per wave
-    const int lane = threadIdx.x & 63;
+    constexpr int EPB = EPW * kWavesPerBlock;
+    __shared__ double red[kWavesPerBlock][kDotCols];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int el = lane / LPE, part = lane - el * LPE;
-    const int64_t e = (int64_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6)) * EPW + el;
-    const bool act = el < EPW && e < a.num_elements;
-    const int64_t ec = act ? e : 0; // idle lanes read element 0 and store nothing
     const int64_t E = a.num_elements;
-    T m[NB][R][BS];
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (int j = 0; j < BS; ++j) m[b][r][j] = a.A[((int64_t(b) * BS + part * R + r) * BS + j) * E + ec];
-    int64_t off[NB];
-    off[0] = ec * BS;
-#pragma unroll
-    for (int f = 0; f + 1 < NB; ++f) off[1 + f] = int64_t(a.nb[int64_t(f) * E + ec]) * BS;
     const int c0 = blockIdx.y * cchunk;
     const int c1 = min(ncols, c0 + cchunk);
-    // small block rows hold the next column's x blocks in flight during this column's products
-    constexpr bool PRE = NB * BS * int(sizeof(T) / 4) <= 48;
-    T v[NB][BS];
-    if (PRE && c0 < c1) {
-        T const* x = xs + int64_t(c0) * ld;
+    if (DOT && lane == 0)
+        for (int k = 0; k < kDotCols; ++k) red[wave][k] = 0.0;
+    auto block = [&](int64_t blk) {
+        const int64_t e = blk * EPB + int64_t(wave) * EPW + el;
+        const bool act = el < EPW && e < E;
+        const int64_t ec = act ? e : 0; // idle lanes read element 0 and store nothing
+        T m[NB][R][BS];
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int j = 0; j < BS; ++j) v[b][j] = x[off[b] + j];
-    }
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int j = 0; j < BS; ++j) m[b][r][j] = a.A[((int64_t(b) * BS + part * R + r) * BS + j) * E + ec];
+        int64_t off[NB];
+        off[0] = ec * BS;
+#pragma unroll
+        for (int f = 0; f + 1 < NB; ++f) off[1 + f] = int64_t(a.nb[int64_t(f) * E + ec]) * BS;
+        // small block rows hold the next column's x blocks in flight during this column's products
+        constexpr bool PRE = NB * BS * int(sizeof(T) / 4) <= 48;
+        T v[NB][BS];
+        if (PRE && c0 < c1) {
+            T const* x = xs + int64_t(c0) * ld;
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int j = 0; j < BS; ++j) v[b][j] = x[off[b] + j];
+        }
 #pragma unroll 1
-    for (int c = c0; c < c1; ++c) {
-        T w[NB][BS];
-        const int cn = PRE ? min(c + 1, c1 - 1) : c;
-        T const* xn = xs + int64_t(cn) * ld;
+        for (int c = c0; c < c1; ++c) {
+            T w[NB][BS];
+            const int cn = PRE ? min(c + 1, c1 - 1) : c;
+            T const* xn = xs + int64_t(cn) * ld;
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+            for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int j = 0; j < BS; ++j) w[b][j] = xn[off[b] + j];
-        if constexpr (!PRE) {
+                for (int j = 0; j < BS; ++j) w[b][j] = xn[off[b] + j];
+            if constexpr (!PRE) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int j = 0; j < BS; ++j) v[b][j] = w[b][j];
+            }
+            T acc[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = T(0);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int j = 0; j < BS; ++j)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) acc[r] += m[b][r][j] * v[b][j];
+            if (act) {
+                T* y = ys + int64_t(c) * ld + ec * BS + part * R;
+#pragma unroll
+                for (int r = 0; r < R; ++r) y[r] = acc[r];
+            }
+            if constexpr (DOT) {
+                double d = 0.0;
+                if (act) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) d += double(v[0][r]) * double(acc[r]); // R == BS: part 0
+                }
+                for (int o = 32; o > 0; o >>= 1) d += __shfl_down(d, o, 64);
+                if (lane == 0) red[wave][c - c0] += d;
+            }
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int j = 0; j < BS; ++j) v[b][j] = w[b][j];
         }
-        T acc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = T(0);
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int j = 0; j < BS; ++j)
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] += m[b][r][j] * v[b][j];
-        if (act) {
-            T* y = ys + int64_t(c) * ld + ec * BS + part * R;
-#pragma unroll
-            for (int r = 0; r < R; ++r) y[r] = acc[r];
-        }
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int j = 0; j < BS; ++j) v[b][j] = w[b][j];
+    };
+    if constexpr (DOT) {
+        const int64_t nblk = (E + EPB - 1) / EPB;
+        for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) block(blk);
+        __syncthreads();
+        if (threadIdx.x < c1 - c0)
+            partial[int64_t(c0 + threadIdx.x) * kDotParts + blockIdx.x] =
+                ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+    } else {
+        block(blockIdx.x);
     }
 }
 
@@ -120,16 +149,28 @@ __global__ __launch_bounds__(kThreads) void assembled_apply_generic(AssembledArg
 }
 
 template <class T, int BS, int NB, int R>
-hipError_t run_reg(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t s) {
+hipError_t run_reg(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, double* partial,
+                   hipStream_t s) {
     constexpr int EPB = (64 / (BS / R)) * kWavesPerBlock;
+    if (partial) {
+        // one element per lane only: with several lanes per element the fused sums cost registers
+        if constexpr (R == BS) {
+            const int chunks = (ncols + kDotCols - 1) / kDotCols;
+            hipLaunchKernelGGL((assembled_apply_kernel<T, BS, NB, R, true>), dim3(kDotParts, unsigned(chunks)),
+                               dim3(kThreads), 0, s, a, x, y, ld, ncols, kDotCols, partial);
+            return hipGetLastError();
+        } else {
+            return hipErrorNotSupported;
+        }
+    }
     const int64_t gx = (a.num_elements + EPB - 1) / EPB;
     // column ranges: enough waves to fill the chip (about 4 per SIMD), each range re-reads the matrix
     const int64_t waves = gx * kWavesPerBlock;
     int chunks = int(std::min<int64_t>(ncols, std::max<int64_t>(1, (4096 + waves - 1) / waves)));
     const int cchunk = (ncols + chunks - 1) / chunks;
     chunks = (ncols + cchunk - 1) / cchunk;
-    hipLaunchKernelGGL((assembled_apply_kernel<T, BS, NB, R>), dim3(unsigned(gx), unsigned(chunks)), dim3(kThreads),
-                       0, s, a, x, y, ld, ncols, cchunk);
+    hipLaunchKernelGGL((assembled_apply_kernel<T, BS, NB, R, false>), dim3(unsigned(gx), unsigned(chunks)),
+                       dim3(kThreads), 0, s, a, x, y, ld, ncols, cchunk, static_cast<double*>(nullptr));
     return hipGetLastError();
 }
 
@@ -144,8 +185,9 @@ template <class T, int BS, int NB> constexpr int rows_per_lane() {
 }
 
 template <class T, int BS, int NB>
-hipError_t run_bs(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t s) {
-    return run_reg<T, BS, NB, rows_per_lane<T, BS, NB>()>(a, x, y, ncols, ld, s);
+hipError_t run_bs(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, double* partial,
+                  hipStream_t s) {
+    return run_reg<T, BS, NB, rows_per_lane<T, BS, NB>()>(a, x, y, ncols, ld, partial, s);
 }
 
 } // namespace
@@ -156,18 +198,21 @@ bool assembled_register_kernel(int block_size, int slots) {
 }
 
 template <class T>
-hipError_t launch_assembled(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t s) {
-    if (a.num_elements <= 0 || ncols <= 0) return hipSuccess;
+hipError_t launch_assembled(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t s,
+                            double* partial) {
+    if (a.num_elements <= 0 || ncols <= 0) return partial ? hipErrorInvalidValue : hipSuccess;
     if (ncols > 1 && ld < a.num_elements * a.block_size) return hipErrorInvalidValue;
+    if (ncols > 65535 * kDotCols) return hipErrorInvalidValue;
     const int bs = a.block_size;
     if (a.slots == 4) {
-        if (bs == 3) return run_bs<T, 3, 4>(a, x, y, ncols, ld, s);
-        if (bs == 6) return run_bs<T, 6, 4>(a, x, y, ncols, ld, s);
-        if (bs == 12) return run_bs<T, 12, 4>(a, x, y, ncols, ld, s);
+        if (bs == 3) return run_bs<T, 3, 4>(a, x, y, ncols, ld, partial, s);
+        if (bs == 6) return run_bs<T, 6, 4>(a, x, y, ncols, ld, partial, s);
+        if (bs == 12) return run_bs<T, 12, 4>(a, x, y, ncols, ld, partial, s);
     } else if (a.slots == 5) {
-        if (bs == 4) return run_bs<T, 4, 5>(a, x, y, ncols, ld, s);
-        if (bs == 12) return run_bs<T, 12, 5>(a, x, y, ncols, ld, s);
+        if (bs == 4) return run_bs<T, 4, 5>(a, x, y, ncols, ld, partial, s);
+        if (bs == 12) return run_bs<T, 12, 5>(a, x, y, ncols, ld, partial, s);
     }
+    if (partial) return hipErrorNotSupported; // the generic kernel computes no partial sums
     const int64_t rows = a.num_elements * bs;
     hipLaunchKernelGGL((assembled_apply_generic<T>), dim3(unsigned((rows + kThreads - 1) / kThreads), unsigned(ncols)),
                        dim3(kThreads), 0, s, a, x, y, ld);
@@ -175,9 +220,9 @@ hipError_t launch_assembled(AssembledArgs<T> const& a, T const* x, T* y, int nco
 }
 
 template hipError_t launch_assembled<double>(AssembledArgs<double> const&, double const*, double*, int, int64_t,
-                                             hipStream_t);
+                                             hipStream_t, double*);
 template hipError_t launch_assembled<float>(AssembledArgs<float> const&, float const*, float*, int, int64_t,
-                                            hipStream_t);
+                                            hipStream_t, double*);
 
 // ------------------------------------------------------------------------------ assembly by probing
 namespace {

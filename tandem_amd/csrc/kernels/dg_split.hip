@@ -11,6 +11,7 @@ TDG_SPLIT_DECL(3, 2)
 TDG_SPLIT_DECL(3, 3)
 TDG_SPLIT_DECL(3, 4)
 TDG_SPLIT_DECL(2, 2)
+TDG_SPLIT_DECL(2, 3)
 TDG_SPLIT_DECL(2, 4)
 TDG_SPLIT_DECL(2, 6)
 #undef TDG_SPLIT_DECL
@@ -28,6 +29,7 @@ hipError_t launch_elasticity_split(int dim, int degree, int shape, SplitArgs<T> 
     } else if (dim == 2) {
         switch (degree) {
         case 2: return launch_split_d2p2<T>(shape, a, x, y, L, ncols, ld, ldL, s);
+        case 3: return launch_split_d2p3<T>(shape, a, x, y, L, ncols, ld, ldL, s);
         case 4: return launch_split_d2p4<T>(shape, a, x, y, L, ncols, ld, ldL, s);
         case 6: return launch_split_d2p6<T>(shape, a, x, y, L, ncols, ld, ldL, s);
         }

@@ -78,9 +78,14 @@ template <class T> struct AssembledArgs {
     int block_size, slots;
 };
 // y_c = A x_c for ncols columns at stride ld (the register kernel for the block sizes of
-// assembled_register_kernel, the generic kernel otherwise).
+// assembled_register_kernel, the generic kernel otherwise).  With partial: also the per-workgroup
+// partial sums of x_c . y_c in partial[c kDotParts + i], i < kDotParts, where the register kernel
+// holds one element per lane (block sizes 3, 4, 6 in FP32, 3 in FP64); hipErrorNotSupported (and
+// nothing launched) otherwise.
+constexpr int kDotParts = 256;
 template <class T>
-hipError_t launch_assembled(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t stream);
+hipError_t launch_assembled(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, hipStream_t stream,
+                            double* partial = nullptr);
 bool assembled_register_kernel(int block_size, int slots);
 // Assembly by probing (FP64): X (m columns of E bs) = unit vectors c0 .. c0 + m - 1 on the elements of
 // colour c; gather stores the isolated block columns of Y = A X into the assembled layout.

@@ -9,6 +9,8 @@
 // PCBJACOBI with one block per element uses.
 #pragma once
 
+#include "../kernels/kernels.h"
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -17,8 +19,6 @@
 namespace tdg {
 
 class Operator;
-
-constexpr int kDotParts = 256; // partial sums per column of the fused block application and dot
 
 class BlockJacobi {
 public:

@@ -63,7 +63,16 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
     MultiWork& W = w ? *w : local;
     DBuf<double>& S = W.S;
     DBuf<double>& partial = W.partial;
-    static_assert(kDotParts == kG, "the fused block application writes the solver's partial-sum layout");
+    static_assert(kDotParts == kG, "the fused kernels write the solver's partial-sum layout");
+    // q = A p and the partial sums of p . q: one kernel on the assembled apply
+    auto applyA_dot = [&](auto const* P, auto* Q) {
+        if (segments || !op.apply_batched_dot(P, Q, ncols, ld, partial.p)) {
+            applyA(P, Q);
+            hipLaunchKernelGGL((mdot<std::remove_const_t<std::remove_pointer_t<decltype(P)>>,
+                                     std::remove_pointer_t<decltype(Q)>>),
+                               dim3(kG, ncols), dim3(kB), 0, st, P, Q, L, partial.p);
+        }
+    };
     // z = M^-1 r and the partial sums of r . z: one fused kernel for small blocks
     auto applyM_dot = [&](auto const* R, auto* Z) {
         if (segments || !M->apply_dot(R, Z, st, ncols, ld, partial.p)) {
@@ -132,8 +141,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, p.p, M ? z.p : r.p, L);
         MHIP(hipGetLastError());
         auto iteration = [&](int c) {
-            applyA(p.p, q.p);
-            hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, p.p, q.p, L, partial.p);
+            applyA_dot(p.p, q.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 2, -1);
             hipLaunchKernelGGL((mxr<double, double>), g, bl, 0, st, X, r.p, p.p, q.p, L, S.p, c, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
@@ -215,8 +223,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         // first half of an iteration: q = A p, step, residual update; second half: preconditioner
         // and new direction (after the host check of the iteration, which may replace residuals)
         auto half1 = [&](int c) {
-            applyA(p.p, q.p);
-            hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, p.p, q.p, L, partial.p);
+            applyA_dot(p.p, q.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 2, -1);
             hipLaunchKernelGGL((mxr<float, float>), g, bl, 0, st, xa.p, r32.p, p.p, q.p, L, S.p, c, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);

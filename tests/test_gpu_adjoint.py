@@ -11,7 +11,7 @@ from cases import Case, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(3, 3, 1, 4, [5] * 6), (3, 2, 0, 2, [5, 1, 1, 5, 3, 1]), (2, 3, 0, 6, [5, 1, 3, 5]), (2, 5, 0, 2, [5] * 4)]
+CASES = [(3, 3, 1, 4, [5] * 6), (3, 2, 0, 2, [5, 1, 1, 5, 3, 1]), (2, 3, 0, 6, [5, 1, 3, 5]), (2, 5, 0, 2, [5] * 4), (2, 4, 0, 3, [5, 1, 5, 5])]
 
 
 @pytest.mark.parametrize("variant", ["auto", "mfma-xcd", "points"])

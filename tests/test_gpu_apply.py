@@ -18,6 +18,7 @@ CASES = [
     (3, 2, 1, 2, 0, BC_D),
     (3, 2, 1, 3, 0, BC_D),
     (2, 4, 0, 2, 0, BC_D),
+    (2, 4, 0, 3, 0, [BC_N, BC_D, BC_D, BC_D]),  # P3 triangles (the middle p-multigrid level)
     (2, 3, 0, 6, 0, [BC_D, BC_N, BC_D, BC_N]),  # P6 triangles (the Green's-function element type)
     (2, 32, 0, 2, 1, BC_D),         # Poisson P2 on the 32x32x2 unit square
     (2, 3, 0, 4, 1, [BC_D, BC_N, BC_N, BC_D]),
@@ -34,7 +35,7 @@ def _ids(c):
 @pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
 def test_apply_matches_oracle(cuda, case, variant):
     """variant "auto" runs the two-kernel matrix-core apply (facet kernel + element kernel) where
-    one is compiled (elasticity 3D P2-P4, 2D P2/P4/P6), "points" the thread-per-point kernel,
+    one is compiled (elasticity 3D P2-P4, 2D P2-P4 and P6), "points" the thread-per-point kernel,
     "mfma-element" the element-centric matrix-core kernel and "mfma-xcd" the same with its
     workgroups renumbered per XCD."""
     import torch
