@@ -318,25 +318,28 @@ def run_greens(args):
     out = {"workload": "2D P6 plane strain, %d elements, %d DoF, %d slip right-hand sides, blocks of %d"
                        % (op.E, op.n, total_cols, nb),
            "autotune_fp32_64col_ms": ms, "autotune_choice": variant, "iterations_budget": args.iters}
-    # batched apply vs column-by-column apply (FP32)
-    Xf = torch.randn((nb, ld), dtype=torch.float32, device="cuda")
-    Yf = torch.empty_like(Xf)
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    op.apply_batched(Xf, Yf)
-    ev[0].record()
-    for _ in range(5):
+    budget = not args.greens_pmg_only
+    if budget:
+        # batched apply vs column-by-column apply (FP32)
+        Xf = torch.randn((nb, ld), dtype=torch.float32, device="cuda")
+        Yf = torch.empty_like(Xf)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         op.apply_batched(Xf, Yf)
-    ev[1].record()
-    for _ in range(5):
-        for c in range(nb):
-            op.apply(Xf[c], Yf[c])
-    ev[2].record()
-    torch.cuda.synchronize()
-    t_b, t_s = ev[0].elapsed_time(ev[1]) / 5, ev[1].elapsed_time(ev[2]) / 5
-    out["apply_fp32_64col_batched_ms"] = t_b
-    out["apply_fp32_64col_one_by_one_ms"] = t_s
-    out["apply_batching_speedup"] = t_s / t_b
+        ev[0].record()
+        for _ in range(5):
+            op.apply_batched(Xf, Yf)
+        ev[1].record()
+        for _ in range(5):
+            for c in range(nb):
+                op.apply(Xf[c], Yf[c])
+        ev[2].record()
+        torch.cuda.synchronize()
+        t_b, t_s = ev[0].elapsed_time(ev[1]) / 5, ev[1].elapsed_time(ev[2]) / 5
+        out["apply_fp32_64col_batched_ms"] = t_b
+        out["apply_fp32_64col_one_by_one_ms"] = t_s
+        out["apply_batching_speedup"] = t_s / t_b
+        del Xf, Yf
     # the slip basis: unit fault states in the fault adapter's basis (nodal P6 per fault facet,
     # along-fault component; up = +y, fault normal oriented to the hanging wall)
     dip = np.radians(30.0)
@@ -350,6 +353,19 @@ def run_greens(args):
     t1 = time.perf_counter()
     fdat = op.fault_slip(S)
     op.rhs_device(B, None, fdat)
+    G = None
+    if budget:
+        G = greens_budget_runs(op, B, Xs, fdat, nb, args, out, t1)
+    if args.pmg_levels != "none":
+        greens_pmg(op, mesh, lf, mf, B, fdat, G, args, out)
+    return out
+
+
+def greens_budget_runs(op, B, Xs, fdat, nb, args, out, t1):
+    """The fixed-budget lockstep block-Jacobi CG of the 64-column block and of one column alone;
+    returns the Green's-function columns of the lockstep solutions."""
+    import torch
+
     r = op.solve_multi(B, Xs, rtol=1e-10, mode="fp32-ru", precond="block-jacobi", max_iter=args.iters)
     torch.cuda.synchronize()
     t_lock = time.perf_counter() - t1
@@ -382,47 +398,52 @@ def run_greens(args):
         "single_rhs_iterations_per_second"]
     xs = Xs[0].cpu().numpy()
     out["column0_rel_diff_lockstep_vs_single"] = float(np.linalg.norm(xs - x.cpu().numpy()) / np.linalg.norm(xs))
-    if args.pmg_levels != "none":
-        # the same block of 64 right-hand sides solved to 1e-10 by flexible CG with a p-multigrid
-        # V-cycle (coarse levels on the same mesh and material)
-        degs = [int(v) for v in args.pmg_levels.split(",")]
-        if degs[0] != op.degree:
-            raise ValueError("--pmg-levels must start at the fine degree %d" % op.degree)
-        del Xf, Yf
-        t0 = time.perf_counter()
-        coarse = []
-        for p in degs[1:]:
-            c = on_stream(T.DGOperator(mesh, T.ELASTICITY, p))
-            Xc = c.volume_points()
-            c.set_material(lf(Xc), mf(Xc))
-            coarse.append(c)
-        M = T.PMultigrid([op] + coarse, smooth=args.pmg_smooth, coarse_max_iter=args.pmg_coarse_max_iter,
-                         galerkin_penalty=args.pmg_galerkin)
-        lmax = M.setup()
-        torch.cuda.synchronize()
-        t_setup = time.perf_counter() - t0
-        log("greens p-multigrid %s: setup %.1fs, lambda_max %s" % (degs, t_setup, lmax))
-        Xp = torch.zeros_like(B)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        rp = M.solve_multi(B, Xp, rtol=1e-10, max_iter=args.pmg_max_iter)
-        torch.cuda.synchronize()
-        t_p = time.perf_counter() - t1
-        log("greens p-multigrid solve %.2fs" % t_p, {k: rp[k] for k in ("iterations", "outer_iterations",
-                                                                          "rel_residual", "converged")})
-        Gp = op.fault_traction(Xp, fdat)
-        torch.cuda.synchronize()
-        out["pmg"] = {
-            "levels": degs, "smooth": M.options["smooth"], "coarse_rtol": M.options["coarse_rtol"],
-            "coarse_max_iter": M.options["coarse_max_iter"], "galerkin_penalty": M.options["galerkin_penalty"],
-            "eig_interval": [M.options["eig_lo"], M.options["eig_hi"]], "lambda_max": [float(v) for v in lmax],
-            "setup_seconds": t_setup, "seconds_to_1e-10": t_p, "iterations": int(rp["iterations"]),
-            "coarse_cg_iterations": int(rp["outer_iterations"]), "converged": int(rp["converged"]),
-            "worst_rel_residual": float(rp["rel_residual"]),
-            "max_column_iterations": int(np.max(rp["column_iterations"])),
-            "green_columns_rel_diff_vs_lockstep": float(torch.linalg.norm(Gp - G) / torch.linalg.norm(Gp)),
-        }
-    return out
+    return G
+
+
+def greens_pmg(op, mesh, lf, mf, B, fdat, G, args, out):
+    """The block of 64 right-hand sides solved to 1e-10 by flexible CG with a p-multigrid V-cycle
+    (coarse levels on the same mesh and material)."""
+    import torch
+    import tandem_amd as T
+
+    degs = [int(v) for v in args.pmg_levels.split(",")]
+    if degs[0] != op.degree:
+        raise ValueError("--pmg-levels must start at the fine degree %d" % op.degree)
+    t0 = time.perf_counter()
+    coarse = []
+    for p in degs[1:]:
+        c = on_stream(T.DGOperator(mesh, T.ELASTICITY, p))
+        Xc = c.volume_points()
+        c.set_material(lf(Xc), mf(Xc))
+        coarse.append(c)
+    M = T.PMultigrid([op] + coarse, smooth=args.pmg_smooth, coarse_max_iter=args.pmg_coarse_max_iter,
+                     galerkin_penalty=args.pmg_galerkin)
+    lmax = M.setup()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    log("greens p-multigrid %s: setup %.1fs, lambda_max %s" % (degs, t_setup, lmax))
+    Xp = torch.zeros_like(B)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rp = M.solve_multi(B, Xp, rtol=1e-10, max_iter=args.pmg_max_iter)
+    torch.cuda.synchronize()
+    t_p = time.perf_counter() - t1
+    log("greens p-multigrid solve %.2fs" % t_p, {k: rp[k] for k in ("iterations", "outer_iterations",
+                                                                      "rel_residual", "converged")})
+    Gp = op.fault_traction(Xp, fdat)
+    torch.cuda.synchronize()
+    out["pmg"] = {
+        "levels": degs, "smooth": M.options["smooth"], "coarse_rtol": M.options["coarse_rtol"],
+        "coarse_max_iter": M.options["coarse_max_iter"], "galerkin_penalty": M.options["galerkin_penalty"],
+        "eig_interval": [M.options["eig_lo"], M.options["eig_hi"]], "lambda_max": [float(v) for v in lmax],
+        "setup_seconds": t_setup, "seconds_to_1e-10": t_p, "iterations": int(rp["iterations"]),
+        "coarse_cg_iterations": int(rp["outer_iterations"]), "converged": int(rp["converged"]),
+        "worst_rel_residual": float(rp["rel_residual"]),
+        "max_column_iterations": int(np.max(rp["column_iterations"])),
+    }
+    if G is not None:
+        out["pmg"]["green_columns_rel_diff_vs_lockstep"] = float(torch.linalg.norm(Gp - G) / torch.linalg.norm(Gp))
 
 
 class SurfaceSampler:
@@ -784,6 +805,8 @@ def main():
     ap.add_argument("--pmg-galerkin", type=int, default=0,
                     help="greens: 1 = coarse levels take the fine facet penalty, 0 = their own")
     ap.add_argument("--pmg-max-iter", type=int, default=300, help="greens: flexible-CG iteration cap")
+    ap.add_argument("--greens-pmg-only", action="store_true",
+                    help="greens: only the p-multigrid solve of the block (no fixed-budget runs)")
     ap.add_argument("--meshes", type=int, default=32, help="multimesh: number of independent meshes")
     ap.add_argument("--precond", choices=["block-jacobi", "none"], default="block-jacobi",
                     help="precision: preconditioner of both solves")
