@@ -29,7 +29,7 @@ public:
 
     // Probes, inverts and stores the blocks (FP64 and FP32 copies) on op's stream.
     void build(Operator& op);
-    bool built() const { return inv64_ != nullptr; }
+    bool built() const { return bs_ > 0; }
     int block_size() const { return bs_; }
     int colours() const { return colours_; }
 
@@ -53,9 +53,10 @@ public:
 private:
     void release();
     int bs_ = 0, colours_ = 0;
-    int64_t E_ = 0;
+    int64_t E_ = 0, tp_ = 0;
     double* inv64_ = nullptr; // [E][bs][bs] column-major blocks
-    float* inv32_ = nullptr;
+    float* inv32_ = nullptr;  //   FP32 copies (blocks of at most 12 rows)
+    float* p32_ = nullptr;    // [E][tp] FP32 packed lower triangles (larger blocks: exactly symmetric)
 };
 
 // The preconditioner as a solver applies it: FP32 vectors always use the FP32 blocks; FP64 vectors

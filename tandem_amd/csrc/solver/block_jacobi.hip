@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace tdg {
 
@@ -72,13 +73,70 @@ __global__ __launch_bounds__(kThreads) void invert_blocks(double* __restrict__ D
     }
     // symmetrise the inverse (Gauss-Jordan leaves rounding-level asymmetry): CG sees an exactly
     // symmetric preconditioner
-    float* g32 = inv32 + e * int64_t(bs) * bs;
+    float* g32 = inv32 ? inv32 + e * int64_t(bs) * bs : nullptr;
     for (int t = threadIdx.x; t < nn; t += kThreads) {
         const int i = t % bs, j = t / bs;
         const double v = 0.5 * (A[i + j * bs] + A[j + i * bs]);
         g[t] = v;
-        g32[t] = float(v);
+        if (g32) g32[t] = float(v);
     }
+}
+
+// Packed symmetric blocks: the lower triangle column by column, entry (i, j), i >= j, at
+// start(j) + i - j with start(j) = j bs - j (j - 1) / 2, each element's block padded to tp entries
+// (16-byte multiples).  The inverse blocks are exactly symmetric, so this halves the bytes of every
+// application and changes no value.
+__device__ __forceinline__ int packed_start(int j, int bs) { return j * bs - (j * (j - 1)) / 2; }
+__device__ __forceinline__ int packed_index(int i, int j, int bs) {
+    return i >= j ? packed_start(j, bs) + (i - j) : packed_start(i, bs) + (j - i);
+}
+
+__global__ __launch_bounds__(kThreads) void pack_blocks(double const* __restrict__ full, float* __restrict__ p32,
+                                                        int bs, int64_t tp) {
+    const int64_t e = blockIdx.x;
+    double const* g = full + e * int64_t(bs) * bs;
+    const int tri = bs * (bs + 1) / 2;
+    for (int t = threadIdx.x; t < bs * bs; t += kThreads) {
+        const int i = t % bs, j = t / bs;
+        if (i < j) continue;
+        p32[e * tp + packed_start(j, bs) + (i - j)] = float(g[t]);
+    }
+    for (int64_t t = tri + threadIdx.x; t < tp; t += kThreads) p32[e * tp + t] = 0.0f;
+}
+
+// z_e = inv_e r_e from packed blocks, one workgroup of 256 lanes per element: the packed block is
+// copied to LDS with 16-byte loads (coalesced, half the bytes of the full block; four waves keep
+// more loads in flight), then lane i < bs forms row i in the same order as the full-block kernel
+// (bit-identical results).
+template <class T, class TB>
+__global__ __launch_bounds__(256) void apply_blocks_packed(TB const* __restrict__ pk, T const* __restrict__ rs_,
+                                                           T* __restrict__ zs, int bs, int64_t tp, int64_t ld) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    TB* sB = reinterpret_cast<TB*>(lds_raw);
+    T* sr = reinterpret_cast<T*>(lds_raw + tp * sizeof(TB));
+    const int64_t e = blockIdx.x;
+    T const* __restrict__ r = rs_ + int64_t(blockIdx.y) * ld;
+    T* __restrict__ z = zs + int64_t(blockIdx.y) * ld;
+    const int tid = threadIdx.x;
+    if (tid < bs) sr[tid] = r[e * bs + tid];
+    using V = typename std::conditional<sizeof(TB) == 8, double2, float4>::type;
+    constexpr int per = 16 / sizeof(TB);
+    V const* g = reinterpret_cast<V const*>(pk + e * tp);
+    V* d = reinterpret_cast<V*>(sB);
+#pragma unroll 4
+    for (int64_t t = tid; t < tp / per; t += 256) d[t] = g[t];
+    __syncthreads();
+    if (tid >= bs) return;
+    const int i = tid;
+    T acc = T(0);
+    // entry (i, j): (i, j) for j <= i, else (j, i); the packed index steps by bs - j - 1 down
+    // column j while j < i and by 1 along column i afterwards
+    int idx = i;
+    for (int j = 0; j < bs; ++j) {
+        acc += T(sB[idx]) * sr[j];
+        idx += j < i ? bs - j - 1 : 1;
+    }
+    z[e * bs + i] = acc;
 }
 
 // z_e = inv_e r_e, one workgroup per element (column-major blocks: lanes read consecutive rows);
@@ -106,13 +164,19 @@ __global__ __launch_bounds__(128) void apply_blocks(TB const* __restrict__ inv, 
 constexpr int kCols = 16; // columns staged per pass
 template <class T, class TB, int BSMAX>
 __global__ __launch_bounds__(256) void apply_blocks_cols(TB const* __restrict__ inv, T const* __restrict__ rs,
-                                                         T* __restrict__ zs, int bs, int ncols, int64_t ld) {
+                                                         T* __restrict__ zs, int bs, int ncols, int64_t ld,
+                                                         int64_t tp) {
     __shared__ TB B[BSMAX * BSMAX];
     __shared__ T R[kCols * BSMAX];
     const int64_t e = blockIdx.x;
-    TB const* Bg = inv + e * int64_t(bs) * bs;
     const int nn = bs * bs;
-    for (int t = threadIdx.x; t < nn; t += 256) B[t] = Bg[t];
+    if (tp > 0) { // packed symmetric blocks, unpacked into LDS
+        TB const* Bg = inv + e * tp;
+        for (int t = threadIdx.x; t < nn; t += 256) B[t] = Bg[packed_index(t % bs, t / bs, bs)];
+    } else {
+        TB const* Bg = inv + e * int64_t(bs) * bs;
+        for (int t = threadIdx.x; t < nn; t += 256) B[t] = Bg[t];
+    }
     for (int c0 = 0; c0 < ncols; c0 += kCols) {
         const int nc = min(kCols, ncols - c0);
         __syncthreads(); // B is loaded / the previous pass's reads of R are done
@@ -237,11 +301,20 @@ std::vector<int32_t> colour_elements(Operator const& op, int* ncolours) {
 BlockJacobi::~BlockJacobi() { release(); }
 
 void BlockJacobi::release() {
-    if (inv64_) (void)hipFree(inv64_);
-    if (inv32_) (void)hipFree(inv32_);
+    for (void* p : {static_cast<void*>(inv64_), static_cast<void*>(inv32_), static_cast<void*>(p32_)})
+        if (p) (void)hipFree(p);
     inv64_ = nullptr;
     inv32_ = nullptr;
+    p32_ = nullptr;
+    bs_ = 0;
+    tp_ = 0;
 }
+
+// FP32 copies of blocks above kSmallBlock rows are kept packed (lower triangle); the FP64 blocks
+// and the FP32 copies of small blocks (the register kernel reads them) stay whole.  (FP64 packed
+// blocks staged through LDS measured slower than the whole blocks read directly: 45 KB of LDS per
+// element leaves too few workgroups per CU to cover the copy.)
+constexpr int kSmallBlock = 12;
 
 void BlockJacobi::build(Operator& op) {
     if (!op.ready()) throw std::logic_error("set_material() must be called before building the preconditioner");
@@ -251,6 +324,8 @@ void BlockJacobi::build(Operator& op) {
     const int bs = int(op.block_size());
     const int64_t E = int64_t(op.num_elements()), n = int64_t(op.local_size());
     if (bs > 112) throw std::invalid_argument("block size above 112 is not supported by the block inversion");
+    const bool packed = bs > kSmallBlock;
+    const int64_t tp = packed ? (int64_t(bs) * (bs + 1) / 2 + 3) / 4 * 4 : 0;
     int nc = 0;
     std::vector<int32_t> col = colour_elements(op, &nc);
     int32_t* dcol = nullptr;
@@ -260,7 +335,7 @@ void BlockJacobi::build(Operator& op) {
         BHIP(hipMalloc(reinterpret_cast<void**>(&dcol), col.size() * sizeof(int32_t)));
         BHIP(hipMemcpy(dcol, col.data(), col.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         BHIP(hipMalloc(reinterpret_cast<void**>(&inv64_), std::size_t(E) * bs * bs * sizeof(double)));
-        BHIP(hipMalloc(reinterpret_cast<void**>(&inv32_), std::size_t(E) * bs * bs * sizeof(float)));
+        if (!packed) BHIP(hipMalloc(reinterpret_cast<void**>(&inv32_), std::size_t(E) * bs * bs * sizeof(float)));
         BHIP(hipMalloc(reinterpret_cast<void**>(&X), std::size_t(chunk) * n * sizeof(double)));
         BHIP(hipMalloc(reinterpret_cast<void**>(&Y), std::size_t(chunk) * n * sizeof(double)));
         for (int c = 0; c < nc; ++c)
@@ -281,6 +356,11 @@ void BlockJacobi::build(Operator& op) {
         else
             hipLaunchKernelGGL((invert_blocks<112>), dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, inv32_, bs);
         BHIP(hipGetLastError());
+        if (packed) {
+            BHIP(hipMalloc(reinterpret_cast<void**>(&p32_), std::size_t(E) * tp * sizeof(float)));
+            hipLaunchKernelGGL(pack_blocks, dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, p32_, bs, tp);
+            BHIP(hipGetLastError());
+        }
         BHIP(hipStreamSynchronize(st));
     } catch (...) {
         if (dcol) (void)hipFree(dcol);
@@ -293,6 +373,7 @@ void BlockJacobi::build(Operator& op) {
     (void)hipFree(X);
     (void)hipFree(Y);
     bs_ = bs;
+    tp_ = tp;
     E_ = E;
     colours_ = nc;
 }
@@ -310,7 +391,7 @@ void launch_small(TB const* inv, T const* r, T* z, int64_t E, int ncols, int64_t
 template <class T, class TB>
 bool apply_small(TB const* inv, T const* r, T* z, int64_t E, int bs, int ncols, int64_t ld, double* partial,
                  hipStream_t s) {
-    if (ncols < 1 || (ncols > 1 && ld < E * bs)) return false;
+    if (!inv || ncols < 1 || (ncols > 1 && ld < E * bs)) return false;
     switch (bs) {
     case 3: launch_small<T, TB, 3>(inv, r, z, E, ncols, ld, partial, s); return true;
     case 4: launch_small<T, TB, 4>(inv, r, z, E, ncols, ld, partial, s); return true;
@@ -330,51 +411,61 @@ bool apply_small(TB const* inv, T const* r, T* z, int64_t E, int bs, int ncols, 
     return false;
 }
 
+// full: blocks stored whole (bs x bs, column-major); tp > 0: packed lower triangles of tp entries
 template <class T, class TB>
-void launch_apply(TB const* inv, T const* r, T* z, int64_t E, int bs, int ncols, int64_t ld, hipStream_t s) {
-    if (apply_small(inv, r, z, E, bs, ncols, ld, static_cast<double*>(nullptr), s)) return;
+void launch_apply(TB const* inv, int64_t tp, T const* r, T* z, int64_t E, int bs, int ncols, int64_t ld,
+                  hipStream_t s) {
+    if (tp == 0 && apply_small(inv, r, z, E, bs, ncols, ld, static_cast<double*>(nullptr), s)) return;
     if (ncols <= 1) {
-        hipLaunchKernelGGL((apply_blocks<T, TB>), dim3(unsigned(E), 1u), dim3(128), 0, s, inv, r, z, bs, ld);
+        if (tp > 0) {
+            const std::size_t lds = std::size_t(tp) * sizeof(TB) + std::size_t(bs) * sizeof(T);
+            hipLaunchKernelGGL((apply_blocks_packed<T, TB>), dim3(unsigned(E), 1u), dim3(256), lds, s, inv, r, z, bs,
+                               tp, ld);
+        } else {
+            hipLaunchKernelGGL((apply_blocks<T, TB>), dim3(unsigned(E), 1u), dim3(128), 0, s, inv, r, z, bs, ld);
+        }
     } else if (bs <= 64) {
         hipLaunchKernelGGL((apply_blocks_cols<T, TB, 64>), dim3(unsigned(E)), dim3(256), 0, s, inv, r, z, bs, ncols,
-                           ld);
+                           ld, tp);
     } else {
         hipLaunchKernelGGL((apply_blocks_cols<T, TB, 112>), dim3(unsigned(E)), dim3(256), 0, s, inv, r, z, bs, ncols,
-                           ld);
+                           ld, tp);
     }
     BHIP(hipGetLastError());
 }
 } // namespace
 
 void BlockJacobi::apply(double const* r, double* z, hipStream_t s, int ncols, int64_t ld) const {
-    if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
-    launch_apply(inv64_, r, z, E_, bs_, ncols, ld, s);
+    if (!built()) throw std::logic_error("block Jacobi preconditioner not built");
+    launch_apply(inv64_, 0, r, z, E_, bs_, ncols, ld, s);
 }
 
 void BlockJacobi::apply_fp32_blocks(double const* r, double* z, hipStream_t s, int ncols, int64_t ld) const {
-    if (!inv32_) throw std::logic_error("block Jacobi preconditioner not built");
-    launch_apply(inv32_, r, z, E_, bs_, ncols, ld, s);
+    if (!built()) throw std::logic_error("block Jacobi preconditioner not built");
+    if (tp_) launch_apply(p32_, tp_, r, z, E_, bs_, ncols, ld, s);
+    else launch_apply(inv32_, 0, r, z, E_, bs_, ncols, ld, s);
 }
 
 void BlockJacobi::apply(float const* r, float* z, hipStream_t s, int ncols, int64_t ld) const {
-    if (!inv32_) throw std::logic_error("block Jacobi preconditioner not built");
-    launch_apply(inv32_, r, z, E_, bs_, ncols, ld, s);
+    if (!built()) throw std::logic_error("block Jacobi preconditioner not built");
+    if (tp_) launch_apply(p32_, tp_, r, z, E_, bs_, ncols, ld, s);
+    else launch_apply(inv32_, 0, r, z, E_, bs_, ncols, ld, s);
 }
 
 bool BlockJacobi::apply_dot(float const* r, float* z, hipStream_t s, int ncols, int64_t ld, double* partial) const {
-    if (!inv32_) throw std::logic_error("block Jacobi preconditioner not built");
-    return apply_small(inv32_, r, z, E_, bs_, ncols, ld, partial, s);
+    if (!built()) throw std::logic_error("block Jacobi preconditioner not built");
+    return apply_small(inv32_, r, z, E_, bs_, ncols, ld, partial, s); // full (small) blocks only
 }
 
 bool BlockJacobi::apply_dot(double const* r, double* z, hipStream_t s, int ncols, int64_t ld, double* partial,
                             bool fp32_blocks) const {
-    if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
+    if (!built()) throw std::logic_error("block Jacobi preconditioner not built");
     return fp32_blocks ? apply_small(inv32_, r, z, E_, bs_, ncols, ld, partial, s)
                        : apply_small(inv64_, r, z, E_, bs_, ncols, ld, partial, s);
 }
 
 void BlockJacobi::inverse_block(int64_t e, double* out) const {
-    if (!inv64_) throw std::logic_error("block Jacobi preconditioner not built");
+    if (!built()) throw std::logic_error("block Jacobi preconditioner not built");
     if (e < 0 || e >= E_) throw std::out_of_range("element index out of range");
     BHIP(hipMemcpy(out, inv64_ + e * int64_t(bs_) * bs_, std::size_t(bs_) * bs_ * sizeof(double),
                    hipMemcpyDeviceToHost));
