@@ -1,5 +1,6 @@
 // Element-block Jacobi preconditioner of the DG operator: the inverse of every element's diagonal
-// block A_KK (bs x bs), in FP64 and FP32.
+// block A_KK (bs x bs), in FP64 and FP32; blocks above 12 rows are stored as packed lower triangles
+// (the inverses are exactly symmetric), half the bytes per application.
 //
 // The blocks are probed with the matrix-free apply on a colouring of the elements (no two elements
 // sharing a facet have the same colour): with x = e_j on every element of one colour and zero
@@ -54,9 +55,10 @@ private:
     void release();
     int bs_ = 0, colours_ = 0;
     int64_t E_ = 0, tp_ = 0;
-    double* inv64_ = nullptr; // [E][bs][bs] column-major blocks
-    float* inv32_ = nullptr;  //   FP32 copies (blocks of at most 12 rows)
-    float* p32_ = nullptr;    // [E][tp] FP32 packed lower triangles (larger blocks: exactly symmetric)
+    double* inv64_ = nullptr; // [E][bs][bs] column-major blocks of at most 12 rows
+    float* inv32_ = nullptr;  //   and their FP32 copies
+    double* p64_ = nullptr;   // [E][tp] packed lower triangles of larger blocks (exactly symmetric)
+    float* p32_ = nullptr;    //   and their FP32 copies
 };
 
 // The preconditioner as a solver applies it: FP32 vectors always use the FP32 blocks; FP64 vectors

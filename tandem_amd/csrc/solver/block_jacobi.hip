@@ -91,7 +91,8 @@ __device__ __forceinline__ int packed_index(int i, int j, int bs) {
     return i >= j ? packed_start(j, bs) + (i - j) : packed_start(i, bs) + (j - i);
 }
 
-__global__ __launch_bounds__(kThreads) void pack_blocks(double const* __restrict__ full, float* __restrict__ p32,
+template <class TB>
+__global__ __launch_bounds__(kThreads) void pack_blocks(double const* __restrict__ full, TB* __restrict__ pk,
                                                         int bs, int64_t tp) {
     const int64_t e = blockIdx.x;
     double const* g = full + e * int64_t(bs) * bs;
@@ -99,9 +100,9 @@ __global__ __launch_bounds__(kThreads) void pack_blocks(double const* __restrict
     for (int t = threadIdx.x; t < bs * bs; t += kThreads) {
         const int i = t % bs, j = t / bs;
         if (i < j) continue;
-        p32[e * tp + packed_start(j, bs) + (i - j)] = float(g[t]);
+        pk[e * tp + packed_start(j, bs) + (i - j)] = TB(g[t]);
     }
-    for (int64_t t = tri + threadIdx.x; t < tp; t += kThreads) p32[e * tp + t] = 0.0f;
+    for (int64_t t = tri + threadIdx.x; t < tp; t += kThreads) pk[e * tp + t] = TB(0);
 }
 
 // z_e = inv_e r_e from packed blocks, one workgroup of 256 lanes per element: the packed block is
@@ -301,19 +302,19 @@ std::vector<int32_t> colour_elements(Operator const& op, int* ncolours) {
 BlockJacobi::~BlockJacobi() { release(); }
 
 void BlockJacobi::release() {
-    for (void* p : {static_cast<void*>(inv64_), static_cast<void*>(inv32_), static_cast<void*>(p32_)})
+    for (void* p : {static_cast<void*>(inv64_), static_cast<void*>(inv32_), static_cast<void*>(p32_),
+                    static_cast<void*>(p64_)})
         if (p) (void)hipFree(p);
     inv64_ = nullptr;
     inv32_ = nullptr;
     p32_ = nullptr;
+    p64_ = nullptr;
     bs_ = 0;
     tp_ = 0;
 }
 
-// FP32 copies of blocks above kSmallBlock rows are kept packed (lower triangle); the FP64 blocks
-// and the FP32 copies of small blocks (the register kernel reads them) stay whole.  (FP64 packed
-// blocks staged through LDS measured slower than the whole blocks read directly: 45 KB of LDS per
-// element leaves too few workgroups per CU to cover the copy.)
+// Blocks above kSmallBlock rows are kept packed (lower triangle, FP64 and FP32); small blocks stay
+// whole (the register kernel reads them).
 constexpr int kSmallBlock = 12;
 
 void BlockJacobi::build(Operator& op) {
@@ -358,10 +359,16 @@ void BlockJacobi::build(Operator& op) {
         BHIP(hipGetLastError());
         if (packed) {
             BHIP(hipMalloc(reinterpret_cast<void**>(&p32_), std::size_t(E) * tp * sizeof(float)));
-            hipLaunchKernelGGL(pack_blocks, dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, p32_, bs, tp);
+            BHIP(hipMalloc(reinterpret_cast<void**>(&p64_), std::size_t(E) * tp * sizeof(double)));
+            hipLaunchKernelGGL((pack_blocks<float>), dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, p32_, bs, tp);
+            hipLaunchKernelGGL((pack_blocks<double>), dim3(unsigned(E)), dim3(kThreads), 0, st, inv64_, p64_, bs, tp);
             BHIP(hipGetLastError());
         }
         BHIP(hipStreamSynchronize(st));
+        if (packed) {
+            (void)hipFree(inv64_);
+            inv64_ = nullptr;
+        }
     } catch (...) {
         if (dcol) (void)hipFree(dcol);
         if (X) (void)hipFree(X);
@@ -437,7 +444,8 @@ void launch_apply(TB const* inv, int64_t tp, T const* r, T* z, int64_t E, int bs
 
 void BlockJacobi::apply(double const* r, double* z, hipStream_t s, int ncols, int64_t ld) const {
     if (!built()) throw std::logic_error("block Jacobi preconditioner not built");
-    launch_apply(inv64_, 0, r, z, E_, bs_, ncols, ld, s);
+    if (tp_) launch_apply(p64_, tp_, r, z, E_, bs_, ncols, ld, s);
+    else launch_apply(inv64_, 0, r, z, E_, bs_, ncols, ld, s);
 }
 
 void BlockJacobi::apply_fp32_blocks(double const* r, double* z, hipStream_t s, int ncols, int64_t ld) const {
@@ -467,8 +475,20 @@ bool BlockJacobi::apply_dot(double const* r, double* z, hipStream_t s, int ncols
 void BlockJacobi::inverse_block(int64_t e, double* out) const {
     if (!built()) throw std::logic_error("block Jacobi preconditioner not built");
     if (e < 0 || e >= E_) throw std::out_of_range("element index out of range");
-    BHIP(hipMemcpy(out, inv64_ + e * int64_t(bs_) * bs_, std::size_t(bs_) * bs_ * sizeof(double),
-                   hipMemcpyDeviceToHost));
+    const int bs = bs_;
+    if (!tp_) {
+        BHIP(hipMemcpy(out, inv64_ + e * int64_t(bs) * bs, std::size_t(bs) * bs * sizeof(double),
+                       hipMemcpyDeviceToHost));
+        return;
+    }
+    std::vector<double> pk(static_cast<std::size_t>(tp_));
+    BHIP(hipMemcpy(pk.data(), p64_ + e * tp_, std::size_t(tp_) * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < bs; ++j)
+        for (int i = j; i < bs; ++i) {
+            const double v = pk[std::size_t(j * bs - (j * (j - 1)) / 2 + (i - j))];
+            out[i + j * bs] = v;
+            out[j + i * bs] = v;
+        }
 }
 
 } // namespace tdg
