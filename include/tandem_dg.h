@@ -270,7 +270,11 @@ int tdg_assembled_block(tdg_op* op, int64_t element, int slot, double* block);
  * transfers) runs in FP32 on the FP32 copies of the operators; the outer CG stays FP64.  With
  * coarse_assembled the coarsest operator is switched to TDG_VARIANT_ASSEMBLED at set-up where its
  * block size has the register kernel (P1 and P2 levels): the coarse CG then reads the assembled
- * block rows once per range of columns instead of the per-point geometry once per column.
+ * block rows once per range of columns instead of the per-point geometry once per column.  With
+ * coarse_amg as well (elasticity, coarse_fp32) the coarse CG is preconditioned by a smoothed-
+ * aggregation AMG built on those block rows (the reference's coarse preconditioner is PETSc's GAMG,
+ * examples/options/mg_cheby.cfg) instead of block Jacobi: rigid-body near-null space, Chebyshev
+ * smoothing, Galerkin coarse operators down to a dense solve.
  * tdg_pmg_create keeps the level pointers (the operators must outlive the hierarchy);
  * tdg_pmg_setup builds the block-Jacobi inverses and the eigenvalue estimates (lambda_max: host
  * array of nlevels-1, may be NULL); tdg_pmg_vcycle applies one V-cycle z = M^-1 r to ncols device
@@ -289,6 +293,7 @@ typedef struct {
     int coarse_fp32;
     int vcycle_fp32;
     int coarse_assembled;
+    int coarse_amg;
 } tdg_pmg_opts;
 void tdg_pmg_default_opts(tdg_pmg_opts* opts);
 int tdg_pmg_create(tdg_op* const* levels, int nlevels, tdg_pmg_opts const* opts, tdg_pmg** out);

@@ -36,7 +36,8 @@ class SolveResult(C.Structure):
 class PMGOpts(C.Structure):
     _fields_ = [("smooth", C.c_int), ("eig_lo", C.c_double), ("eig_hi", C.c_double), ("eig_iters", C.c_int),
                 ("coarse_rtol", C.c_double), ("coarse_max_iter", C.c_int), ("galerkin_penalty", C.c_int),
-                ("coarse_fp32", C.c_int), ("vcycle_fp32", C.c_int), ("coarse_assembled", C.c_int)]
+                ("coarse_fp32", C.c_int), ("vcycle_fp32", C.c_int), ("coarse_assembled", C.c_int),
+                ("coarse_amg", C.c_int)]
 
 
 class MeshDesc(C.Structure):

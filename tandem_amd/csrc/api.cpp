@@ -291,6 +291,7 @@ void tdg_pmg_default_opts(tdg_pmg_opts* opts) {
     opts->coarse_fp32 = d.coarse_fp32;
     opts->vcycle_fp32 = d.vcycle_fp32;
     opts->coarse_assembled = d.coarse_assembled;
+    opts->coarse_amg = d.coarse_amg;
 }
 
 int tdg_pmg_create(tdg_op* const* levels, int nlevels, tdg_pmg_opts const* opts, tdg_pmg** out) {
@@ -311,6 +312,7 @@ int tdg_pmg_create(tdg_op* const* levels, int nlevels, tdg_pmg_opts const* opts,
             o.coarse_fp32 = opts->coarse_fp32;
             o.vcycle_fp32 = opts->vcycle_fp32;
             o.coarse_assembled = opts->coarse_assembled;
+            o.coarse_amg = opts->coarse_amg;
         }
         auto h = std::make_unique<tdg_pmg>();
         h->impl = std::make_unique<tdg::PMG>(ops, o);

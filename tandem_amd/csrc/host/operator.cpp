@@ -1356,6 +1356,15 @@ void Operator::assembled_block(int64_t e, int slot, double* out) const {
 
 bool Operator::assembled_fast() const { return assembled_register_kernel(int(block_size()), int(D_ + 2)); }
 
+void Operator::assembled_host(std::vector<double>& A, std::vector<int32_t>& nb) {
+    assemble();
+    TDG_HIP(hipStreamSynchronize(stream_));
+    A.resize(asm_->A64.n);
+    nb.resize(asm_->nb.n);
+    TDG_HIP(hipMemcpy(A.data(), asm_->A64.ptr, A.size() * sizeof(double), hipMemcpyDeviceToHost));
+    TDG_HIP(hipMemcpy(nb.data(), asm_->nb.ptr, nb.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+}
+
 template <class T> AssembledArgs<T> Operator::assembled_args() const {
     AssembledArgs<T> a{};
     if constexpr (std::is_same<T, double>::value) a.A = asm_->A64.ptr;

@@ -218,6 +218,10 @@ public:
     void assembled_block(int64_t e, int slot, double* out) const;
     // Whether the assembled apply has its register kernel for this block size (P1 / P2 levels).
     bool assembled_fast() const;
+    // Host copies of the FP64 block rows (A[slot][row][col][element]) and face neighbours
+    // (nb[face][element]); assembles first if needed.
+    void assembled_host(std::vector<double>& A, std::vector<int32_t>& nb);
+    SimplexMesh const& mesh() const { return mesh_; }
 
     // g_lam, g_mu (E x Nq, device) = d/d(lam_Q, mu_Q) of psi^T A u for device vectors u, psi
     // (elasticity): the adjoint gradient kernel.

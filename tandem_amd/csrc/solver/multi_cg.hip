@@ -34,9 +34,11 @@ namespace tdg {
 // The lockstep recursions over ncols columns laid out by L (tot entries per vector).  Segments:
 // one apply of the whole operator per iteration; columns: one batched apply.
 MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols, Cols L, std::size_t tot,
-                             bool segments, SolveOptions const& o, MultiWork* w) {
+                             bool segments, SolveOptions const& o, MultiWork* w, PrecondFn32 const* pre) {
     if (o.mode != Solve_FP64_CG && o.mode != Solve_FP32_CG_FP64_RELIABLE)
         throw std::invalid_argument("multi-column solves support FP64 CG and FP32 CG with FP64 replacement");
+    if (pre && (segments || o.mode != Solve_FP32_CG_FP64_RELIABLE || o.precond != Precond_None))
+        throw std::invalid_argument("a caller's preconditioner takes the FP32 mode, no segments, no other preconditioner");
     op.upload_device();
     op.reserve_columns(segments ? 1 : ncols); // workspace of the batched apply, outside any capture
     BlockJacobiApply bja;
@@ -197,7 +199,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         xa.ensure(tot);
         p.ensure(tot);
         q.ensure(tot);
-        if (M) z.ensure(tot);
+        if (M || pre) z.ensure(tot);
         push();
         hipLaunchKernelGGL((mfill<float>), g, bl, 0, st, xa.p, 0.0f, L);
         hipLaunchKernelGGL((mfill<double>), g, bl, 0, st, Ax.p, 0.0, L);
@@ -205,7 +207,10 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
         // rho into slot `slot` of the columns flagged in slot msk (all if msk < 0)
         auto prec = [&](int slot, int msk) {
-            if (M) {
+            if (pre) {
+                (*pre)(r32.p, z.p);
+                hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, z.p, L, partial.p);
+            } else if (M) {
                 applyM_dot(r32.p, z.p);
             } else {
                 hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, r32.p, L, partial.p);
@@ -213,7 +218,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, slot, msk);
         };
         prec(0, -1);
-        hipLaunchKernelGGL((mcopy<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, L);
+        hipLaunchKernelGGL((mcopy<float>), g, bl, 0, st, p.p, (M || pre) ? z.p : r32.p, L);
         MHIP(hipGetLastError());
         pull();
         std::vector<double> maxr(ncols);
@@ -230,7 +235,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         };
         auto half2 = [&](int c) {
             prec(1 - c, -1);
-            hipLaunchKernelGGL((mp<float>), g, bl, 0, st, p.p, M ? z.p : r32.p, L, S.p, 1 - c, c);
+            hipLaunchKernelGGL((mp<float>), g, bl, 0, st, p.p, (M || pre) ? z.p : r32.p, L, S.p, 1 - c, c);
         };
         // between two checks: (half1 half2) x (check_every - 1), then half1, replayed from one
         // captured graph; the check's half2 follows it directly (an even count keeps the phase)

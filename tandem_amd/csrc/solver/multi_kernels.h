@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 
@@ -201,10 +202,14 @@ __global__ __launch_bounds__(kB) void mpflex(T* __restrict__ p, T const* __restr
 } // namespace
 
 class Operator;
+// A caller's preconditioner of the FP32 recursion: z = M^-1 r on the solver's columns.
+using PrecondFn32 = std::function<void(float const* r, float* z)>;
 // ncols lockstep CG recursions over the columns laid out by L (tot entries per vector): one batched
 // apply per iteration, or with segments one apply of the whole operator.  Buffers from w when given
-// (grown as needed), else allocated for the call.
+// (grown as needed), else allocated for the call.  pre (FP32 mode without segments, in place of
+// o.precond) is applied to the FP32 residuals.
 MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols, Cols L, std::size_t tot,
-                             bool segments, SolveOptions const& o, MultiWork* w = nullptr);
+                             bool segments, SolveOptions const& o, MultiWork* w = nullptr,
+                             PrecondFn32 const* pre = nullptr);
 
 } // namespace tdg

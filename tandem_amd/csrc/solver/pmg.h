@@ -29,10 +29,13 @@ struct PMGOptions {
     int vcycle_fp32 = 1;        // smoothing and transfers in FP32 (the outer flexible CG stays FP64)
     int coarse_assembled = 1;   // coarsest level on its assembled block rows where it has the register
                                 //   kernel (P1 / P2)
+    int coarse_amg = 1;         // coarse CG preconditioned by smoothed-aggregation AMG on those block
+                                //   rows (elasticity, FP32 coarse solve), else block Jacobi
 };
 
 struct PMGLevel;
 struct MultiWork;
+class AMG;
 
 class PMG {
 public:
@@ -66,6 +69,7 @@ private:
     std::vector<std::unique_ptr<PMGLevel>> lv_;
     std::vector<double> lmax_;
     std::unique_ptr<MultiWork> coarse_work_;
+    std::unique_ptr<AMG> amg_;
     int64_t coarse_its_ = 0, cycles_ = 0;
     bool ready_ = false;
 };

@@ -21,6 +21,7 @@
 #include "pmg.h"
 
 #include "../host/operator.h"
+#include "amg.h"
 #include "block_jacobi.h"
 #include "multi_kernels.h"
 #include "progress.h"
@@ -292,6 +293,11 @@ void PMG::setup() {
     if (o_.coarse_assembled && coarse.assembled_fast()) {
         coarse.assemble();
         coarse.set_variant(Variant_Assembled);
+        // (a coarse level small enough for one dense solve keeps block Jacobi: an exact FP32
+        // preconditioner leaves the FP32 recursion in rounding noise between two residual checks)
+        if (o_.coarse_amg && o_.coarse_fp32 && coarse.kind() == Kind_Elasticity &&
+            int64_t(coarse.local_size()) > AMGOptions().coarse_dofs)
+            amg_ = std::make_unique<AMG>(coarse);
     }
     for (auto* a : ops_) {
         a->upload_device();
@@ -349,8 +355,8 @@ template <class T> void PMG::cycle(int l, T const* b, T* x, int ncols, int64_t l
         co.max_iter = o_.coarse_max_iter;
         co.mode = o_.coarse_fp32 ? Solve_FP32_CG_FP64_RELIABLE : Solve_FP64_CG;
         co.reliable_delta = 1e-3; // no replacement above the loose coarse tolerance
-        co.precond = Precond_BlockJacobi;
-        co.check_every = 10;
+        co.precond = amg_ ? Precond_None : Precond_BlockJacobi;
+        co.check_every = amg_ ? 2 : 10; // AMG steps gain an order of magnitude each: check often
         co.graphs = 0;
         Cols C;
         C.ld = ld;
@@ -365,8 +371,9 @@ template <class T> void PMG::cycle(int l, T const* b, T* x, int ncols, int64_t l
             b64 = v.r.p;
             x64 = v.t.p;
         }
+        PrecondFn32 amg = [&](float const* rr, float* zz) { amg_->vcycle(rr, zz, ncols, ld); };
         auto r = lockstep_cg(*ops_[l], b64, x64, ncols, C, std::size_t(ncols) * std::size_t(ld), false, co,
-                             coarse_work_.get());
+                             coarse_work_.get(), amg_ ? &amg : nullptr);
         if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((vconvert<T, double>), g, bl, 0, st, x, x64, C.n, ld, ld);
         MHIP(hipGetLastError());
         coarse_its_ += r.total_iterations;

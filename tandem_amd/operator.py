@@ -530,13 +530,14 @@ class PMultigrid:
 
     def __init__(self, levels, smooth=None, eig_lo=None, eig_hi=None, eig_iters=None, coarse_rtol=None,
                  coarse_max_iter=None, galerkin_penalty=None, coarse_fp32=None, vcycle_fp32=None,
-                 coarse_assembled=None):
+                 coarse_assembled=None, coarse_amg=None):
         o = _lib.PMGOpts()
         lib().tdg_pmg_default_opts(C.byref(o))
         for k, v in (("smooth", smooth), ("eig_lo", eig_lo), ("eig_hi", eig_hi), ("eig_iters", eig_iters),
                      ("coarse_rtol", coarse_rtol), ("coarse_max_iter", coarse_max_iter),
                      ("galerkin_penalty", galerkin_penalty), ("coarse_fp32", coarse_fp32),
-                     ("vcycle_fp32", vcycle_fp32), ("coarse_assembled", coarse_assembled)):
+                     ("vcycle_fp32", vcycle_fp32), ("coarse_assembled", coarse_assembled),
+                     ("coarse_amg", coarse_amg)):
             if v is not None:
                 setattr(o, k, type(getattr(o, k))(v))
         self.options = {k: getattr(o, k) for k, _ in _lib.PMGOpts._fields_}

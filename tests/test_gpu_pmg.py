@@ -138,3 +138,28 @@ def test_pmg_solve_matches_block_jacobi_cg(cuda, dim, degrees, n, fp32):
         print("  column %d: |x_pmg - x_bj| / |x_bj| = %.3e" % (c, err), flush=True)
         assert err <= 1e-6
     assert r["iterations"] * 5 <= r2["iterations"]
+
+
+def test_amg_coarse_preconditioner(cuda):
+    """The coarse CG preconditioned by the smoothed-aggregation AMG (the default for an elasticity P1
+    coarse level solved in FP32; the reference's coarse preconditioner is GAMG) against block Jacobi:
+    the same solution to the solver tolerance, and a fraction of the coarse iterations."""
+    import torch
+
+    mesh = T.Mesh.box(2, 24, bc=T.BC_DIRICHLET)
+    B = None
+    out = {}
+    for amg in (0, 1):
+        ops, M = hierarchy(mesh, (4, 1), 7, coarse_amg=amg)
+        if B is None:
+            B = torch.from_numpy(np.random.default_rng(9).uniform(-1, 1, (4, ops[0].n))).to(cuda)
+        X = torch.zeros_like(B)
+        r = M.solve_multi(B, X, rtol=1e-10, max_iter=300)
+        print("coarse_amg", amg, {k: r[k] for k in ("iterations", "outer_iterations", "rel_residual", "seconds")},
+              flush=True)
+        assert r["converged"] == 1 and r["rel_residual"] <= 1e-10
+        out[amg] = (X.clone(), r["outer_iterations"], r["iterations"])
+    err = (torch.linalg.norm(out[1][0] - out[0][0]) / torch.linalg.norm(out[0][0])).item()
+    print("AMG vs block-Jacobi coarse CG: solution rel diff %.3e" % err, flush=True)
+    assert err <= 1e-7
+    assert out[1][1] * 3 <= out[0][1]
