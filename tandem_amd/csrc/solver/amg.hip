@@ -467,6 +467,8 @@ void bsr_launch(DevBsr const& A, float const* x, int64_t ldx, float* y, int64_t 
     TDG_BSR(3, 3)
     TDG_BSR(6, 3)
     TDG_BSR(3, 6)
+    TDG_BSR(12, 3)
+    TDG_BSR(3, 12)
     TDG_BSR(6, 6)
     TDG_BSR(12, 6)
     TDG_BSR(6, 12)
@@ -494,6 +496,8 @@ AMG::AMG(Operator& op, AMGOptions const& o) : op_(op), o_(o) {
     const int bs = int(op.block_size());
     const int m = D == 2 ? 3 : 6;
     const int64_t E = int64_t(op.num_elements());
+    if (!((D == 2 && (bs == 6 || bs == 12)) || (D == 3 && bs == 12)))
+        throw std::invalid_argument("AMG: elasticity P1 / P2 (2D) and P1 (3D) coarse levels only");
     op.assemble();
     std::vector<double> Ah;
     std::vector<int32_t> nb;

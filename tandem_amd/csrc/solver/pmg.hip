@@ -295,8 +295,9 @@ void PMG::setup() {
         coarse.set_variant(Variant_Assembled);
         // (a coarse level small enough for one dense solve keeps block Jacobi: an exact FP32
         // preconditioner leaves the FP32 recursion in rounding noise between two residual checks)
-        if (o_.coarse_amg && o_.coarse_fp32 && coarse.kind() == Kind_Elasticity &&
-            int64_t(coarse.local_size()) > AMGOptions().coarse_dofs)
+        const bool amg_shape = coarse.kind() == Kind_Elasticity &&
+                               ((coarse.dim() == 2 && coarse.degree() <= 2) || (coarse.dim() == 3 && coarse.degree() == 1));
+        if (o_.coarse_amg && o_.coarse_fp32 && amg_shape && int64_t(coarse.local_size()) > AMGOptions().coarse_dofs)
             amg_ = std::make_unique<AMG>(coarse);
     }
     for (auto* a : ops_) {
