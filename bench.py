@@ -166,8 +166,9 @@ def traffic_from_profile():
 
 def solve_comparison(op, mesh, nodes, fields, rtol=1e-10):
     """Time to a 1e-10 FP64 true relative residual: FP64 CG vs FP32 CG with FP64 residual
-    replacement, each without and with the element-block Jacobi preconditioner, on the
-    manufactured cosine solution (right-hand side built on the GPU)."""
+    replacement, each without and with the element-block Jacobi preconditioner, and FP64 flexible CG
+    with a p-multigrid V-cycle (AMG on its coarsest level), on the manufactured cosine solution
+    (right-hand side built on the GPU)."""
     import torch
     from tandem_amd import scenario as S
 
@@ -211,6 +212,32 @@ def solve_comparison(op, mesh, nodes, fields, rtol=1e-10):
                                                                 out["fp32-ru/" + tag]["seconds"])
         out["solution_rel_diff_fp32ru_vs_fp64/" + tag] = float(torch.linalg.norm(xs["fp32-ru/" + tag] - xs[ref]) /
                                                                torch.linalg.norm(xs[ref]))
+        if pc is None:
+            x_plain = xs["fp64/none"]
+    # p-multigrid: FP64 flexible CG with an FP32 V-cycle over P4 -> P2 -> P1 (coarse levels on the same
+    # mesh, warped geometry and material, their own facet penalties), the P1 level's CG
+    # preconditioned by smoothed-aggregation AMG on its assembled block rows
+    import tandem_amd as T
+    t0 = time.perf_counter()
+    levels = [op]
+    for p in (2, 1):
+        c = T.DGOperator(mesh, T.ELASTICITY, p, S.warp(mesh.affine_nodes(p)))
+        Xc = c.volume_points()
+        c.set_material(lam(Xc), mu(Xc))
+        levels.append(c)
+    M = T.PMultigrid(levels, galerkin_penalty=0)
+    M.setup()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    B = bd.view(1, -1).clone()
+    Xp = torch.zeros_like(B)
+    r = M.solve_multi(B, Xp, rtol=rtol, max_iter=1000)
+    log("solve pmg/amg", {k: r[k] for k in ("iterations", "outer_iterations", "rel_residual", "seconds", "converged")})
+    out["pmg-fp32vcycle/amg"] = {"seconds": r["seconds"], "setup_seconds": t_setup, "iterations": int(r["iterations"]),
+                                 "coarse_cg_iterations": int(r["outer_iterations"]), "levels": [4, 2, 1],
+                                 "rel_residual": float(r["rel_residual"]), "converged": bool(r["converged"])}
+    out["solution_rel_diff_pmg_vs_fp64/none"] = float(torch.linalg.norm(Xp[0] - x_plain) / torch.linalg.norm(x_plain))
+    out["speedup_pmg_vs_fp64/none"] = out["fp64/none"]["seconds"] / r["seconds"]
     return out
 
 
