@@ -322,40 +322,34 @@ Bsr smooth_prolongator(Bsr const& A, std::vector<double> const& Dinv, double ome
 // ------------------------------------------------------------------------------ device kernels
 constexpr int kT = 256;
 
+// One lane per scalar row (block row i, row r of its blocks) and column: BR lanes share a block row
+// (the same x blocks, consecutive block rows), so a level of a few thousand block rows still fills
+// the chip for a single column.
 template <int BR, int BC, int MODE> // MODE 0: y = A x; 1: y = b - A x; 2: y += A x
 __global__ __launch_bounds__(kT) void bsr_kernel(int64_t n, int64_t const* __restrict__ rp,
                                                  int32_t const* __restrict__ ci, float const* __restrict__ v,
                                                  float const* __restrict__ xs, int64_t ldx, float* __restrict__ ys,
                                                  int64_t ldy, float const* __restrict__ bs, int64_t ldb) {
-    const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
-    if (i >= n) return;
+    const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
+    if (t >= n * BR) return;
+    const int64_t i = t / BR;
+    const int r = int(t - i * BR);
     const int64_t c = blockIdx.y;
     float const* x = xs + c * ldx;
-    float acc[BR];
-#pragma unroll
-    for (int r = 0; r < BR; ++r) acc[r] = 0.0f;
+    float acc = 0.0f;
     for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
-        float const* b = v + k * (BR * BC);
+        float const* b = v + k * (BR * BC) + r * BC;
         float const* xv = x + int64_t(ci[k]) * BC;
-        float xr[BC];
 #pragma unroll
-        for (int s = 0; s < BC; ++s) xr[s] = xv[s];
-#pragma unroll
-        for (int r = 0; r < BR; ++r)
-#pragma unroll
-            for (int s = 0; s < BC; ++s) acc[r] += b[r * BC + s] * xr[s];
+        for (int s = 0; s < BC; ++s) acc += b[s] * xv[s];
     }
-    float* y = ys + c * ldy + i * BR;
+    float* y = ys + c * ldy + t;
     if constexpr (MODE == 0) {
-#pragma unroll
-        for (int r = 0; r < BR; ++r) y[r] = acc[r];
+        *y = acc;
     } else if constexpr (MODE == 1) {
-        float const* b = bs + c * ldb + i * BR;
-#pragma unroll
-        for (int r = 0; r < BR; ++r) y[r] = b[r] - acc[r];
+        *y = bs[c * ldb + t] - acc;
     } else {
-#pragma unroll
-        for (int r = 0; r < BR; ++r) y[r] += acc[r];
+        *y += acc;
     }
 }
 
@@ -456,7 +450,7 @@ struct DevBsr {
 template <int MODE>
 void bsr_launch(DevBsr const& A, float const* x, int64_t ldx, float* y, int64_t ldy, float const* b, int64_t ldb,
                 int ncols, hipStream_t s) {
-    const dim3 g(blocks_for(A.n), unsigned(ncols)), t(kT);
+    const dim3 g(blocks_for(A.n * A.br), unsigned(ncols)), t(kT);
 #define TDG_BSR(R, C)                                                                                        \
     if (A.br == R && A.bc == C) {                                                                            \
         hipLaunchKernelGGL((bsr_kernel<R, C, MODE>), g, t, 0, s, A.n, A.rp.p, A.ci.p, A.v.p, x, ldx, y, ldy, b, \
