@@ -9,10 +9,11 @@
 // there is none).
 //
 // Apply kernels:
-//   * register kernel (small blocks: P1 and P2 levels of p-multigrid): each lane keeps R rows of its
-//     element's block row in registers and sweeps a range of columns, reading per column the
-//     element's and its neighbours' x blocks (lanes of consecutive elements read consecutive
-//     addresses for the own block) and writing R entries of y.  The matrix is read from HBM once per
+//   * register kernel (small blocks: P1 and P2 levels of p-multigrid): one lane per element; it keeps
+//     R rows of its element's block row in registers (all BS rows where they fit, else BS / R row
+//     groups one after another) and sweeps a range of columns, reading per column the element's and
+//     its neighbours' x blocks (lanes of consecutive elements read consecutive addresses for the own
+//     block) and writing R entries of y.  The matrix is read from HBM once per
 //     column range instead of once per column, so a 64-column apply costs little more than the
 //     64 x / y column reads and writes.
 //   * generic kernel (any block size; the cross-check of the matrix-free apply at high order): one
@@ -36,21 +37,28 @@ template <class T, int BS, int NB, int R, bool DOT>
 __global__ __launch_bounds__(kThreads) void assembled_apply_kernel(AssembledArgs<T> a, T const* __restrict__ xs,
                                                                    T* __restrict__ ys, int64_t ld, int ncols,
                                                                    int cchunk, double* __restrict__ partial) {
-    constexpr int LPE = BS / R;   // lanes per element
-    constexpr int EPW = 64 / LPE; // elements per wave
-    constexpr int EPB = EPW * kWavesPerBlock;
+    constexpr int PARTS = BS / R; // row groups, taken one after another by each lane
+    constexpr int EPB = 64 * kWavesPerBlock;
+    static_assert(!DOT || PARTS == 1, "the fused sums need the whole block row in one pass");
     __shared__ double red[kWavesPerBlock][kDotCols];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int el = lane / LPE, part = lane - el * LPE;
     const int64_t E = a.num_elements;
     const int c0 = blockIdx.y * cchunk;
     const int c1 = min(ncols, c0 + cchunk);
     if (DOT && lane == 0)
         for (int k = 0; k < kDotCols; ++k) red[wave][k] = 0.0;
     auto block = [&](int64_t blk) {
-        const int64_t e = blk * EPB + int64_t(wave) * EPW + el;
-        const bool act = el < EPW && e < E;
+        // one element per lane: 64 consecutive elements per wave, so every block-row load is one
+        // contiguous 64-element stream
+        const int64_t e = blk * EPB + int64_t(wave) * 64 + lane;
+        const bool act = e < E;
         const int64_t ec = act ? e : 0; // idle lanes read element 0 and store nothing
+        int64_t off[NB];
+        off[0] = ec * BS;
+#pragma unroll
+        for (int f = 0; f + 1 < NB; ++f) off[1 + f] = int64_t(a.nb[int64_t(f) * E + ec]) * BS;
+#pragma unroll 1
+        for (int part = 0; part < PARTS; ++part) {
         T m[NB][R][BS];
 #pragma unroll
         for (int b = 0; b < NB; ++b)
@@ -58,10 +66,6 @@ __global__ __launch_bounds__(kThreads) void assembled_apply_kernel(AssembledArgs
             for (int r = 0; r < R; ++r)
 #pragma unroll
                 for (int j = 0; j < BS; ++j) m[b][r][j] = a.A[((int64_t(b) * BS + part * R + r) * BS + j) * E + ec];
-        int64_t off[NB];
-        off[0] = ec * BS;
-#pragma unroll
-        for (int f = 0; f + 1 < NB; ++f) off[1 + f] = int64_t(a.nb[int64_t(f) * E + ec]) * BS;
         // small block rows hold the next column's x blocks in flight during this column's products
         constexpr bool PRE = NB * BS * int(sizeof(T) / 4) <= 48;
         T v[NB][BS];
@@ -115,6 +119,7 @@ __global__ __launch_bounds__(kThreads) void assembled_apply_kernel(AssembledArgs
 #pragma unroll
                 for (int j = 0; j < BS; ++j) v[b][j] = w[b][j];
         }
+        }
     };
     if constexpr (DOT) {
         const int64_t nblk = (E + EPB - 1) / EPB;
@@ -151,7 +156,7 @@ __global__ __launch_bounds__(kThreads) void assembled_apply_generic(AssembledArg
 template <class T, int BS, int NB, int R>
 hipError_t run_reg(AssembledArgs<T> const& a, T const* x, T* y, int ncols, int64_t ld, double* partial,
                    hipStream_t s) {
-    constexpr int EPB = (64 / (BS / R)) * kWavesPerBlock;
+    constexpr int EPB = 64 * kWavesPerBlock;
     if (partial) {
         // one element per lane only: with several lanes per element the fused sums cost registers
         if constexpr (R == BS) {
