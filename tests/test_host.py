@@ -291,3 +291,22 @@ def test_pmultigrid_rejects_inconsistent_levels():
     unset = T.DGOperator(m, T.ELASTICITY, 2)
     with pytest.raises(L.TdgError, match="set_material"):
         T.PMultigrid([p4, unset])
+    # the defaults put the coarsest level on its assembled block rows with the AMG coarse
+    # preconditioner (the C struct and its Python mirror agree field by field)
+    o = L.PMGOpts()
+    L.lib().tdg_pmg_default_opts(C.byref(o))
+    assert (o.smooth, o.coarse_max_iter, o.galerkin_penalty, o.coarse_fp32, o.vcycle_fp32) == (6, 500, 1, 1, 1)
+    assert (o.eig_lo, o.eig_hi, o.coarse_rtol) == (0.05, 1.1, 0.1)
+    assert o.coarse_assembled == 1 and o.coarse_amg == 1
+
+
+def test_assembled_variant_and_block_arguments_on_the_host():
+    """The assembled variant is a valid kernel choice (nothing runs until an apply); block queries
+    outside the block row are rejected before any device work."""
+    op = T.DGOperator(T.Mesh.box(2, 2, 0, T.BC_DIRICHLET), T.ELASTICITY, 1)
+    op.set_material(np.full((op.E, op.Nq), 2.0), np.full((op.E, op.Nq), 1.0))
+    op.set_variant("assembled")
+    assert L.lib().tdg_set_variant(op._h, 5) != 0
+    for e, slot in ((-1, 0), (op.E, 0), (0, -1), (0, op.dim + 2)):
+        with pytest.raises(L.TdgError, match="no such block"):
+            op.assembled_block(e, slot)
