@@ -275,7 +275,9 @@ int tdg_assembled_block(tdg_op* op, int64_t element, int slot, double* block);
  * aggregation AMG built on those block rows (the reference's coarse preconditioner is PETSc's GAMG,
  * examples/options/mg_cheby.cfg) instead of block Jacobi: rigid-body near-null space, Chebyshev
  * smoothing, Galerkin coarse operators down to a dense solve.
- * tdg_pmg_create keeps the level pointers (the operators must outlive the hierarchy);
+ * tdg_pmg_create keeps the level pointers (the operators must outlive the hierarchy, and a level
+ * whose material or penalty changes after tdg_pmg_setup needs a new hierarchy: the set-up's
+ * eigenvalue estimates and AMG levels are not rebuilt);
  * tdg_pmg_setup builds the block-Jacobi inverses and the eigenvalue estimates (lambda_max: host
  * array of nlevels-1, may be NULL); tdg_pmg_vcycle applies one V-cycle z = M^-1 r to ncols device
  * FP64 columns at stride ld; tdg_solve_multi_pmg solves ncols right-hand sides together by flexible
