@@ -218,14 +218,18 @@ def solve_comparison(op, mesh, nodes, fields, rtol=1e-10):
     out["pmg-fp32vcycle/amg"] = rec
     out["solution_rel_diff_pmg_vs_fp64/none"] = float(torch.linalg.norm(xp - x_plain) / torch.linalg.norm(x_plain))
     out["speedup_pmg_vs_fp64/none"] = out["fp64/none"]["seconds"] / rec["seconds"]
+    # the same hierarchy and preconditioners with the V-cycle (and its coarse AMG-CG) in FP64
+    rec64, _ = pmg_solve(op, mesh, fields, bd, rtol, fp32=False)
+    out["pmg-fp64vcycle/amg"] = rec64
+    out["speedup_pmg_fp32vcycle_vs_fp64vcycle"] = rec64["seconds"] / rec["seconds"]
     return out
 
 
-def pmg_solve(op, mesh, fields, b, rtol):
+def pmg_solve(op, mesh, fields, b, rtol, fp32=True):
     """p-multigrid: FP64 flexible CG with an FP32 V-cycle over P4 -> P2 -> P1 (coarse levels on the
     same mesh, warped geometry and material, their own facet penalties), the P1 level's CG
-    preconditioned by smoothed-aggregation AMG on its assembled block rows.  Returns the record and
-    the solution."""
+    preconditioned by smoothed-aggregation AMG on its assembled block rows (with fp32=False the
+    V-cycle and that coarse solve run in FP64).  Returns the record and the solution."""
     import torch
     import tandem_amd as T
     from tandem_amd import scenario as S
@@ -238,14 +242,15 @@ def pmg_solve(op, mesh, fields, b, rtol):
         Xc = c.volume_points()
         c.set_material(lam(Xc), mu(Xc))
         levels.append(c)
-    M = T.PMultigrid(levels, galerkin_penalty=0)
+    M = T.PMultigrid(levels, galerkin_penalty=0, coarse_fp32=int(fp32), vcycle_fp32=int(fp32))
     M.setup()
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     B = b.view(1, -1).clone()
     Xp = torch.zeros_like(B)
     r = M.solve_multi(B, Xp, rtol=rtol, max_iter=1000)
-    log("solve pmg/amg", {k: r[k] for k in ("iterations", "outer_iterations", "rel_residual", "seconds", "converged")})
+    log("solve pmg/amg %s V-cycle" % ("FP32" if fp32 else "FP64"),
+        {k: r[k] for k in ("iterations", "outer_iterations", "rel_residual", "seconds", "converged")})
     rec = {"seconds": r["seconds"], "setup_seconds": t_setup, "iterations": int(r["iterations"]),
            "coarse_cg_iterations": int(r["outer_iterations"]), "levels": [4, 2, 1],
            "rel_residual": float(r["rel_residual"]), "converged": bool(r["converged"])}

@@ -271,7 +271,7 @@ int tdg_assembled_block(tdg_op* op, int64_t element, int slot, double* block);
  * coarse_assembled the coarsest operator is switched to TDG_VARIANT_ASSEMBLED at set-up where its
  * block size has the register kernel (P1 and P2 levels): the coarse CG then reads the assembled
  * block rows once per range of columns instead of the per-point geometry once per column.  With
- * coarse_amg as well (elasticity, coarse_fp32) the coarse CG is preconditioned by a smoothed-
+ * coarse_amg as well (elasticity; FP32 or FP64 with coarse_fp32) the coarse CG is preconditioned by a smoothed-
  * aggregation AMG built on those block rows (the reference's coarse preconditioner is PETSc's GAMG,
  * examples/options/mg_cheby.cfg) instead of block Jacobi: rigid-body near-null space, Chebyshev
  * smoothing, Galerkin coarse operators down to a dense solve.

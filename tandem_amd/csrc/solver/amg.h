@@ -10,7 +10,7 @@
 //     P = (I - 4/(3 rho) D^-1 A) P_tent with rho = lambda_max(D^-1 A) by power iteration (D the
 //     block diagonal); coarse operator P^T A P; down to at most coarse_dofs unknowns, solved by a
 //     dense inverse (an operator that small from the start is inverted directly);
-//   V-cycle (device, FP32, ncols columns at once): Chebyshev smoothing on D^-1 A over
+//   V-cycle (device, FP32 or FP64, ncols columns at once): Chebyshev smoothing on D^-1 A over
 //     [0.1, 1.1] x rho (GAMG's default interval), the same polynomial before and after the coarse
 //     correction, restriction P^T: a fixed symmetric positive map, so plain CG may use it.
 // Level 0 multiplies with the operator itself (its assembled apply) and its block-Jacobi inverses.
@@ -45,17 +45,21 @@ public:
     AMG(AMG const&) = delete;
     AMG& operator=(AMG const&) = delete;
 
-    // Z = M^-1 R for ncols FP32 columns of the operator's size at stride ld (one V-cycle).
+    // Z = M^-1 R for ncols columns of the operator's size at stride ld (one V-cycle, in the
+    // precision of the vectors: FP32 operators and transfers, or FP64).
     void vcycle(float const* r, float* z, int ncols, int64_t ld);
+    void vcycle(double const* r, double* z, int ncols, int64_t ld);
     int num_levels() const { return int(dofs_.size()); }
     std::vector<int64_t> const& level_dofs() const { return dofs_; }
     // sum over levels of the stored block entries / the level-0 entries
     double operator_complexity() const { return complexity_; }
 
 private:
-    void smooth(int l, float const* b, int64_t ldb, float* x, int64_t ldx, int ncols, bool zero_guess);
-    void residual(int l, float const* b, int64_t ldb, float const* x, int64_t ldx, float* r, int64_t ldr, int ncols);
-    void cycle(int l, float const* b, int64_t ldb, float* x, int64_t ldx, int ncols);
+    template <class T> void run(T const* r, T* z, int ncols, int64_t ld);
+    template <class T> void smooth(int l, T const* b, int64_t ldb, T* x, int64_t ldx, int ncols, bool zero_guess);
+    template <class T>
+    void residual(int l, T const* b, int64_t ldb, T const* x, int64_t ldx, T* r, int64_t ldr, int ncols);
+    template <class T> void cycle(int l, T const* b, int64_t ldb, T* x, int64_t ldx, int ncols);
 
     Operator& op_;
     AMGOptions o_;
@@ -63,8 +67,8 @@ private:
     std::vector<int64_t> dofs_;
     std::vector<double> rho_;
     double complexity_ = 1.0;
-    int cap_cols_ = 0;
-    int64_t cap_ld_ = 0;
+    int cap_cols_[2] = {0, 0}; // per precision (FP32, FP64)
+    int64_t cap_ld_[2] = {0, 0};
 };
 
 } // namespace tdg

@@ -325,25 +325,25 @@ constexpr int kT = 256;
 // One lane per scalar row (block row i, row r of its blocks) and column: BR lanes share a block row
 // (the same x blocks, consecutive block rows), so a level of a few thousand block rows still fills
 // the chip for a single column.
-template <int BR, int BC, int MODE> // MODE 0: y = A x; 1: y = b - A x; 2: y += A x
+template <class T, int BR, int BC, int MODE> // MODE 0: y = A x; 1: y = b - A x; 2: y += A x
 __global__ __launch_bounds__(kT) void bsr_kernel(int64_t n, int64_t const* __restrict__ rp,
-                                                 int32_t const* __restrict__ ci, float const* __restrict__ v,
-                                                 float const* __restrict__ xs, int64_t ldx, float* __restrict__ ys,
-                                                 int64_t ldy, float const* __restrict__ bs, int64_t ldb) {
+                                                 int32_t const* __restrict__ ci, T const* __restrict__ v,
+                                                 T const* __restrict__ xs, int64_t ldx, T* __restrict__ ys,
+                                                 int64_t ldy, T const* __restrict__ bs, int64_t ldb) {
     const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
     if (t >= n * BR) return;
     const int64_t i = t / BR;
     const int r = int(t - i * BR);
     const int64_t c = blockIdx.y;
-    float const* x = xs + c * ldx;
-    float acc = 0.0f;
+    T const* x = xs + c * ldx;
+    T acc = T(0);
     for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
-        float const* b = v + k * (BR * BC) + r * BC;
-        float const* xv = x + int64_t(ci[k]) * BC;
+        T const* b = v + k * (BR * BC) + r * BC;
+        T const* xv = x + int64_t(ci[k]) * BC;
 #pragma unroll
         for (int s = 0; s < BC; ++s) acc += b[s] * xv[s];
     }
-    float* y = ys + c * ldy + t;
+    T* y = ys + c * ldy + t;
     if constexpr (MODE == 0) {
         *y = acc;
     } else if constexpr (MODE == 1) {
@@ -354,47 +354,47 @@ __global__ __launch_bounds__(kT) void bsr_kernel(int64_t n, int64_t const* __res
 }
 
 // x = Ainv b (dense N x N, row-major), ncols columns
-__global__ __launch_bounds__(kT) void dense_kernel(int64_t N, float const* __restrict__ Ainv,
-                                                   float const* __restrict__ bs, int64_t ldb, float* __restrict__ xs,
-                                                   int64_t ldx) {
+template <class T>
+__global__ __launch_bounds__(kT) void dense_kernel(int64_t N, T const* __restrict__ Ainv, T const* __restrict__ bs,
+                                                   int64_t ldb, T* __restrict__ xs, int64_t ldx) {
     const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
     if (i >= N) return;
-    float const* b = bs + int64_t(blockIdx.y) * ldb;
-    float const* a = Ainv + i * N;
-    float s = 0.0f;
+    T const* b = bs + int64_t(blockIdx.y) * ldb;
+    T const* a = Ainv + i * N;
+    T s = T(0);
     for (int64_t j = 0; j < N; ++j) s += a[j] * b[j];
     xs[int64_t(blockIdx.y) * ldx + i] = s;
 }
 
 // d = z / theta; x = d (zero guess) or x += d
-__global__ __launch_bounds__(kT) void cheb_first_kernel(int64_t N, float* __restrict__ x, int64_t ldx,
-                                                        float* __restrict__ d, int64_t ldd,
-                                                        float const* __restrict__ z, int64_t ldz, float inv_theta,
+template <class T>
+__global__ __launch_bounds__(kT) void cheb_first_kernel(int64_t N, T* __restrict__ x, int64_t ldx, T* __restrict__ d,
+                                                        int64_t ldd, T const* __restrict__ z, int64_t ldz, T inv_theta,
                                                         int zero) {
     const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
     if (i >= N) return;
     const int64_t c = blockIdx.y;
-    const float dv = z[c * ldz + i] * inv_theta;
+    const T dv = z[c * ldz + i] * inv_theta;
     d[c * ldd + i] = dv;
     x[c * ldx + i] = zero ? dv : x[c * ldx + i] + dv;
 }
 
 // d = c1 d + c2 z; x += d
-__global__ __launch_bounds__(kT) void cheb_next_kernel(int64_t N, float* __restrict__ x, int64_t ldx,
-                                                       float* __restrict__ d, int64_t ldd, float const* __restrict__ z,
-                                                       int64_t ldz, float c1, float c2) {
+template <class T>
+__global__ __launch_bounds__(kT) void cheb_next_kernel(int64_t N, T* __restrict__ x, int64_t ldx, T* __restrict__ d,
+                                                       int64_t ldd, T const* __restrict__ z, int64_t ldz, T c1, T c2) {
     const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
     if (i >= N) return;
     const int64_t c = blockIdx.y;
-    const float dv = c1 * d[c * ldd + i] + c2 * z[c * ldz + i];
+    const T dv = c1 * d[c * ldd + i] + c2 * z[c * ldz + i];
     d[c * ldd + i] = dv;
     x[c * ldx + i] += dv;
 }
 
 // r = b - t
-__global__ __launch_bounds__(kT) void sub_kernel(int64_t N, float* __restrict__ r, int64_t ldr,
-                                                 float const* __restrict__ b, int64_t ldb, float const* __restrict__ t,
-                                                 int64_t ldt) {
+template <class T>
+__global__ __launch_bounds__(kT) void sub_kernel(int64_t N, T* __restrict__ r, int64_t ldr, T const* __restrict__ b,
+                                                 int64_t ldb, T const* __restrict__ t, int64_t ldt) {
     const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
     if (i >= N) return;
     const int64_t c = blockIdx.y;
@@ -430,12 +430,14 @@ template <class T> struct Dev {
     }
 };
 
+// A Bsr on the device in both precisions (structure shared).
 struct DevBsr {
     int64_t n = 0;
     int br = 0, bc = 0;
     Dev<int64_t> rp;
     Dev<int32_t> ci;
-    Dev<float> v;
+    Dev<float> v32;
+    Dev<double> v64;
     void upload(Bsr const& h) {
         n = h.n;
         br = h.br;
@@ -443,17 +445,23 @@ struct DevBsr {
         rp.upload(h.rp);
         ci.upload(h.ci);
         std::vector<float> f(h.v.begin(), h.v.end());
-        v.upload(f);
+        v32.upload(f);
+        v64.upload(h.v);
+    }
+    template <class T> T const* values() const {
+        if constexpr (sizeof(T) == 4) return v32.p;
+        else return v64.p;
     }
 };
 
-template <int MODE>
-void bsr_launch(DevBsr const& A, float const* x, int64_t ldx, float* y, int64_t ldy, float const* b, int64_t ldb,
-                int ncols, hipStream_t s) {
+template <int MODE, class T>
+void bsr_launch(DevBsr const& A, T const* x, int64_t ldx, T* y, int64_t ldy, T const* b, int64_t ldb, int ncols,
+                hipStream_t s) {
     const dim3 g(blocks_for(A.n * A.br), unsigned(ncols)), t(kT);
+    T const* v = A.values<T>();
 #define TDG_BSR(R, C)                                                                                        \
     if (A.br == R && A.bc == C) {                                                                            \
-        hipLaunchKernelGGL((bsr_kernel<R, C, MODE>), g, t, 0, s, A.n, A.rp.p, A.ci.p, A.v.p, x, ldx, y, ldy, b, \
+        hipLaunchKernelGGL((bsr_kernel<T, R, C, MODE>), g, t, 0, s, A.n, A.rp.p, A.ci.p, v, x, ldx, y, ldy, b,   \
                            ldb);                                                                             \
         MHIP(hipGetLastError());                                                                             \
         return;                                                                                              \
@@ -470,17 +478,32 @@ void bsr_launch(DevBsr const& A, float const* x, int64_t ldx, float* y, int64_t 
     throw std::logic_error("AMG: no kernel for this block shape");
 }
 
+// Work vectors of one level in one precision.
+template <class T> struct LevelVectors {
+    Dev<T> x, b, r, d, z, t;
+};
+
 } // namespace
 
 // Device data of one level: the operator (levels >= 1), block-diagonal inverse as a diagonal Bsr,
-// the prolongator from the next level and its transpose, and work vectors.
+// the prolongator from the next level and its transpose (both precisions), and work vectors.
 struct AMGLevelDev {
     int64_t n = 0; // block rows
     int bs = 0;
     DevBsr A, Dinv, P, R;
-    Dev<float> dense; // coarsest level: dense inverse
-    Dev<float> x, b, r, d, z, t;
-    int64_t ld = 0;   // stride of this level's work vectors
+    Dev<float> dense32; // coarsest level: dense inverse
+    Dev<double> dense64;
+    LevelVectors<float> w32;
+    LevelVectors<double> w64;
+    int64_t ld = 0; // stride of this level's work vectors
+    template <class T> LevelVectors<T>& w() {
+        if constexpr (sizeof(T) == 4) return w32;
+        else return w64;
+    }
+    template <class T> T const* dense() const {
+        if constexpr (sizeof(T) == 4) return dense32.p;
+        else return dense64.p;
+    }
 };
 
 AMG::AMG(Operator& op, AMGOptions const& o) : op_(op), o_(o) {
@@ -597,7 +620,8 @@ AMG::AMG(Operator& op, AMGOptions const& o) : op_(op), o_(o) {
                                 A.blk(k)[r * A.bc + c];
             if (!dense_inverse(Dn, N)) throw std::runtime_error("AMG: coarse matrix is not positive definite");
             std::vector<float> f(Dn.begin(), Dn.end());
-            lv->dense.upload(f);
+            lv->dense32.upload(f);
+            lv->dense64.upload(Dn);
             lv_.push_back(std::move(lv));
             break;
         }
@@ -625,15 +649,16 @@ AMG::AMG(Operator& op, AMGOptions const& o) : op_(op), o_(o) {
 AMG::~AMG() = default;
 
 // r = b - A_l x
-void AMG::residual(int l, float const* b, int64_t ldb, float const* x, int64_t ldx, float* r, int64_t ldr,
-                   int ncols) {
+template <class T>
+void AMG::residual(int l, T const* b, int64_t ldb, T const* x, int64_t ldx, T* r, int64_t ldr, int ncols) {
     AMGLevelDev& v = *lv_[l];
     hipStream_t st = op_.stream();
     if (l == 0) {
-        op_.apply_batched(x, v.t.p, ncols, v.ld);
+        T* t = v.w<T>().t.p;
+        op_.apply_batched(x, t, ncols, v.ld);
         const int64_t N = v.n * v.bs;
-        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(N), unsigned(ncols)), dim3(kT), 0, st, N, r, ldr, b, ldb, v.t.p,
-                           v.ld);
+        hipLaunchKernelGGL((sub_kernel<T>), dim3(blocks_for(N), unsigned(ncols)), dim3(kT), 0, st, N, r, ldr, b, ldb,
+                           static_cast<T const*>(t), v.ld);
         MHIP(hipGetLastError());
     } else {
         bsr_launch<1>(v.A, x, ldx, r, ldr, b, ldb, ncols, st);
@@ -641,79 +666,88 @@ void AMG::residual(int l, float const* b, int64_t ldb, float const* x, int64_t l
 }
 
 // Chebyshev steps on D^-1 A from x (or from zero)
-void AMG::smooth(int l, float const* b, int64_t ldb, float* x, int64_t ldx, int ncols, bool zero_guess) {
+template <class T>
+void AMG::smooth(int l, T const* b, int64_t ldb, T* x, int64_t ldx, int ncols, bool zero_guess) {
     AMGLevelDev& v = *lv_[l];
+    LevelVectors<T>& w = v.w<T>();
     hipStream_t st = op_.stream();
     const int64_t N = v.n * v.bs;
     const dim3 g(blocks_for(N), unsigned(ncols)), t(kT);
     const double lo = o_.eig_lo * rho_[l], hi = o_.eig_hi * rho_[l];
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
     double rho0 = 1.0 / sigma;
-    auto precond = [&](float const* r, float* z) {
+    auto precond = [&](T const* r, T* z) {
         if (l == 0) op_.block_jacobi().apply(r, z, st, ncols, v.ld);
-        else bsr_launch<0>(v.Dinv, r, v.ld, z, v.ld, nullptr, 0, ncols, st);
+        else bsr_launch<0>(v.Dinv, r, v.ld, z, v.ld, static_cast<T const*>(nullptr), 0, ncols, st);
     };
     if (zero_guess) {
-        precond(b, v.z.p);
+        precond(b, w.z.p);
     } else {
-        residual(l, b, ldb, x, ldx, v.r.p, v.ld, ncols);
-        precond(v.r.p, v.z.p);
+        residual<T>(l, b, ldb, x, ldx, w.r.p, v.ld, ncols);
+        precond(w.r.p, w.z.p);
     }
-    hipLaunchKernelGGL(cheb_first_kernel, g, t, 0, st, N, x, ldx, v.d.p, v.ld, v.z.p, v.ld, float(1.0 / theta),
-                       zero_guess ? 1 : 0);
+    hipLaunchKernelGGL((cheb_first_kernel<T>), g, t, 0, st, N, x, ldx, w.d.p, v.ld, static_cast<T const*>(w.z.p),
+                       v.ld, T(1.0 / theta), zero_guess ? 1 : 0);
     MHIP(hipGetLastError());
     for (int k = 1; k < o_.smooth; ++k) {
-        residual(l, b, ldb, x, ldx, v.r.p, v.ld, ncols);
-        precond(v.r.p, v.z.p);
+        residual<T>(l, b, ldb, x, ldx, w.r.p, v.ld, ncols);
+        precond(w.r.p, w.z.p);
         const double rho1 = 1.0 / (2.0 * sigma - rho0);
-        hipLaunchKernelGGL(cheb_next_kernel, g, t, 0, st, N, x, ldx, v.d.p, v.ld, v.z.p, v.ld, float(rho1 * rho0),
-                           float(2.0 * rho1 / delta));
+        hipLaunchKernelGGL((cheb_next_kernel<T>), g, t, 0, st, N, x, ldx, w.d.p, v.ld, static_cast<T const*>(w.z.p),
+                           v.ld, T(rho1 * rho0), T(2.0 * rho1 / delta));
         MHIP(hipGetLastError());
         rho0 = rho1;
     }
 }
 
-void AMG::cycle(int l, float const* b, int64_t ldb, float* x, int64_t ldx, int ncols) {
+template <class T> void AMG::cycle(int l, T const* b, int64_t ldb, T* x, int64_t ldx, int ncols) {
     AMGLevelDev& v = *lv_[l];
     hipStream_t st = op_.stream();
     if (l + 1 == int(lv_.size())) {
         const int64_t N = v.n * v.bs;
-        hipLaunchKernelGGL(dense_kernel, dim3(blocks_for(N), unsigned(ncols)), dim3(kT), 0, st, N, v.dense.p, b, ldb, x,
-                           ldx);
+        hipLaunchKernelGGL((dense_kernel<T>), dim3(blocks_for(N), unsigned(ncols)), dim3(kT), 0, st, N, v.dense<T>(), b,
+                           ldb, x, ldx);
         MHIP(hipGetLastError());
         return;
     }
     AMGLevelDev& c = *lv_[l + 1];
-    smooth(l, b, ldb, x, ldx, ncols, true);
-    residual(l, b, ldb, x, ldx, v.r.p, v.ld, ncols);
-    bsr_launch<0>(v.R, v.r.p, v.ld, c.b.p, c.ld, nullptr, 0, ncols, st);
-    cycle(l + 1, c.b.p, c.ld, c.x.p, c.ld, ncols);
-    bsr_launch<2>(v.P, c.x.p, c.ld, x, ldx, nullptr, 0, ncols, st);
-    smooth(l, b, ldb, x, ldx, ncols, false);
+    LevelVectors<T>& w = v.w<T>();
+    LevelVectors<T>& wc = c.w<T>();
+    smooth<T>(l, b, ldb, x, ldx, ncols, true);
+    residual<T>(l, b, ldb, x, ldx, w.r.p, v.ld, ncols);
+    bsr_launch<0>(v.R, static_cast<T const*>(w.r.p), v.ld, wc.b.p, c.ld, static_cast<T const*>(nullptr), 0, ncols, st);
+    cycle<T>(l + 1, wc.b.p, c.ld, wc.x.p, c.ld, ncols);
+    bsr_launch<2>(v.P, static_cast<T const*>(wc.x.p), c.ld, x, ldx, static_cast<T const*>(nullptr), 0, ncols, st);
+    smooth<T>(l, b, ldb, x, ldx, ncols, false);
 }
 
-void AMG::vcycle(float const* r, float* z, int ncols, int64_t ld) {
+template <class T> void AMG::run(T const* r, T* z, int ncols, int64_t ld) {
     if (ncols < 1 || ld < dofs_[0] || (ncols > 65535)) throw std::invalid_argument("AMG: bad column layout");
-    if (ncols > cap_cols_ || ld != cap_ld_) {
+    const int p = sizeof(T) == 4 ? 0 : 1;
+    if (ncols > cap_cols_[p] || ld != cap_ld_[p]) {
         for (std::size_t l = 0; l < lv_.size(); ++l) {
             AMGLevelDev& v = *lv_[l];
             v.ld = l == 0 ? ld : v.n * v.bs;
+            LevelVectors<T>& w = v.w<T>();
             const std::size_t tot = std::size_t(ncols) * std::size_t(v.ld);
             if (l > 0) {
-                v.x.ensure(tot);
-                v.b.ensure(tot);
+                w.x.ensure(tot);
+                w.b.ensure(tot);
             }
             if (l + 1 < lv_.size()) {
-                v.r.ensure(tot);
-                v.d.ensure(tot);
-                v.z.ensure(tot);
-                if (l == 0) v.t.ensure(tot);
+                w.r.ensure(tot);
+                w.d.ensure(tot);
+                w.z.ensure(tot);
+                if (l == 0) w.t.ensure(tot);
             }
         }
-        cap_cols_ = ncols;
-        cap_ld_ = ld;
+        cap_cols_[p] = ncols;
+        cap_ld_[p] = ld;
     }
-    cycle(0, r, ld, z, ld, ncols);
+    cycle<T>(0, r, ld, z, ld, ncols);
 }
+
+void AMG::vcycle(float const* r, float* z, int ncols, int64_t ld) { run(r, z, ncols, ld); }
+void AMG::vcycle(double const* r, double* z, int ncols, int64_t ld) { run(r, z, ncols, ld); }
 
 } // namespace tdg

@@ -34,11 +34,13 @@ namespace tdg {
 // The lockstep recursions over ncols columns laid out by L (tot entries per vector).  Segments:
 // one apply of the whole operator per iteration; columns: one batched apply.
 MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols, Cols L, std::size_t tot,
-                             bool segments, SolveOptions const& o, MultiWork* w, PrecondFn32 const* pre) {
+                             bool segments, SolveOptions const& o, MultiWork* w, CallerPrecond const* pre) {
     if (o.mode != Solve_FP64_CG && o.mode != Solve_FP32_CG_FP64_RELIABLE)
         throw std::invalid_argument("multi-column solves support FP64 CG and FP32 CG with FP64 replacement");
-    if (pre && (segments || o.mode != Solve_FP32_CG_FP64_RELIABLE || o.precond != Precond_None))
-        throw std::invalid_argument("a caller's preconditioner takes the FP32 mode, no segments, no other preconditioner");
+    if (pre && (segments || o.precond != Precond_None ||
+                (o.mode == Solve_FP64_CG ? !pre->f64 : !pre->f32)))
+        throw std::invalid_argument("a caller's preconditioner takes no segments, no other preconditioner and "
+                                    "a function for the mode's precision");
     op.upload_device();
     op.reserve_columns(segments ? 1 : ncols); // workspace of the batched apply, outside any capture
     BlockJacobiApply bja;
@@ -126,13 +128,16 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         r.ensure(tot);
         p.ensure(tot);
         q.ensure(tot);
-        if (M) z.ensure(tot);
+        if (M || pre) z.ensure(tot);
         push();
         hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, r.p, B, L);
         hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, L, partial.p);
         hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
         auto prec = [&](int slot) {
-            if (M) {
+            if (pre) {
+                pre->f64(r.p, z.p);
+                hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, z.p, L, partial.p);
+            } else if (M) {
                 applyM_dot(r.p, z.p);
             } else {
                 hipLaunchKernelGGL((mdot<double, double>), g, bl, 0, st, r.p, r.p, L, partial.p);
@@ -140,7 +145,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, slot, -1);
         };
         prec(0);
-        hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, p.p, M ? z.p : r.p, L);
+        hipLaunchKernelGGL((mcopy<double>), g, bl, 0, st, p.p, (M || pre) ? z.p : r.p, L);
         MHIP(hipGetLastError());
         auto iteration = [&](int c) {
             applyA_dot(p.p, q.p);
@@ -148,7 +153,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
             hipLaunchKernelGGL((mxr<double, double>), g, bl, 0, st, X, r.p, p.p, q.p, L, S.p, c, partial.p);
             hipLaunchKernelGGL(mfinal, dim3(ncols), bl, 0, st, partial.p, S.p, 4, -1);
             prec(1 - c);
-            hipLaunchKernelGGL((mp<double>), g, bl, 0, st, p.p, M ? z.p : r.p, L, S.p, 1 - c, c);
+            hipLaunchKernelGGL((mp<double>), g, bl, 0, st, p.p, (M || pre) ? z.p : r.p, L, S.p, 1 - c, c);
         };
         // the check_every iterations between host checks replayed from one captured graph (an even
         // count keeps the rho slots in phase)
@@ -208,7 +213,7 @@ MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols
         // rho into slot `slot` of the columns flagged in slot msk (all if msk < 0)
         auto prec = [&](int slot, int msk) {
             if (pre) {
-                (*pre)(r32.p, z.p);
+                pre->f32(r32.p, z.p);
                 hipLaunchKernelGGL((mdot<float, float>), g, bl, 0, st, r32.p, z.p, L, partial.p);
             } else if (M) {
                 applyM_dot(r32.p, z.p);

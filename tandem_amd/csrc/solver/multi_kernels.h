@@ -202,14 +202,18 @@ __global__ __launch_bounds__(kB) void mpflex(T* __restrict__ p, T const* __restr
 } // namespace
 
 class Operator;
-// A caller's preconditioner of the FP32 recursion: z = M^-1 r on the solver's columns.
-using PrecondFn32 = std::function<void(float const* r, float* z)>;
+// A caller's preconditioner, z = M^-1 r on the solver's columns: f32 for the FP32 recursion of the
+// FP32 mode, f64 for the FP64 mode.
+struct CallerPrecond {
+    std::function<void(float const* r, float* z)> f32;
+    std::function<void(double const* r, double* z)> f64;
+};
 // ncols lockstep CG recursions over the columns laid out by L (tot entries per vector): one batched
 // apply per iteration, or with segments one apply of the whole operator.  Buffers from w when given
-// (grown as needed), else allocated for the call.  pre (FP32 mode without segments, in place of
-// o.precond) is applied to the FP32 residuals.
+// (grown as needed), else allocated for the call.  pre (no segments, in place of o.precond) is
+// applied to the residuals of the mode's recursion.
 MultiSolveResult lockstep_cg(Operator& op, double const* B, double* X, int ncols, Cols L, std::size_t tot,
                              bool segments, SolveOptions const& o, MultiWork* w = nullptr,
-                             PrecondFn32 const* pre = nullptr);
+                             CallerPrecond const* pre = nullptr);
 
 } // namespace tdg

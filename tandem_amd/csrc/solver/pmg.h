@@ -30,7 +30,7 @@ struct PMGOptions {
     int coarse_assembled = 1;   // coarsest level on its assembled block rows where it has the register
                                 //   kernel (P1 / P2)
     int coarse_amg = 1;         // coarse CG preconditioned by smoothed-aggregation AMG on those block
-                                //   rows (elasticity, FP32 coarse solve), else block Jacobi
+                                //   rows (elasticity), else block Jacobi
 };
 
 struct PMGLevel;

@@ -297,7 +297,7 @@ void PMG::setup() {
         // preconditioner leaves the FP32 recursion in rounding noise between two residual checks)
         const bool amg_shape = coarse.kind() == Kind_Elasticity &&
                                ((coarse.dim() == 2 && coarse.degree() <= 2) || (coarse.dim() == 3 && coarse.degree() == 1));
-        if (o_.coarse_amg && o_.coarse_fp32 && amg_shape && int64_t(coarse.local_size()) > AMGOptions().coarse_dofs)
+        if (o_.coarse_amg && amg_shape && int64_t(coarse.local_size()) > AMGOptions().coarse_dofs)
             amg_ = std::make_unique<AMG>(coarse);
     }
     for (auto* a : ops_) {
@@ -372,7 +372,9 @@ template <class T> void PMG::cycle(int l, T const* b, T* x, int ncols, int64_t l
             b64 = v.r.p;
             x64 = v.t.p;
         }
-        PrecondFn32 amg = [&](float const* rr, float* zz) { amg_->vcycle(rr, zz, ncols, ld); };
+        CallerPrecond amg;
+        amg.f32 = [&](float const* rr, float* zz) { amg_->vcycle(rr, zz, ncols, ld); };
+        amg.f64 = [&](double const* rr, double* zz) { amg_->vcycle(rr, zz, ncols, ld); };
         auto r = lockstep_cg(*ops_[l], b64, x64, ncols, C, std::size_t(ncols) * std::size_t(ld), false, co,
                              coarse_work_.get(), amg_ ? &amg : nullptr);
         if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((vconvert<T, double>), g, bl, 0, st, x, x64, C.n, ld, ld);

@@ -140,21 +140,22 @@ def test_pmg_solve_matches_block_jacobi_cg(cuda, dim, degrees, n, fp32):
     assert r["iterations"] * 5 <= r2["iterations"]
 
 
-@pytest.mark.parametrize("dim,cells,degrees", [(2, 24, (4, 1)), (2, 12, (4, 2)), (3, 4, (2, 1))],
-                         ids=["D2-P4-P1", "D2-P4-P2", "D3-P2-P1"])
-def test_amg_coarse_preconditioner(cuda, dim, cells, degrees):
+@pytest.mark.parametrize("dim,cells,degrees,fp32", [(2, 24, (4, 1), 1), (2, 12, (4, 2), 1), (3, 4, (2, 1), 1),
+                                                   (2, 24, (4, 1), 0)],
+                         ids=["D2-P4-P1", "D2-P4-P2", "D3-P2-P1", "D2-P4-P1-fp64"])
+def test_amg_coarse_preconditioner(cuda, dim, cells, degrees, fp32):
     """The coarse CG preconditioned by the smoothed-aggregation AMG (the default for elasticity P1 / 2D
     P2 coarse levels solved in FP32; the reference's coarse preconditioner is GAMG) against block
     Jacobi: the same solution to the solver tolerance, and a fraction of the coarse iterations.  The
     three hierarchies cover the AMG's block shapes (element blocks of 6, 12 and 12 with 3, 3 and 6
-    rigid-body modes)."""
+    rigid-body modes); the last runs the V-cycle and the coarse CG (with its AMG) in FP64."""
     import torch
 
     mesh = T.Mesh.box(dim, cells, scheme=1 if dim == 3 else 0, bc=T.BC_DIRICHLET)
     B = None
     out = {}
     for amg in (0, 1):
-        ops, M = hierarchy(mesh, degrees, 7, coarse_amg=amg)
+        ops, M = hierarchy(mesh, degrees, 7, coarse_amg=amg, coarse_fp32=fp32, vcycle_fp32=fp32)
         if B is None:
             B = torch.from_numpy(np.random.default_rng(9).uniform(-1, 1, (4, ops[0].n))).to(cuda)
         X = torch.zeros_like(B)
