@@ -98,7 +98,9 @@ void spmv(Bsr const& A, std::vector<double> const& x, std::vector<double>& y) {
         }
 }
 
-// lambda_max(D^-1 A) by power iteration in the A inner product's equivalent: iterate v <- D^-1 A v
+// lambda_max(D^-1 A) by power iteration v <- D^-1 A v / |.| (the eigenvalues of D^-1 A are real and
+// positive for SPD A and D); 30 steps from a seeded random vector, raised by 5% for the Chebyshev
+// interval's upper end since the iterates approach it from below
 double power_rho(Bsr const& A, std::vector<double> const& Dinv) {
     const int bs = A.br;
     const std::size_t N = std::size_t(A.n) * bs;
@@ -125,7 +127,7 @@ double power_rho(Bsr const& A, std::vector<double> const& Dinv) {
         lam = num;
         v.swap(z);
     }
-    return lam * 1.05; // power iteration approaches from below
+    return lam * 1.05;
 }
 
 // Aggregates: a node whose neighbours are all free starts an aggregate with them; free nodes then
